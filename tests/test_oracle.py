@@ -1,0 +1,319 @@
+"""Pins for the CPU FP64 oracle (oracle/oracle.c).
+
+Three kinds of evidence, chosen so that a dropped term, a wrong sign or index,
+or a transposed operand fails at least one of them:
+  1. the reference's own golden fixtures (tests/<system>/IPD_*, ENE_*,
+     summary_* of the reference, copied under tests/golden/reference/),
+     both at the reference's tolerances and through tighter invariants
+     (residual of the reference's dipoles inside the oracle's system; energy
+     of the reference's dipoles with the oracle's field);
+  2. closed forms and mathematical identities (single site, field = -grad V
+     for the damped kernels, dT = dE_q/dr, symmetric / traceless T);
+  3. brute force on small inputs (dense numpy solve, an independent Python
+     statement of the screening rules).
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import mmp_io
+import oracle_api as oa
+
+FIXTURES = [
+    ("N-methylacetamide", "AMOEBA", ""), ("N-methylacetamide", "WANG_AL", ""),
+    ("N-methylacetamide", "AMOEBA", "_CUT"), ("N-methylacetamide", "WANG_AL", "_CUT"),
+    ("1crn", "AMOEBA", ""), ("1crn", "WANG_AL", ""),
+]
+
+
+def _system(name, ff, cut=""):
+    return oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path(name, f"input_{ff}{cut}.mmp")))
+
+
+def _ext(o, name, k):
+    if k == 0:
+        return np.zeros((o.npol, 3))
+    return mmp_io.read_ef(mmp_io.fixture_path(name, "EF_1.txt"))[o.polar_mm]
+
+
+@pytest.mark.parametrize("name,ff,cut", FIXTURES)
+@pytest.mark.parametrize("k", [0, 1])
+def test_reference_fixture_tolerances(name, ff, cut, k):
+    """Reproduces the reference's own ipd/energy ctest rows (tests/test_list:3-40):
+    IPD rtol 1e-6 atol 1e-5, energy rtol 1e-6 (generate_cmake_tests.py:7-8, 96-99)."""
+    o = _system(name, ff, cut)
+    o.set_external_field(_ext(o, name, k))
+    ep = o.get_polelec_energy()
+    ref = mmp_io.read_ipd(mmp_io.fixture_path(name, f"IPD_{k}_{ff}{cut}.ref"))
+    assert np.allclose(o.ipd.reshape(-1, 3), ref, rtol=1e-6, atol=1e-5)
+    if not cut:
+        eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_{k}_{ff}.ref"))["EP"]
+        assert abs(ep - eref) <= 1e-6 * abs(eref) + 1e-6
+
+
+@pytest.mark.parametrize("name,ff", [("N-methylacetamide", "AMOEBA"), ("N-methylacetamide", "WANG_AL"),
+                                     ("1crn", "AMOEBA"), ("1crn", "WANG_AL"),
+                                     ("1ubq", "AMOEBA"), ("1ubq", "WANG_AL")])
+@pytest.mark.parametrize("k", [0, 1])
+def test_reference_dipoles_solve_oracle_system(name, ff, k):
+    """The reference's converged dipoles must satisfy the oracle's linear system to
+    the reference's own convergence criterion sqrt(r.alpha r / n) < 1e-8
+    (mod_solvers.F90:188-199), and with the oracle's E_M2D they must give the
+    reference's polarization energy to ~1e-12.  This pins T, both RHS sets and
+    the energy formula far below the fixture tolerances.  1UBQ ran with FMM
+    (l=18) in the reference; the direct oracle still reproduces it."""
+    o = _system(name, ff)
+    E = o.m2d()
+    ext = _ext(o, name, k)
+    ipd_ref = mmp_io.read_ipd(mmp_io.fixture_path(name, f"IPD_{k}_{ff}.ref")).reshape(o.n_ipd, o.npol, 3)
+    alpha = o.prep["pol"][o.polar_mm][:, None]
+    for s in range(o.n_ipd):
+        r = E[s] + ext - o.tmatvec(ipd_ref[s])
+        assert np.sqrt((r * r * alpha).sum() / r.size) < 1e-8
+    eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_{k}_{ff}.ref"))["EP"]
+    assert abs(o.energy(ipd_ref) - eref) <= 1e-11 * abs(eref)
+
+
+@pytest.mark.parametrize("name,ff", [("N-methylacetamide", "AMOEBA"), ("N-methylacetamide", "WANG_AL"),
+                                     ("1crn", "AMOEBA"), ("1crn", "WANG_AL")])
+def test_prepared_state_matches_summary(name, ff):
+    """summary_*.ref (mod_mmpol.F90:375-484): units, rotated multipoles, thole,
+    1-2..1-5 neighbours and polarization-group neighbour sets."""
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path(name, f"input_{ff}.mmp"))
+    summ = mmp_io.read_summary(mmp_io.fixture_path(name, f"summary_{ff}.ref"))
+    o = oa.OracleSystem(prep)
+    assert np.abs(prep["cmm"] - summ["cmm"]).max() < 1e-8
+    assert np.abs(prep["q"] - summ["q"]).max() < 1e-8
+    assert np.abs(prep["pol"][o.polar_mm] - summ["pol"][:, 0]).max() < 1e-8
+    assert np.abs(o.thole() - summ["thole"][:, 0]).max() < 1e-8
+    assert np.array_equal(o.polar_mm + 1, summ["polar_mm"])
+    for lvl in range(4):
+        ri, ci, _ = o.list_csr(3 + lvl)
+        for atom, levels in summ["conn"].items():
+            if lvl + 2 in levels:  # the Wang summaries print 1-2..1-4 only
+                assert sorted(ci[ri[atom]:ri[atom + 1]]) == levels[lvl + 2]
+    if ff == "AMOEBA":
+        members = {}
+        grp = prep["grp"]
+        for g in range(grp.max() + 1):
+            members[g] = sorted(np.nonzero(grp == g)[0])
+        for lvl in range(4):
+            ri, ci, _ = o.list_csr(7 + lvl)
+            for atom, levels in summ["pgconn"].items():
+                g = grp[atom]
+                got = sorted(m for gg in ci[ri[g]:ri[g + 1]] for m in members[gg])
+                assert got == levels[lvl + 1]
+
+
+# ----------------------------------------------------------------- closed forms
+def _tiny(amoeba, coords, pol, q=None, bonds=(), groups=None, scales=None, thole_scale=None):
+    n = len(coords)
+    ld = 10 if amoeba else 1
+    adj = [[] for _ in range(n)]
+    for a, b in bonds:
+        adj[a].append(b)
+        adj[b].append(a)
+    ri = np.zeros(n + 1, dtype=np.int32)
+    for i in range(n):
+        ri[i + 1] = ri[i] + len(adj[i])
+    ci = np.array([v for a in adj for v in sorted(a)], dtype=np.int32)
+    prep = dict(nmm=n, amoeba=amoeba, cmm=np.array(coords, dtype=float),
+                q=np.zeros((n, ld)) if q is None else np.array(q, dtype=float),
+                pol=np.array(pol, dtype=float), adj_ri=ri, adj_ci=ci,
+                grp=np.arange(n, dtype=np.int32) if groups is None else np.array(groups, dtype=np.int32),
+                td=np.full(n, 0.39),
+                scales=(mmp_io.SCALES_AMOEBA if amoeba else mmp_io.SCALES_WANG_AL) if scales is None else scales,
+                thole_scale=0.0 if amoeba else (mmp_io.THOLE_SCALE_WANG_AL if thole_scale is None else thole_scale))
+    return oa.OracleSystem(prep)
+
+
+@pytest.mark.parametrize("solver", [oa.SOLVER_CG, oa.SOLVER_DIIS, oa.SOLVER_INVERSION])
+def test_single_site_mu_equals_alpha_E(solver):
+    o = _tiny(True, [[0.0, 0.0, 0.0], [40.0, 0.0, 0.0]], [7.5, 0.0])
+    E = np.array([[[0.01, -0.02, 0.03]], [[0.005, 0.0, -0.01]]])
+    ipd = o.polarization(E, solver=solver, use_guess=False)
+    assert np.allclose(ipd, 7.5 * E, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("amoeba", [True, False])
+@pytest.mark.parametrize("r", [1.2, 2.5, 4.0, 9.0])
+def test_damped_fields_are_gradients(amoeba, r):
+    """E_mu = -grad V_mu and E_Theta = -grad V_Theta with the damped kernels;
+    dE_q/dr = T.  These identities fix lambda5 from lambda3 and lambda7 from
+    lambda5 (Thole 1981 / Ren-Ponder 2003 / Wang 2011 forms) without restating them."""
+    o = _tiny(amoeba, [[0.1, -0.2, 0.3], [0.1 + r * 0.6, -0.2 + r * 0.48, 0.3 + r * 0.64]], [9.0, 6.0])
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    for part in ("mu", "quad"):
+        qs = np.zeros(10)
+        if part == "mu":
+            qs[1:4] = rng.normal(size=3)
+        else:
+            Q = rng.normal(size=(3, 3))
+            Q = Q + Q.T
+            Q -= np.eye(3) * np.trace(Q) / 3
+            qs[4:10] = [Q[0, 0], Q[0, 1], Q[1, 1], Q[0, 2], Q[1, 2], Q[2, 2]]
+        if not amoeba and part == "quad":
+            continue
+
+        def VE(delta):
+            p2 = dict(o.prep)
+            c = o.prep["cmm"].copy()
+            c[1] += delta
+            p2["cmm"] = c
+            o2 = oa.OracleSystem(p2)
+            kk, dd = o2.damped_kernel(0, 1, 4 if amoeba else 3)
+            kk = np.concatenate([kk, np.zeros(5 - len(kk))])
+            E = np.zeros(3)
+            V = oa.lib().oracle_multipole_potential_field(
+                qs.ctypes.data_as(oa._dp), 10, dd.ctypes.data_as(oa._dp),
+                kk.ctypes.data_as(oa._dp), E.ctypes.data_as(oa._dp))
+            return V, E
+        V0, E0 = VE(np.zeros(3))
+        grad = np.array([(VE(h * e)[0] - VE(-h * e)[0]) / (2 * h) for e in np.eye(3)])
+        # V includes the (undamped) charge term only when q != 0; here q = 0
+        assert np.allclose(E0, -grad, rtol=1e-6, atol=1e-9 * max(1.0, np.abs(E0).max()))
+    # charge-field Jacobian == dipole tensor k2 I - 3 k3 d d^T
+    def Eq(delta):
+        p2 = dict(o.prep)
+        c = o.prep["cmm"].copy()
+        c[1] += delta
+        p2["cmm"] = c
+        o2 = oa.OracleSystem(p2)
+        kk, dd = o2.damped_kernel(0, 1, 2)
+        return kk[1] * dd, kk, dd
+    _, k, d = Eq(np.zeros(3))
+    J = np.array([(Eq(h * e)[0] - Eq(-h * e)[0]) / (2 * h) for e in np.eye(3)]).T
+    T = k[1] * np.eye(3) - 3 * k[2] * np.outer(d, d)
+    assert np.allclose(J, T, rtol=1e-6, atol=1e-9)
+
+
+def test_undamped_tensor_is_traceless_and_T_symmetric():
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-12, 12, size=(20, 3))
+    o = _tiny(True, pts, rng.uniform(2, 15, size=20))
+    T = o.tmat()
+    assert np.array_equal(T, T.T)
+    # far pair: damping is off (a u^3 >= 50), trace must vanish
+    far = max(itertools.combinations(range(20), 2), key=lambda p: np.linalg.norm(pts[p[0]] - pts[p[1]]))
+    i, j = far
+    blk = T[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+    assert abs(np.trace(blk)) < 1e-14 * np.abs(blk).max()
+
+
+def _py_rules(prep, npol_idx):
+    """Independent statement of screening_rules (mod_electrostatics.F90:3338-3487)
+    from breadth-first distances, used to cross-check the oracle's lists."""
+    n = prep["nmm"]
+    adj = [list(prep["adj_ci"][prep["adj_ri"][i]:prep["adj_ri"][i + 1]]) for i in range(n)]
+    def bfs(adjl, s, maxd):
+        d = {s: 0}
+        front = [s]
+        for lvl in range(1, maxd + 1):
+            nxt = []
+            for u in front:
+                for v in adjl[u]:
+                    if v not in d:
+                        d[v] = lvl
+                        nxt.append(v)
+            front = nxt
+        return d
+    grp = prep["grp"]
+    ng = grp.max() + 1
+    gadj = [set() for _ in range(ng)]
+    for i in range(n):
+        for j in adj[i]:
+            if grp[i] != grp[j]:
+                gadj[grp[i]].add(grp[j])
+    gadj = [sorted(s) for s in gadj]
+    m, p, d, u, pi = prep["scales"]
+    amoeba = prep["amoeba"]
+    bd = [bfs(adj, i, 4) for i in range(n)]
+    gd = [bfs(gadj, g, 3) for g in range(ng)]
+    def bond(i, j):
+        x = bd[i].get(j, 99)
+        return x if 1 <= x <= 4 else None
+    def gdist(i, j):
+        x = gd[grp[i]].get(grp[j], 99)
+        return x if x <= 3 else None
+    def sp(i, j):  # static i -> pol site j (MM indices), polarization field
+        b = bond(i, j)
+        if b is None:
+            return 1.0
+        if amoeba and grp[i] == grp[j]:
+            return pi[b - 1]
+        return p[b - 1]
+    def sd(i, j):
+        g = gdist(i, j)
+        return 1.0 if g is None else d[g]
+    def su_rule(i, j):
+        if amoeba:
+            g = gdist(i, j)
+            return 1.0 if g is None else u[g]
+        b = bond(i, j)
+        return 1.0 if b is None else u[b - 1]
+    def su_list(i, j):  # P-P lists only scan topological neighbours
+        return su_rule(i, j) if bond(i, j) is not None else 1.0
+    return sp, sd, su_rule, su_list
+
+
+@pytest.mark.parametrize("amoeba", [True, False])
+def test_screening_lists_brute_force(amoeba):
+    """A chain of 14 sites in 4 polarization groups, non-default scale factors
+    so that every level gives a distinct value."""
+    n = 14
+    coords = [[1.9 * i, 0.3 * (i % 3), 0.2 * (i % 2)] for i in range(n)]
+    bonds = [(i, i + 1) for i in range(n - 1)] + [(3, 12)]
+    groups = [0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 3, 3, 3, 3]
+    scales = np.array([[0.0, 0.1, 0.4, 0.8], [0.0, 0.2, 0.45, 0.85], [0.05, 0.3, 0.6, 0.9],
+                       [0.15, 0.35, 0.65, 0.95], [0.01, 0.25, 0.55, 0.75]])
+    pol = [5.0] * n
+    pol[6] = 0.0  # one non-polarizable site
+    o = _tiny(amoeba, coords, pol, bonds=bonds, groups=groups, scales=scales, thole_scale=2.0)
+    sp, sd, su_rule, su_list = _py_rules(o.prep, None)
+    pm = list(o.polar_mm)
+    ri, ci, val = o.list_csr(0)
+    got = {(i, pm[ci[k]]): val[k] for i in range(n) for k in range(ri[i], ri[i + 1])}
+    want = {(i, j): sp(i, j) for i in range(n) for j in pm if abs(sp(i, j) - 1) > 1e-12}
+    assert got.keys() == want.keys() and all(abs(got[k] - want[k]) < 1e-15 for k in got)
+    if amoeba:
+        ri, ci, val = o.list_csr(1)
+        got = {(i, pm[ci[k]]): val[k] for i in range(n) for k in range(ri[i], ri[i + 1])}
+        want = {(i, j): sd(i, j) for i in range(n) for j in pm if abs(sd(i, j) - 1) > 1e-12}
+        assert got.keys() == want.keys() and all(abs(got[k] - want[k]) < 1e-15 for k in got)
+    ri, ci, val = o.list_csr(2)
+    got = {(pm[a], pm[ci[k]]): val[k] for a in range(len(pm)) for k in range(ri[a], ri[a + 1])}
+    want = {(i, j): su_list(i, j) for i in pm for j in pm if i != j and abs(su_list(i, j) - 1) > 1e-12}
+    assert got.keys() == want.keys() and all(abs(got[k] - want[k]) < 1e-15 for k in got)
+    # the dense T (create_tmat) applies the rule to every pair
+    for a, b in itertools.product(range(len(pm)), repeat=2):
+        if a != b:
+            assert abs(o.screening_rule(a, "P", b, "P", "-") - su_rule(pm[a], pm[b])) < 1e-15
+
+
+@pytest.mark.parametrize("amoeba", [True, False])
+def test_otf_matvec_equals_dense(amoeba):
+    """DIRECT (list-based) and dense T agree for the reference scale sets."""
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path("1crn", "input_AMOEBA.mmp" if amoeba else "input_WANG_AL.mmp"))
+    o = oa.OracleSystem(prep)
+    x = np.random.default_rng(1).normal(size=(o.npol, 3))
+    y1 = o.tmatvec(x)
+    y2 = (o.tmat() @ x.ravel()).reshape(-1, 3)
+    assert np.allclose(y1, y2, rtol=0, atol=1e-13 * np.abs(y2).max())
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_solvers_agree_with_dense_solve(ff):
+    """CG (direct and incore), Jacobi/DIIS and inversion all reach numpy's dense
+    solution; inversion to rounding, the iterative ones to their stopping rule."""
+    o = _system("1crn", ff)
+    E = o.m2d()
+    T = o.tmat()
+    exact = np.stack([np.linalg.solve(T, E[s].ravel()).reshape(-1, 3) for s in range(o.n_ipd)])
+    for solver, matv, tol in [(oa.SOLVER_CG, oa.MATV_DIRECT, 5e-7), (oa.SOLVER_CG, oa.MATV_INCORE, 5e-7),
+                              (oa.SOLVER_DIIS, oa.MATV_DIRECT, 5e-7), (oa.SOLVER_DIIS, oa.MATV_INCORE, 5e-7),
+                              (oa.SOLVER_INVERSION, 0, 1e-12)]:
+        o.ipd[:] = 0
+        ipd = o.polarization(E, solver=solver, matv=matv, use_guess=False)
+        assert np.abs(ipd - exact).max() < tol, (solver, matv)
