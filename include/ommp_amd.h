@@ -1,0 +1,112 @@
+/*
+ * ommp_amd.h -- extensions of the OpenMMPol C interface for the MI355X build.
+ *
+ * None of these has a counterpart in the reference's C interface; each says
+ * which reference routine or behaviour it exposes or extends.  They use the
+ * same opaque handle as openmmpol.h and only plain C types.
+ */
+#ifndef _OMMP_AMD_H
+#define _OMMP_AMD_H
+
+#include <stdbool.h>
+#include <stdint.h>
+#include "openmmpol.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision of the T matvec inside the solvers.
+   FP64  : every step in FP64, reference algorithm and iterate sequence.
+   MIXED : FP32 T matvec (pair arithmetic) inside an FP64 iterative-refinement
+           loop; the residual r = E - T mu, CG dot products and step lengths,
+           the dipole accumulation and the energy sum are FP64.  D and P sets
+           are solved together (two right-hand sides per matvec). */
+#define OMMP_AMD_PREC_FP64 0
+#define OMMP_AMD_PREC_MIXED 1
+
+/* Solver status codes returned by ommp_amd_get_last_status. */
+#define OMMP_AMD_STATUS_CONVERGED 0   /* residual rms below tolerance                 */
+#define OMMP_AMD_STATUS_SMALL_STEP 1  /* |p.Tp| < eps_rp quick return (CG)            */
+#define OMMP_AMD_STATUS_NOT_CONVERGED 2
+
+/* Build a system from arrays instead of a file.  This is what the reference
+   does in memory between mmpol_init and mmpol_prepare (mod_mmpol.F90:49-237)
+   when a front-end has already parsed its input.
+     ff_type   OMMP_FF_AMOEBA or OMMP_FF_AMBER
+     ff_rules  for non-AMOEBA: 0 = Wang AL, 1 = Wang DL (scale sets and Thole
+               scale of mod_constants.F90:37-58); ignored for AMOEBA
+     cmm       [mm_atoms][3] coordinates in bohr
+     q         [mm_atoms][ld_cart] multipoles in the .mmp convention
+               (AMOEBA quadrupoles are divided by 3 and rotated here)
+     pol       [mm_atoms] polarizability in bohr^3 (|pol| <= eps: not polarizable)
+     thole_a   [mm_atoms] AMOEBA Thole damping factor, or NULL for 0.39
+     adj_ri, adj_ci  1-2 connectivity as a 0-based CSR (adj_ri has mm_atoms+1 entries)
+     pgroup    [mm_atoms] 0-based polarization group (AMOEBA), or NULL
+     frames    [mm_atoms][4] (mol_frame, iz, ix, iy; 1-based atom indices, 0 = none),
+               or NULL for no rotation (all frames 0) */
+OMMP_SYSTEM_PRT ommp_amd_init_arrays(int32_t ff_type, int32_t ff_rules, int32_t mm_atoms,
+                                     const double *cmm, const double *q, const double *pol,
+                                     const double *thole_a, const int32_t *adj_ri,
+                                     const int32_t *adj_ci, const int32_t *pgroup,
+                                     const int32_t *frames);
+
+/* set_screening_parameters (mod_electrostatics.F90:440-467) followed by a
+   rebuild of the screening lists; intra may be NULL for non-AMOEBA. */
+void ommp_amd_set_screening_parameters(OMMP_SYSTEM_PRT s, const double *m, const double *p,
+                                       const double *d, const double *u, const double *intra);
+
+/* Solver tolerance and iteration limit (the reference hard-codes 1e-8 and 200,
+   mod_solvers.F90:30-32; those remain the defaults). */
+void ommp_amd_set_solver_tolerance(OMMP_SYSTEM_PRT s, double tol, int32_t max_iter);
+void ommp_amd_set_precision(OMMP_SYSTEM_PRT s, int32_t mode);
+int32_t ommp_amd_get_precision(OMMP_SYSTEM_PRT s);
+
+/* Iterations of the last solve: out[0..n_ipd-1] matvec-driving iterations per
+   set (FP64), and for MIXED out[2] = outer FP64 steps, out[3] = total inner
+   FP32 iterations.  out must hold 4 entries. */
+void ommp_amd_get_last_iterations(OMMP_SYSTEM_PRT s, int32_t *out);
+int32_t ommp_amd_get_last_status(OMMP_SYSTEM_PRT s);
+
+/* Cached static field E_M2D (prepare_polelec, mod_electrostatics.F90:1487-1596):
+   returns [n_ipd][pol_atoms][3] on the host (computed on the device if needed). */
+double *ommp_amd_get_e_m2d(OMMP_SYSTEM_PRT s);
+
+/* Device-resident variants for callers that keep fields and dipoles in HBM.
+   d_ext is a device pointer [pol_atoms][3]; semantics as ommp_set_external_field.
+   The host copy of the dipoles is refreshed only when ommp_get_ipd is called. */
+void ommp_amd_set_external_field_device(OMMP_SYSTEM_PRT s, const double *d_ext, int32_t solver);
+/* Device pointer of the dipoles [n_ipd][pol_atoms][3] (valid until terminate). */
+const double *ommp_amd_device_ipd(OMMP_SYSTEM_PRT s);
+/* Stream used by the library for this system (a hipStream_t; NULL = default). */
+void ommp_amd_set_stream(OMMP_SYSTEM_PRT s, void *hip_stream);
+/* Drop the warm-start guess and the cached static field, as the reference does
+   after a coordinate update (mod_mmpol.F90:356-361), without changing coordinates. */
+void ommp_amd_reset_guess(OMMP_SYSTEM_PRT s, bool also_static_field);
+
+/* Host-side introspection of the prepared state (no device work):
+   which = 0 list_S_P_P, 1 list_S_P_D, 2 list_P_P (rows = source, reference layout),
+   3..6 exact bond distance 1-2..1-5, 7..10 polarization-group distance 0..3.
+   Returns nnz; when ri/ci/val are non-NULL they receive rows+1, nnz, nnz entries. */
+int32_t ommp_amd_get_list(OMMP_SYSTEM_PRT s, int32_t which, int32_t *ri, int32_t *ci, double *val);
+/* Thole factors per MM site (thole_init, mod_electrostatics.F90:866-897). */
+double *ommp_amd_get_thole(OMMP_SYSTEM_PRT s);
+/* Polarizabilities per polarizable site (bohr^3). */
+double *ommp_amd_get_pol(OMMP_SYSTEM_PRT s);
+
+/* One T matvec y = T x on the device for ncols (1 or 2) host vectors
+   x, y [ncols][pol_atoms][3] (TMatVec_otf / TMatVec_incore with the diagonal,
+   mod_polarization.F90:437-504).  kind: 0 on-the-fly FP64, 1 on-the-fly FP32,
+   2 packed in-core FP32, 3 packed in-core FP64.  repeat > 1 launches it that
+   many times back to back and returns the mean device time per launch (ms). */
+double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const double *x, double *y,
+                        int32_t repeat);
+
+/* Timing of the last solve on the device, in milliseconds: out[0] static field,
+   out[1] solver, out[2] energy/copy-back.  out must hold 3 entries. */
+void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+/*
+ * openmmpol.h -- drop-in C interface of the induced-dipole polarization path.
+ *
+ * Every entry point below has the name, signature, argument meaning, array
+ * layout and error behaviour of the reference function it replaces (cited as
+ * reference include/openmmpol.h:<line> and the Fortran routine behind it).
+ * Arrays are C row-major views of the reference's Fortran column-major
+ * storage: ipd is [n_ipd][pol_atoms][3], fields are [pol_atoms][3],
+ * coordinates [mm_atoms][3], multipoles [mm_atoms][ld_cart].  All lengths are
+ * in atomic units.  Errors print "[OMMP]   [stop] ..." and exit(1), as the
+ * reference's fatal_error (src/mod_io.F90:155-175).
+ *
+ * Pointers returned by the getters point into library-owned host memory that
+ * stays valid until ommp_terminate (or until ommp_update_coordinates for the
+ * coordinate-derived arrays), and is refreshed in place after every solve.
+ */
+#ifndef _OPENMMPOL
+#define _OPENMMPOL
+
+#include <stdbool.h>
+#include <stdint.h>
+#include "openmmpol_const.h"
+
+typedef void *OMMP_SYSTEM_PRT;
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference include/openmmpol.h:17; mod_c_interface.F90:216-231 -> mod_inputloader.F90:15-257 */
+OMMP_SYSTEM_PRT ommp_init_mmp(const char *filename);
+/* :19; mod_electrostatics.F90 set_def_solver (fatal on unknown id) */
+void ommp_set_default_solver(OMMP_SYSTEM_PRT s, int32_t solver);
+/* :20; mod_electrostatics.F90 set_def_matv (fatal on unknown id) */
+void ommp_set_default_matv(OMMP_SYSTEM_PRT s, int32_t matv);
+/* :24; mod_interface.F90:161-170 */
+void ommp_terminate(OMMP_SYSTEM_PRT s);
+/* :28; mod_io.F90:92-103 (clamped to 0..3) */
+void ommp_set_verbose(int32_t level);
+/* :29-30; mod_io.F90 set_iof_mmpol / close_output */
+void ommp_set_outputfile(const char *filename);
+void ommp_close_outputfile(void);
+/* :31-32; mod_io.F90:105-153 and :155-175 */
+void ommp_message(const char *msg, int32_t level, const char *prefix);
+void ommp_fatal(const char *msg);
+/* :33-34; mod_profiling.F90:20-71 */
+void ommp_time_pull(const char *label);
+void ommp_time_push(void);
+
+/* :38; mod_interface.F90:343-370 -> energy_MM_pol (mod_electrostatics.F90:943-976).
+   Solves with the default solver first if no dipoles are available. */
+double ommp_get_polelec_energy(OMMP_SYSTEM_PRT s);
+/* :40; mod_c_interface.F90:289-305 -> mod_interface.F90:184-223.  ext is
+   [pol_atoms][3]; the MM field is added and both D and P sets are solved with
+   the default matrix-vector method (the matv argument is ignored, as in the
+   reference). */
+void ommp_set_external_field(OMMP_SYSTEM_PRT s, const double *ext, int32_t solver, int32_t matv);
+/* :41; mod_interface.F90:224-232: field = ext only, solves the D set only. */
+void ommp_set_external_field_nomm(OMMP_SYSTEM_PRT s, const double *ext, int32_t solver, int32_t matv);
+
+/* :65; mod_c_interface.F90 C_ommp_ff_is_amoeba */
+bool ommp_ff_is_amoeba(OMMP_SYSTEM_PRT s);
+/* :68-71 */
+int32_t ommp_get_n_ipd(OMMP_SYSTEM_PRT s);
+int32_t ommp_get_ld_cart(OMMP_SYSTEM_PRT s);
+int32_t ommp_get_mm_atoms(OMMP_SYSTEM_PRT s);
+int32_t ommp_get_pol_atoms(OMMP_SYSTEM_PRT s);
+/* :73 [mm_atoms][3] */
+double *ommp_get_cmm(OMMP_SYSTEM_PRT s);
+/* :76 [pol_atoms][3] */
+double *ommp_get_cpol(OMMP_SYSTEM_PRT s);
+/* :77 [mm_atoms][ld_cart], lab-frame (rotated) multipoles */
+double *ommp_get_q(OMMP_SYSTEM_PRT s);
+/* :78; mod_c_interface.F90:999-1008: [n_ipd][pol_atoms][3] */
+double *ommp_get_ipd(OMMP_SYSTEM_PRT s);
+/* :79; mod_c_interface.F90:1010-1026: 0-based MM index of each polarizable site */
+int32_t *ommp_get_polar_mm(OMMP_SYSTEM_PRT s);
+
+/* :86; mod_mmpol.F90:325-372: new coordinates [mm_atoms][3]; clears the cached
+   static field, the dipoles and the warm-start guess, re-rotates multipoles. */
+void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c);
+
+/* :153-155.  This library computes every interaction directly (no fast
+   multipoles): enable only records the request with a warning, use returns false. */
+void ommp_enable_fmm(OMMP_SYSTEM_PRT s);
+void ommp_disable_fmm(OMMP_SYSTEM_PRT s);
+bool ommp_use_fmm(OMMP_SYSTEM_PRT s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

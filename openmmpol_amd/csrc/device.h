@@ -1,0 +1,118 @@
+// HBM layout and kernel launchers of the MI355X induced-dipole path.
+//
+// Layout (per system, all resident in HBM for the lifetime of the system):
+//   pol sites   ps64[npol]  double4 {x, y, z, 1/thole}    ps32: float4 copy
+//               pa64[npol]  Thole damping a                pa32: float copy
+//               alpha[npol] polarizability (FP64)
+//   MM sources  ms64[nmm]   double4 {x, y, z, 1/thole} (1/thole = +inf when not
+//               polarizable: the damping test then fails and the kernel is the
+//               bare Coulomb one, as in the reference when thole_i*thole_j < eps)
+//               ma64[nmm]   damping a (+inf when not polarizable, so that
+//               min(a_i, a_j) picks the polarizable end)
+//               mq[nmm][ld] lab-frame multipoles
+//   correction lists, rows = polarizable target (CSR, 0-based):
+//               sp_*: sources whose D or P scale differs from 1, f = 1 - s
+//               pp_*: P-P pairs of the on-the-fly matvec, f = 1 - s
+//   vectors     [column][3*npol] FP64, column = D (0) / P (1)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ommp {
+
+constexpr int kBlock = 256;      // threads per block for every kernel here
+constexpr int kMaxCols = 2;      // D and P sets
+constexpr double kEpsRpDev = 2.220446049250313080847e-16 * 100.0;
+
+struct CgState {
+    double g[2];        // r.z, double-buffered by iteration parity
+    double rms;
+    double last_beta;
+    double tol;
+    double eps;         // |p.Tp| below this ends the solve (reference: eps_rp)
+    int done;
+    int status;         // 0 converged, 1 small step (|p.Tp| < eps), 2 running/not converged
+    int iters;
+    int pad;
+};
+
+struct CsrDev {
+    int *ri = nullptr;
+    int *ci = nullptr;
+    double *f = nullptr;   // one value per entry (pp lists)
+    double2 *f2 = nullptr; // two values per entry (sp list: x = D set, y = P set)
+    int nnz = 0;
+};
+
+// ----------------------------------------------------------------- launchers
+// Static field of the multipoles at the polarizable sites (E_M2D), FP64.
+void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double4 *ms64,
+                    const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
+                    const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d);
+
+// T matvec on the fly: y[m] = T x[m] (dodiag) or (T - diag) x[m], m < M.
+// prec 64 or 32 selects the pair arithmetic; corrections and the final sum are FP64.
+// If dotpart != nullptr, per-block partial sums of y.x go to dotpart[m * nblk + b]
+// where nblk = ceil(npol / kBlock).
+void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, const double4 *ps64,
+                    const double *pa64, const float4 *ps32, const float *pa32, const double *alpha,
+                    const CsrDev &pp, const double *x, long xstride, double *y, long ystride,
+                    bool dodiag, double *dotpart, void *part, int nsplit, const int *skip_flag);
+int tmv_split(int npol);
+size_t tmv_part_bytes(int npol, int M, int prec, int nsplit);
+int vec_blocks(long n);     // blocks used by the element-wise kernels
+int site_blocks(int npol);  // blocks used by the per-site kernels
+
+// Packed symmetric in-core T (upper tile pairs, 64x64 sites per tile).
+size_t tpack_bytes(int npol, int prec);
+void launch_tpack_build(hipStream_t st, int prec, bool amoeba, int npol, const double4 *ps64,
+                        const double *pa64, void *tpack, const int *pd_ri, const int *pd_ci,
+                        const double *pd_s);
+void launch_tpack_mv(hipStream_t st, int prec, int npol, int M, const void *tpack, const double *alpha,
+                     const double *x, long xstride, double *y, long ystride, bool dodiag,
+                     double *dotpart, void *part, const int *skip_flag);
+size_t tpack_part_bytes(int npol, int M);
+
+// CG pieces (columns m < M, vectors [m * n + k]).
+void launch_norm2_guess(hipStream_t st, long n, int M, double *x, const double *alpha, double *red);
+void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double *h, double *x,
+                    double *r, double *z, double *p, const double *alpha, double *red, CgState *cs,
+                    bool zero_guess, const double *tol_eps /* [tol0, tol1, eps0, eps1] */);
+void launch_cg_update(hipStream_t st, long n, int M, int it, const double *h, double *x, double *r,
+                      double *z, const double *p, const double *alpha, const double *red_hp,
+                      int nred_hp, double *red_rz, CgState *cs);
+void launch_cg_direction(hipStream_t st, long n, int M, int it, const double *z, double *p,
+                         const double *red_rz, CgState *cs, int *all_done);
+
+// Element-wise helpers.
+void launch_add_field(hipStream_t st, long n, int nsets, const double *e_m2d, const double *ext,
+                      double *ef);
+void launch_copy(hipStream_t st, long n, const double *src, double *dst);
+void launch_axpy(hipStream_t st, long n, int M, double a, const double *x, double *y);
+void launch_residual_rms(hipStream_t st, long n, int M, const double *b, const double *tx,
+                         double *r, const double *alpha, double *red, double *rms_out);
+void launch_dot(hipStream_t st, long n, const double *a, const double *b, double *red, double *out);
+void launch_energy(hipStream_t st, long n, const double *ipd_d, const double *e_p, double *red,
+                   double *out);
+
+// Jacobi / DIIS pieces.
+void launch_jacobi(hipStream_t st, long n, const double *rhs, const double *ox, const double *x,
+                   const double *alpha, double *xnew, double *err);
+void launch_multi_dot(hipStream_t st, long n, int k, const double *const *vecs, const double *e,
+                      double *red, double *out);
+void launch_lincomb(hipStream_t st, long n, int k, const double *const *vecs, const double *coef,
+                    double *out);
+void launch_maxabs_diff(hipStream_t st, long n, const double *a, const double *b, double *red,
+                        double *out);
+
+// Dense T and LU for the inversion solver.
+void launch_tdense_build(hipStream_t st, bool amoeba, int npol, const double4 *ps64,
+                         const double *pa64, const double *alpha, const int *pd_ri,
+                         const int *pd_ci, const double *pd_s, double *T);
+int launch_lu_factor(hipStream_t st, long n, double *A, int *piv, double *work);
+void launch_lu_apply(hipStream_t st, long n, const double *A, const int *piv, double *rhs);
+
+}  // namespace ommp

@@ -1,0 +1,754 @@
+// Host-side drivers of the device path: HBM residency of a system, the static
+// field, the polarization() driver (mod_polarization.F90:49-313) and its three
+// solvers, the polarization energy.  Every arithmetic step runs on the GPU;
+// the host only sequences launches and reads back convergence flags.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "ommp_internal.h"
+
+namespace ommp {
+
+#define HCHECK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess)                                                                 \
+            fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
+    } while (0)
+
+enum MvKind { kOtf64 = 0, kOtf32 = 1, kPack32 = 2, kPack64 = 3 };
+
+struct DeviceState {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int npol = 0, nmm = 0, ld = 1;
+    long n = 0;
+    // sites
+    double4 *ps64 = nullptr;
+    double *pa64 = nullptr, *alpha = nullptr;
+    float4 *ps32 = nullptr;
+    float *pa32 = nullptr;
+    int *polar_mm = nullptr;
+    double4 *ms64 = nullptr;
+    double *ma64 = nullptr, *mq = nullptr;
+    CsrDev sp, pp, pd;
+    // fields, dipoles, work vectors ([2][n] each)
+    double *e_m2d = nullptr, *ipd = nullptr, *ext = nullptr;
+    double *vb = nullptr, *vx = nullptr, *vh = nullptr, *vr = nullptr, *vz = nullptr, *vp = nullptr;
+    double *vd = nullptr, *vr2 = nullptr;
+    double *red_a = nullptr, *red_b = nullptr, *red_c = nullptr;
+    double *scal = nullptr;
+    double *tol_dev = nullptr;
+    CgState *cs = nullptr;
+    CgState *cs_host = nullptr;
+    double *scal_host = nullptr;
+    void *mvpart = nullptr;
+    size_t mvpart_bytes = 0;
+    int nsplit = 1;
+    double *rhs_part = nullptr;
+    int rhs_split = 1;
+    // packed T
+    void *tpack = nullptr;
+    int tpack_prec = 0;
+    bool tpack_valid = false;
+    void *tpart = nullptr;
+    // dense T / LU (inversion)
+    double *tdense = nullptr;
+    int *piv = nullptr;
+    bool lu_valid = false;
+    double *lu_work = nullptr;
+    // DIIS
+    double *diis_x = nullptr, *diis_e = nullptr, *diis_coef = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+void *dmalloc(DeviceState &d, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        size_t fr = 0, tot = 0;
+        (void)hipGetLastError();
+        (void)hipMemGetInfo(&fr, &tot);
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "Cannot allocate %.3f GB of device memory (%.3f GB free of %.3f GB)",
+                      bytes / 1e9, fr / 1e9, tot / 1e9);
+        fatal(buf);
+    }
+    d.allocs.push_back(p);
+    return p;
+}
+
+template <typename T> void dfree(DeviceState &d, T *&p) {
+    if (!p) return;
+    void *v = (void *)p;
+    (void)hipFree(v);
+    d.allocs.erase(std::remove(d.allocs.begin(), d.allocs.end(), v), d.allocs.end());
+    p = nullptr;
+}
+
+template <typename T> T *dalloc(DeviceState &d, size_t count) { return (T *)dmalloc(d, count * sizeof(T)); }
+
+void upload(const void *h, void *d, size_t bytes) {
+    if (bytes) HCHECK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+}
+
+// Per-target CSR with one or two factors per entry from reference-layout lists.
+void upload_csr(DeviceState &d, int nrow, const std::vector<std::vector<std::pair<int, double2>>> &rows,
+                CsrDev &out, bool two) {
+    std::vector<int> ri(nrow + 1, 0), ci;
+    std::vector<double> f;
+    std::vector<double2> f2;
+    for (int j = 0; j < nrow; j++) {
+        ri[j + 1] = ri[j] + (int)rows[j].size();
+        for (auto &e : rows[j]) {
+            ci.push_back(e.first);
+            if (two) f2.push_back(e.second);
+            else f.push_back(e.second.x);
+        }
+    }
+    out.nnz = ri[nrow];
+    out.ri = dalloc<int>(d, nrow + 1);
+    out.ci = dalloc<int>(d, std::max(out.nnz, 1));
+    upload(ri.data(), out.ri, sizeof(int) * (nrow + 1));
+    upload(ci.data(), out.ci, sizeof(int) * out.nnz);
+    if (two) {
+        out.f2 = dalloc<double2>(d, std::max(out.nnz, 1));
+        upload(f2.data(), out.f2, sizeof(double2) * out.nnz);
+    } else {
+        out.f = dalloc<double>(d, std::max(out.nnz, 1));
+        upload(f.data(), out.f, sizeof(double) * out.nnz);
+    }
+}
+
+void free_csr(DeviceState &d, CsrDev &c) {
+    dfree(d, c.ri);
+    dfree(d, c.ci);
+    dfree(d, c.f);
+    dfree(d, c.f2);
+    c = CsrDev();
+}
+
+void upload_lists(System &s, DeviceState &d) {
+    free_csr(d, d.sp);
+    free_csr(d, d.pp);
+    free_csr(d, d.pd);
+    const int npol = s.npol;
+    // S->P corrections per polarizable target: f = 1 - s for the D and P sets.
+    std::vector<std::vector<std::pair<int, double2>>> rows(npol);
+    auto add = [&](int i, int jp, double sd, double spv, bool setd, bool setp) {
+        if (s.polar_mm[jp] == i) return;  // the self pair is never summed
+        for (auto &e : rows[jp])
+            if (e.first == i) {
+                if (setd) e.second.x = 1.0 - sd;
+                if (setp) e.second.y = 1.0 - spv;
+                return;
+            }
+        double2 f;
+        f.x = setd ? 1.0 - sd : 0.0;
+        f.y = setp ? 1.0 - spv : 0.0;
+        rows[jp].push_back({i, f});
+    };
+    for (int i = 0; i < s.nmm; i++)
+        for (int k = s.list_S_P_P.ri[i]; k < s.list_S_P_P.ri[i + 1]; k++) {
+            const double sc = s.list_S_P_P.val[k];
+            // AMBER: one set, stored in both slots
+            add(i, s.list_S_P_P.ci[k], sc, sc, !s.amoeba, true);
+        }
+    if (s.amoeba)
+        for (int i = 0; i < s.nmm; i++)
+            for (int k = s.list_S_P_D.ri[i]; k < s.list_S_P_D.ri[i + 1]; k++)
+                add(i, s.list_S_P_D.ci[k], s.list_S_P_D.val[k], 1.0, true, false);
+    for (auto &r : rows) std::sort(r.begin(), r.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    upload_csr(d, npol, rows, d.sp, true);
+    // P->P corrections of the on-the-fly matvec (list_P_P, rows = source; transpose)
+    rows.assign(npol, {});
+    for (int ip = 0; ip < npol; ip++)
+        for (int k = s.list_P_P.ri[ip]; k < s.list_P_P.ri[ip + 1]; k++) {
+            double2 f;
+            f.x = 1.0 - s.list_P_P.val[k];
+            f.y = 0.0;
+            rows[s.list_P_P.ci[k]].push_back({ip, f});
+        }
+    for (auto &r : rows) std::sort(r.begin(), r.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    upload_csr(d, npol, rows, d.pp, false);
+    // dense-rule scale factors (values s, not 1 - s), rows = i
+    rows.assign(npol, {});
+    for (int ip = 0; ip < npol; ip++)
+        for (int k = s.list_P_P_dense.ri[ip]; k < s.list_P_P_dense.ri[ip + 1]; k++) {
+            double2 f;
+            f.x = s.list_P_P_dense.val[k];
+            f.y = 0.0;
+            rows[ip].push_back({s.list_P_P_dense.ci[k], f});
+        }
+    upload_csr(d, npol, rows, d.pd, false);
+}
+
+void upload_geometry(System &s, DeviceState &d) {
+    const double inf = std::numeric_limits<double>::infinity();
+    std::vector<double4> ps(s.npol), ms(s.nmm);
+    std::vector<float4> ps32(s.npol);
+    std::vector<double> pa(s.npol), ma(s.nmm);
+    std::vector<float> pa32(s.npol);
+    for (int i = 0; i < s.nmm; i++) {
+        const double t = s.thole[i];
+        ms[i] = make_double4(s.cmm[3 * i], s.cmm[3 * i + 1], s.cmm[3 * i + 2], t > kEpsRp ? 1.0 / t : inf);
+        ma[i] = s.mm_polar[i] >= 0 ? s.thole_damping_mm[i] : inf;
+    }
+    for (int p = 0; p < s.npol; p++) {
+        const int i = s.polar_mm[p];
+        ps[p] = ms[i];
+        pa[p] = s.thole_damping_mm[i];
+        ps32[p] = make_float4((float)ps[p].x, (float)ps[p].y, (float)ps[p].z, (float)ps[p].w);
+        pa32[p] = (float)pa[p];
+    }
+    upload(ps.data(), d.ps64, sizeof(double4) * s.npol);
+    upload(ps32.data(), d.ps32, sizeof(float4) * s.npol);
+    upload(pa.data(), d.pa64, sizeof(double) * s.npol);
+    upload(pa32.data(), d.pa32, sizeof(float) * s.npol);
+    upload(s.pol.data(), d.alpha, sizeof(double) * s.npol);
+    upload(s.polar_mm.data(), d.polar_mm, sizeof(int) * s.npol);
+    upload(ms.data(), d.ms64, sizeof(double4) * s.nmm);
+    upload(ma.data(), d.ma64, sizeof(double) * s.nmm);
+    upload(s.q.data(), d.mq, sizeof(double) * s.ld_cart * s.nmm);
+    d.tpack_valid = false;
+    d.lu_valid = false;
+    s.dev_geometry_dirty = false;
+}
+
+DeviceState &ensure_device(System &s) {
+    if (!s.dev) {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+            fatal("No GPU available: this library runs the polarization solver on an AMD Instinct GPU only");
+        DeviceState *dp = new DeviceState();
+        DeviceState &d = *dp;
+        HCHECK(hipGetDevice(&d.device));
+        if (s.user_stream) {
+            d.stream = (hipStream_t)s.user_stream;
+        } else {
+            HCHECK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+            d.own_stream = true;
+        }
+        for (auto &e : d.ev) HCHECK(hipEventCreate(&e));
+        d.npol = s.npol;
+        d.nmm = s.nmm;
+        d.ld = s.ld_cart;
+        d.n = 3L * s.npol;
+        const long n = d.n;
+        d.ps64 = dalloc<double4>(d, s.npol);
+        d.ps32 = dalloc<float4>(d, s.npol);
+        d.pa64 = dalloc<double>(d, s.npol);
+        d.pa32 = dalloc<float>(d, s.npol);
+        d.alpha = dalloc<double>(d, s.npol);
+        d.polar_mm = dalloc<int>(d, s.npol);
+        d.ms64 = dalloc<double4>(d, s.nmm);
+        d.ma64 = dalloc<double>(d, s.nmm);
+        d.mq = dalloc<double>(d, (size_t)s.ld_cart * s.nmm);
+        d.e_m2d = dalloc<double>(d, 2 * n);
+        d.ipd = dalloc<double>(d, 2 * n);
+        d.ext = dalloc<double>(d, n);
+        double **vecs[] = {&d.vb, &d.vx, &d.vh, &d.vr, &d.vz, &d.vp, &d.vd, &d.vr2};
+        for (auto v : vecs) *v = dalloc<double>(d, 2 * n);
+        const int nb = std::max(vec_blocks(n), site_blocks(s.npol)) + 8;
+        d.red_a = dalloc<double>(d, 24 * (size_t)nb);
+        d.red_b = dalloc<double>(d, 24 * (size_t)nb);
+        d.red_c = dalloc<double>(d, 24 * (size_t)nb);
+        d.scal = dalloc<double>(d, 64);
+        d.tol_dev = dalloc<double>(d, 4);
+        d.cs = dalloc<CgState>(d, 2);
+        HCHECK(hipHostMalloc((void **)&d.cs_host, 2 * sizeof(CgState), hipHostMallocDefault));
+        HCHECK(hipHostMalloc((void **)&d.scal_host, 64 * sizeof(double), hipHostMallocDefault));
+        d.nsplit = tmv_split(s.npol);
+        d.mvpart_bytes = tmv_part_bytes(s.npol, 2, 64, d.nsplit);
+        d.mvpart = dmalloc(d, d.mvpart_bytes);
+        d.rhs_split = std::max(1, std::min(64, (1024 + site_blocks(s.npol) - 1) / std::max(1, site_blocks(s.npol))));
+        d.rhs_split = std::min(d.rhs_split, std::max(1, (s.nmm + 2 * kBlock - 1) / (2 * kBlock)));
+        d.rhs_part = dalloc<double>(d, (size_t)d.rhs_split * n + 1);
+        HCHECK(hipMemset(d.ipd, 0, sizeof(double) * 2 * n));
+        HCHECK(hipMemset(d.e_m2d, 0, sizeof(double) * 2 * n));
+        s.dev = dp;
+        upload_lists(s, d);
+        s.dev_geometry_dirty = true;
+    }
+    DeviceState &d = *s.dev;
+    if (s.dev_geometry_dirty) upload_geometry(s, d);
+    return d;
+}
+
+void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double *y, bool dodiag,
+            double *dotpart, const CgState *skip) {
+    const long n = d.n;
+    if (kind == kOtf64 || kind == kOtf32) {
+        launch_tmv_otf(d.stream, kind == kOtf32 ? 32 : 64, s.amoeba, s.npol, M, d.ps64, d.pa64, d.ps32, d.pa32,
+                       d.alpha, d.pp, x, n, y, n, dodiag, dotpart, d.mvpart, d.nsplit,
+                       reinterpret_cast<const int *>(skip));
+    } else {
+        const int prec = kind == kPack32 ? 32 : 64;
+        if (!d.tpack_valid || d.tpack_prec != prec) {
+            dfree(d, d.tpack);
+            dfree(d, d.tpart);
+            size_t fr = 0, tot = 0;
+            HCHECK(hipMemGetInfo(&fr, &tot));
+            const size_t need = tpack_bytes(s.npol, prec) + tpack_part_bytes(s.npol, 2);
+            if (need > fr) {
+                char buf[256];
+                std::snprintf(buf, sizeof(buf),
+                              "In-core T matrix needs %.2f GB of device memory, %.2f GB are free", need / 1e9,
+                              fr / 1e9);
+                fatal(buf);
+            }
+            d.tpack = dmalloc(d, tpack_bytes(s.npol, prec));
+            d.tpart = dmalloc(d, tpack_part_bytes(s.npol, 2));
+            launch_tpack_build(d.stream, prec, s.amoeba, s.npol, d.ps64, d.pa64, d.tpack, d.pd.ri, d.pd.ci, d.pd.f);
+            d.tpack_prec = prec;
+            d.tpack_valid = true;
+        }
+        launch_tpack_mv(d.stream, prec, s.npol, M, d.tpack, d.alpha, x, n, y, n, dodiag, dotpart, d.tpart,
+                        reinterpret_cast<const int *>(skip));
+    }
+}
+
+// conjugate_gradient_solver (mod_solvers.F90:86-216) on M columns that share
+// the matvec.  Vectors are column-strided by n.  Returns per-column status.
+struct CgResult {
+    int iters[2] = {0, 0};
+    int status[2] = {2, 2};
+    double rms[2] = {0, 0};
+    double beta[2] = {0, 0};
+};
+
+CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, double *x, const double *tol,
+                  int maxit, bool zero_guess, double small_step_eps) {
+    const long n = d.n;
+    CgResult res;
+    double te[4] = {tol[0], M > 1 ? tol[1] : tol[0], small_step_eps, small_step_eps};
+    HCHECK(hipMemcpyAsync(d.tol_dev, te, sizeof(te), hipMemcpyHostToDevice, d.stream));
+    if (!zero_guess) {
+        launch_norm2_guess(d.stream, n, M, x, d.alpha, d.red_a);
+        matvec(s, d, kind, M, x, d.vh, true, nullptr, nullptr);
+    }
+    launch_cg_init(d.stream, n, M, b, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_a, d.cs, zero_guess, d.tol_dev);
+    const int nred_hp = site_blocks(s.npol);
+    int it = 1;
+    int chunk = 2;
+    bool all_done = false;
+    while (it <= maxit && !all_done) {
+        const int end = std::min(maxit, it + chunk - 1);
+        for (; it <= end; it++) {
+            matvec(s, d, kind, M, d.vp, d.vh, true, d.red_b, d.cs);
+            launch_cg_update(d.stream, n, M, it, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_b, nred_hp, d.red_c, d.cs);
+            launch_cg_direction(d.stream, n, M, it, d.vz, d.vp, d.red_c, d.cs, nullptr);
+        }
+        HCHECK(hipMemcpyAsync(d.cs_host, d.cs, sizeof(CgState) * M, hipMemcpyDeviceToHost, d.stream));
+        HCHECK(hipStreamSynchronize(d.stream));
+        all_done = true;
+        for (int m = 0; m < M; m++) all_done = all_done && d.cs_host[m].done;
+        chunk = std::min(8, chunk * 2);
+    }
+    for (int m = 0; m < M; m++) {
+        res.iters[m] = d.cs_host[m].done ? d.cs_host[m].iters : maxit;
+        res.status[m] = d.cs_host[m].done ? d.cs_host[m].status : 2;
+        res.rms[m] = d.cs_host[m].rms;
+        res.beta[m] = d.cs_host[m].last_beta;
+    }
+    return res;
+}
+
+// Pulay DIIS on Jacobi iterations (jacobi_diis_solver, diis, makeb;
+// mod_solvers.F90:218-469).  One column; the B matrix and its (m+1)^2 solve
+// stay on the host, vectors and dot products on the device.
+bool host_lu_solve(int n, std::vector<double> A, std::vector<double> &b) {
+    std::vector<int> piv(n);
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double mx = std::fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(A[i * n + k]) > mx) {
+                mx = std::fabs(A[i * n + k]);
+                p = i;
+            }
+        piv[k] = p;
+        if (A[p * n + k] == 0.0) return false;
+        if (p != k)
+            for (int j = 0; j < n; j++) std::swap(A[k * n + j], A[p * n + j]);
+        for (int i = k + 1; i < n; i++) {
+            A[i * n + k] /= A[k * n + k];
+            for (int j = k + 1; j < n; j++) A[i * n + j] -= A[i * n + k] * A[k * n + j];
+        }
+    }
+    for (int k = 0; k < n; k++)
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++) b[i] -= A[i * n + j] * b[j];
+    for (int i = n - 1; i >= 0; i--) {
+        for (int j = i + 1; j < n; j++) b[i] -= A[i * n + j] * b[j];
+        b[i] /= A[i * n + i];
+    }
+    return true;
+}
+
+int diis_solve(System &s, DeviceState &d, int kind, const double *rhs, double *x, double tol, int maxit,
+               int *iters_out) {
+    const long n = d.n;
+    const int ndiis = 20;
+    if (!d.diis_x) {
+        d.diis_x = dalloc<double>(d, (size_t)ndiis * n);
+        d.diis_e = dalloc<double>(d, (size_t)ndiis * n);
+        d.diis_coef = dalloc<double>(d, 32);
+    }
+    // zero-norm guess -> inv_diag * rhs (mod_solvers.F90:300-308)
+    launch_dot(d.stream, n, x, x, d.red_a, d.scal);
+    HCHECK(hipMemcpyAsync(d.scal_host, d.scal, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    if (d.scal_host[0] < kEpsRp) {
+        HCHECK(hipMemsetAsync(d.vz, 0, sizeof(double) * n, d.stream));
+        launch_jacobi(d.stream, n, rhs, d.vz, d.vz, d.alpha, x, d.vr2);  // x = alpha * rhs
+    }
+    std::vector<int> slot(ndiis);
+    for (int k = 0; k < ndiis; k++) slot[k] = k;
+    std::vector<double> B((ndiis + 1) * (ndiis + 1), 0.0);
+    const int W = ndiis + 1;
+    int nmat = 1;
+    double max_norm = 0.0;
+    int it;
+    double *xnew = d.vd, *y = d.vh;
+    for (it = 1; it <= maxit; it++) {
+        matvec(s, d, kind, 1, x, y, false, nullptr, nullptr);
+        double *xs = d.diis_x + (size_t)slot[nmat - 1] * n;
+        double *es = d.diis_e + (size_t)slot[nmat - 1] * n;
+        launch_jacobi(d.stream, n, rhs, y, x, d.alpha, xs, es);
+        // diis(): shift when full (keep the last ndiis-10 points)
+        if (nmat >= ndiis) {
+            for (int j = 2; j <= nmat - 10; j++)
+                for (int k = 2; k <= nmat - 10; k++) B[(j - 1) * W + (k - 1)] = B[(j + 9) * W + (k + 9)];
+            std::vector<int> ns(slot);
+            for (int j = 1; j <= nmat - 10; j++) ns[j - 1] = slot[j + 9];
+            for (int j = nmat - 10 + 1; j <= ndiis; j++) ns[j - 1] = slot[j - 1 - (nmat - 10)];
+            slot = ns;
+            nmat -= 10;
+        }
+        // makeb: new row against all stored error vectors
+        std::vector<const double *> ev(nmat);
+        for (int q = 0; q < nmat; q++) ev[q] = d.diis_e + (size_t)slot[q] * n;
+        launch_multi_dot(d.stream, n, nmat, ev.data(), ev[nmat - 1], d.red_a, d.scal);
+        HCHECK(hipMemcpyAsync(d.scal_host, d.scal, sizeof(double) * nmat, hipMemcpyDeviceToHost, d.stream));
+        HCHECK(hipStreamSynchronize(d.stream));
+        if (nmat == 1) {
+            B[0] = 0.0;
+            B[1] = 1.0;
+            B[W] = 1.0;
+            B[W + 1] = d.scal_host[0];
+        } else {
+            B[nmat * W] = 1.0;
+            B[nmat] = 1.0;
+            for (int i = 1; i <= nmat - 1; i++) B[nmat * W + i] = B[i * W + nmat] = d.scal_host[i - 1];
+            B[nmat * W + nmat] = d.scal_host[nmat - 1];
+        }
+        const int nm1 = nmat + 1;
+        std::vector<double> bloc(nm1 * nm1), cex(nm1, 0.0);
+        for (int i = 0; i < nm1; i++)
+            for (int j = 0; j < nm1; j++) bloc[i * nm1 + j] = B[i * W + j];
+        cex[0] = 1.0;
+        const double *jac = xs;
+        if (host_lu_solve(nm1, bloc, cex)) {
+            std::vector<const double *> xv(nmat);
+            for (int q = 0; q < nmat; q++) xv[q] = d.diis_x + (size_t)slot[q] * n;
+            HCHECK(hipMemcpyAsync(d.diis_coef, cex.data() + 1, sizeof(double) * nmat, hipMemcpyHostToDevice,
+                                  d.stream));
+            launch_lincomb(d.stream, n, nmat, xv.data(), d.diis_coef, xnew);
+            nmat += 1;
+        } else {
+            launch_copy(d.stream, n, jac, xnew);
+            nmat = 1;
+        }
+        launch_maxabs_diff(d.stream, n, xnew, x, d.red_a, d.scal);
+        launch_copy(d.stream, n, xnew, x);
+        HCHECK(hipMemcpyAsync(d.scal_host, d.scal, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+        HCHECK(hipStreamSynchronize(d.stream));
+        max_norm = d.scal_host[0];
+        if (max_norm < tol) break;
+    }
+    *iters_out = std::min(it, maxit);
+    return max_norm > tol ? 1 : 0;
+}
+
+// FP32 matvec inside FP64 iterative refinement, columns solved together.
+struct MixedResult {
+    int outer = 0, inner = 0;
+    bool ok = false;
+};
+
+MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, double *x, bool zero_guess, double tol,
+                        int maxit, int kind32) {
+    const long n = d.n;
+    MixedResult res;
+    const int max_outer = 30;
+    const double eta = 2e-4;  // inner relative reduction per refinement step
+    bool xzero = zero_guess;
+    for (int outer = 0; outer < max_outer; outer++) {
+        // FP64 residual r = b - T x and its preconditioned rms per column
+        if (xzero) {
+            HCHECK(hipMemsetAsync(d.vh, 0, sizeof(double) * M * n, d.stream));
+            launch_residual_rms(d.stream, n, M, b, d.vh, d.vr2, d.alpha, d.red_a, d.scal);
+        } else {
+            matvec(s, d, kOtf64, M, x, d.vh, true, nullptr, nullptr);
+            launch_residual_rms(d.stream, n, M, b, d.vh, d.vr2, d.alpha, d.red_a, d.scal);
+        }
+        HCHECK(hipMemcpyAsync(d.scal_host, d.scal, sizeof(double) * M, hipMemcpyDeviceToHost, d.stream));
+        HCHECK(hipStreamSynchronize(d.stream));
+        res.outer = outer + 1;
+        bool conv = true;
+        double tin[2];
+        for (int m = 0; m < M; m++) {
+            const double rms = d.scal_host[m];
+            conv = conv && rms < tol;
+            tin[m] = rms < tol ? 1e300 : std::max(eta * rms, 0.5 * tol);
+        }
+        if (conv) {
+            res.ok = true;
+            return res;
+        }
+        // inner FP32 PCG on T delta = r from delta = 0
+        CgResult cr = cg_solve(s, d, kind32, M, d.vr2, d.vd, tin, maxit, true, 0.0);
+        res.inner += std::max(cr.iters[0], M > 1 ? cr.iters[1] : 0);
+        launch_axpy(d.stream, n, M, 1.0, d.vd, x);
+        xzero = false;
+    }
+    return res;
+}
+
+}  // namespace
+
+// ===================================================================== API
+void device_release(System &s) {
+    if (!s.dev) return;
+    DeviceState &d = *s.dev;
+    (void)hipStreamSynchronize(d.stream);
+    for (void *p : d.allocs) (void)hipFree(p);
+    d.allocs.clear();
+    if (d.cs_host) (void)hipHostFree(d.cs_host);
+    if (d.scal_host) (void)hipHostFree(d.scal_host);
+    for (auto &e : d.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (d.own_stream) (void)hipStreamDestroy(d.stream);
+    delete s.dev;
+    s.dev = nullptr;
+}
+
+void invalidate_geometry(System &s) {
+    s.dev_geometry_dirty = true;
+    if (s.dev) {
+        s.dev->tpack_valid = false;
+        s.dev->lu_valid = false;
+    }
+}
+
+// prepare_polelec (mod_electrostatics.F90:1487-1596), cached until the geometry changes.
+void prepare_polelec(System &s) {
+    if (s.m2d_done) return;
+    DeviceState &d = ensure_device(s);
+    if (s.npol > 0) {
+        check_coincident_sites(s);
+        launch_rhs_m2d(d.stream, s.amoeba, s.nmm, s.npol, d.ms64, d.ma64, d.mq, d.ps64, d.pa64, d.polar_mm, d.sp,
+                       d.rhs_part, d.rhs_split, d.e_m2d);
+    }
+    s.m2d_done = true;
+    s.host_m2d_stale = true;
+}
+
+void refresh_host_m2d(System &s) {
+    prepare_polelec(s);
+    if (!s.host_m2d_stale) return;
+    DeviceState &d = *s.dev;
+    if (s.npol > 0)
+        HCHECK(hipMemcpyAsync(s.e_m2d.data(), d.e_m2d, sizeof(double) * s.n_ipd * d.n, hipMemcpyDeviceToHost,
+                              d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    s.host_m2d_stale = false;
+}
+
+// polarization() (mod_polarization.F90:49-313).  d_field: [n_ipd][3 npol] on the device.
+void polarization(System &s, const double *d_field, int solver, int matv, int mask) {
+    if (s.npol == 0) return;
+    DeviceState &d = ensure_device(s);
+    const long n = d.n;
+    if (solver == kSolverNone) solver = s.def_solver;
+    if (matv == kMatvNone) matv = s.def_matv;
+    if (s.n_ipd == 1) mask = 1;
+    const bool mD = mask & 1, mP = s.n_ipd > 1 && (mask & 2);
+    if (solver != kSolverCG && solver != kSolverDIIS && solver != kSolverInversion)
+        fatal("Unknown solver for calculation of the induced point dipoles");
+    if (solver != kSolverInversion && matv != kMatvIncore && matv != kMatvDirect)
+        fatal("Unknown matrix-vector method requested");
+    const bool mixed = s.precision == 1;
+    const int kind = matv == kMatvIncore ? (mixed ? kPack32 : kPack64) : (mixed ? kOtf32 : kOtf64);
+    // ipd0 = 0, then the guess for the requested sets
+    HCHECK(hipMemsetAsync(d.vx, 0, sizeof(double) * 2 * n, d.stream));
+    const bool guess = s.ipd_use_guess && solver != kSolverInversion;
+    if (guess) {
+        if (mD) launch_copy(d.stream, n, d.ipd, d.vx);
+        if (mP) launch_copy(d.stream, n, d.ipd + n, d.vx + n);
+    }
+    launch_copy(d.stream, s.n_ipd * n, d_field, d.vb);
+    HCHECK(hipEventRecord(d.ev[1], d.stream));
+    for (int k = 0; k < 4; k++) s.last_iters[k] = 0;
+    s.last_status = 0;
+    if (solver == kSolverCG && mixed) {
+        // FP32 T matvec + FP64 refinement; both sets at once when both are requested
+        const int M = (mD && mP) ? 2 : 1;
+        const int c0 = mD ? 0 : 1;
+        MixedResult mr = mixed_solve(s, d, M, d.vb + c0 * n, d.vx + c0 * n, !guess, s.tol, s.max_iter,
+                                     matv == kMatvIncore ? kPack32 : kOtf32);
+        s.last_iters[2] = mr.outer;
+        s.last_iters[3] = mr.inner;
+        s.last_iters[0] = s.last_iters[1] = mr.inner;
+        if (!mr.ok) fatal("Iterative solver did not converged");
+    } else if (solver == kSolverCG) {
+        for (int k = 0; k < s.n_ipd; k++) {
+            if (!(k == 0 ? mD : mP)) continue;
+            // P guess = converged D when there is no previous guess (mod_polarization.F90:241-243)
+            if (k == 1 && mD && !s.ipd_use_guess) launch_copy(d.stream, n, d.vx, d.vx + n);
+            CgState *cs_save = d.cs;
+            d.cs = cs_save + k;
+            double tol = s.tol;
+            CgResult r = cg_solve(s, d, kind, 1, d.vb + k * n, d.vx + k * n, &tol, s.max_iter, false, kEpsRp);
+            d.cs = cs_save;
+            s.last_iters[k] = r.iters[0];
+            if (r.status[0] == 1) s.last_status = 1;
+            // mod_solvers.F90:212-214
+            if (r.status[0] == 2 && std::fabs(r.beta[0]) > kEpsRp) fatal("Iterative solver did not converged");
+        }
+    } else if (solver == kSolverDIIS) {
+        for (int k = 0; k < s.n_ipd; k++) {
+            if (!(k == 0 ? mD : mP)) continue;
+            if (k == 1 && mD && !s.ipd_use_guess) launch_copy(d.stream, n, d.vx, d.vx + n);
+            int it = 0;
+            // DIIS runs the FP64 matvec of the requested kind
+            const int dk = matv == kMatvIncore ? kPack64 : kOtf64;
+            int bad = diis_solve(s, d, dk, d.vb + k * n, d.vx + k * n, s.tol, s.max_iter, &it);
+            s.last_iters[k] = it;
+            if (bad) fatal("Iterative solver did not converged");
+        }
+    } else {
+        // inversion_solver: dense T, LU with partial pivoting, triangular solves
+        if (n > 40000) fatal("The inversion solver is limited to 13333 polarizable sites; use CG or DIIS");
+        if (!d.lu_valid) {
+            dfree(d, d.tdense);
+            dfree(d, d.piv);
+            dfree(d, d.lu_work);
+            d.tdense = dalloc<double>(d, (size_t)n * n);
+            d.piv = dalloc<int>(d, n);
+            d.lu_work = dalloc<double>(d, 8);
+            launch_tdense_build(d.stream, s.amoeba, s.npol, d.ps64, d.pa64, d.alpha, d.pd.ri, d.pd.ci, d.pd.f,
+                                d.tdense);
+            int info = launch_lu_factor(d.stream, n, d.tdense, d.piv, d.lu_work);
+            if (info != 0) fatal("Polarization matrix is singular (LU factorization failed)");
+            d.lu_valid = true;
+        }
+        for (int k = 0; k < s.n_ipd; k++) {
+            if (!(k == 0 ? mD : mP)) continue;
+            launch_copy(d.stream, n, d.vb + k * n, d.vx + k * n);
+            launch_lu_apply(d.stream, n, d.tdense, d.piv, d.vx + k * n);
+        }
+    }
+    HCHECK(hipEventRecord(d.ev[2], d.stream));
+    // eel%ipd = ipd0: unrequested sets become zero
+    launch_copy(d.stream, s.n_ipd * n, d.vx, d.ipd);
+    s.ipd_done = true;
+    s.ipd_use_guess = true;
+    s.host_ipd_stale = true;
+}
+
+double polelec_energy(System &s) {
+    if (s.npol == 0) return 0.0;
+    if (!s.ipd_done) {
+        prepare_polelec(s);
+        polarization(s, s.dev->e_m2d, kSolverNone, kMatvNone, 3);
+    }
+    DeviceState &d = *s.dev;
+    const long n = d.n;
+    // -1/2 mu_D . E_M2D(P) for AMOEBA, -1/2 mu . E_M2D for AMBER
+    launch_energy(d.stream, n, d.ipd, d.e_m2d + (s.amoeba ? n : 0), d.red_a, d.scal);
+    HCHECK(hipMemcpyAsync(d.scal_host, d.scal, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    return d.scal_host[0];
+}
+
+void refresh_host_ipd(System &s) {
+    if (!s.host_ipd_stale || !s.dev) return;
+    DeviceState &d = *s.dev;
+    HCHECK(hipMemcpyAsync(s.ipd.data(), d.ipd, sizeof(double) * s.n_ipd * d.n, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    s.host_ipd_stale = false;
+}
+
+const double *device_ipd(System &s) {
+    DeviceState &d = ensure_device(s);
+    return d.ipd;
+}
+
+double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat) {
+    if (ncols < 1 || ncols > 2) fatal("tmatvec supports 1 or 2 columns");
+    if (kind < 0 || kind > 3) fatal("Unknown matvec kind");
+    if (s.npol == 0) return 0.0;
+    DeviceState &d = ensure_device(s);
+    const long n = d.n;
+    HCHECK(hipMemcpyAsync(d.vd, x, sizeof(double) * ncols * n, hipMemcpyHostToDevice, d.stream));
+    matvec(s, d, kind, ncols, d.vd, d.vr2, true, nullptr, nullptr);  // builds packed T if needed
+    if (repeat < 1) repeat = 1;
+    HCHECK(hipEventRecord(d.ev[0], d.stream));
+    for (int r = 0; r < repeat; r++) matvec(s, d, kind, ncols, d.vd, d.vr2, true, nullptr, nullptr);
+    HCHECK(hipEventRecord(d.ev[1], d.stream));
+    HCHECK(hipMemcpyAsync(y, d.vr2, sizeof(double) * ncols * n, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);
+    return ms / repeat;
+}
+
+// ommp_set_external_field[_nomm] (mod_interface.F90:184-233)
+void set_external_field(System &s, const double *ext, bool ext_on_device, int solver, int matv, bool nomm) {
+    s.ipd_done = false;
+    if (s.npol == 0) return;
+    DeviceState &d = ensure_device(s);
+    const long n = d.n;
+    HCHECK(hipEventRecord(d.ev[0], d.stream));
+    if (ext_on_device) launch_copy(d.stream, n, ext, d.ext);
+    else HCHECK(hipMemcpyAsync(d.ext, ext, sizeof(double) * n, hipMemcpyHostToDevice, d.stream));
+    double *ef = d.vr;  // staging: [n_ipd][n]
+    if (!nomm) {
+        prepare_polelec(s);
+        launch_add_field(d.stream, n, s.n_ipd, d.e_m2d, d.ext, ef);
+        // the MM-field branch never forwards matv (mod_interface.F90:221)
+        polarization(s, ef, solver, kMatvNone, 3);
+    } else {
+        launch_add_field(d.stream, n, 1, nullptr, d.ext, ef);
+        if (s.n_ipd > 1) HCHECK(hipMemsetAsync(ef + n, 0, sizeof(double) * n, d.stream));
+        polarization(s, ef, solver, matv, 1);
+    }
+    HCHECK(hipEventRecord(d.ev[3], d.stream));
+    if (!ext_on_device) refresh_host_ipd(s);
+    HCHECK(hipEventSynchronize(d.ev[3]));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);
+    s.last_ms[0] = ms;
+    (void)hipEventElapsedTime(&ms, d.ev[1], d.ev[2]);
+    s.last_ms[1] = ms;
+    (void)hipEventElapsedTime(&ms, d.ev[2], d.ev[3]);
+    s.last_ms[2] = ms;
+}
+
+}  // namespace ommp

@@ -1,0 +1,112 @@
+// Internal data model of the MI355X induced-dipole library.
+//
+// Host side: the prepared system (what the reference keeps in ommp_system /
+// ommp_electrostatics_type, src/mod_electrostatics.F90:23-241), built on the
+// CPU in O(N * neighbours).  Device side: HBM-resident copies laid out for the
+// pair kernels (see device.h).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ommp {
+
+constexpr double kEpsRp = 2.220446049250313080847e-16 * 100.0;  // mod_constants.F90:61
+constexpr double kAng2Au = 1.8897261245650;                     // openmmpol_const.h.in:34
+constexpr int kSolverNone = 0, kSolverCG = 1, kSolverDIIS = 2, kSolverInversion = 3;
+constexpr int kMatvNone = 0, kMatvIncore = 1, kMatvDirect = 2;
+
+// Compressed sparse rows, 0-based.
+struct Csr {
+    int n = 0;
+    std::vector<int> ri{0};
+    std::vector<int> ci;
+    std::vector<double> val;
+    int nnz() const { return ri.empty() ? 0 : ri.back(); }
+};
+
+struct DeviceState;  // device.h
+
+struct System {
+    // ---------------------------------------------------------------- model
+    bool amoeba = false;
+    int ff_rules = 0;
+    int nmm = 0, npol = 0, ld_cart = 1, n_ipd = 1;
+    std::vector<double> cmm;       // [nmm][3] bohr
+    std::vector<double> q0;        // [nmm][ld] unrotated, AMOEBA quadrupoles / 3
+    std::vector<double> q;         // [nmm][ld] lab frame
+    std::vector<int> mol_frame, ix, iy, iz;  // AMOEBA frames, 0-based atom or -1
+    std::vector<double> pol;       // [npol] bohr^3
+    std::vector<int> polar_mm;     // [npol]
+    std::vector<int> mm_polar;     // [nmm], -1 if not polarizable
+    std::vector<double> cpol;      // [npol][3]
+    std::vector<double> thole;     // [nmm]
+    std::vector<double> thole_damping_mm;  // [nmm] damping a per MM site
+    double thole_scale = 0.0;
+    Csr adj;                       // 1-2
+    Csr conn[4];                   // exact bond distance 1..4
+    std::vector<int> grp;          // [nmm] polarization group (AMOEBA)
+    int ngrp = 0;
+    Csr grp_members;
+    Csr gconn[4];                  // group distance 0..3
+    double mscale[4], pscale[4], dscale[4], uscale[4], pscale_intra[4];
+    // screening lists, rows = source (reference layout), val = scale factor
+    Csr list_S_P_P, list_S_P_D, list_P_P;
+    // P-P scale factors the dense/incore T uses: the reference builds that
+    // matrix with screening_rules on every pair (mod_polarization.F90:348),
+    // which for AMOEBA reaches group neighbours that are not bond neighbours.
+    Csr list_P_P_dense;
+
+    // ------------------------------------------------------------ API state
+    int def_solver = kSolverCG, def_matv = kMatvDirect;
+    bool fmm_requested = false;
+    bool m2d_done = false, ipd_done = false, ipd_use_guess = false;
+    bool host_ipd_stale = false;
+    std::vector<double> ipd;       // [n_ipd][npol][3] host mirror
+    std::vector<double> e_m2d;     // [n_ipd][npol][3] host mirror
+    bool host_m2d_stale = true;
+
+    // -------------------------------------------------------------- options
+    double tol = 1e-8;
+    int max_iter = 200;
+    int precision = 0;             // OMMP_AMD_PREC_*
+    int last_iters[4] = {0, 0, 0, 0};
+    int last_status = 0;
+    double last_ms[3] = {0, 0, 0};
+    void *user_stream = nullptr;
+
+    DeviceState *dev = nullptr;
+    bool dev_geometry_dirty = true;  // positions/multipoles need re-upload
+};
+
+// prep.cpp
+System *read_mmp(const std::string &path);
+System *build_from_arrays(int ff_type, int ff_rules, int nmm, const double *cmm, const double *q,
+                          const double *pol, const double *thole_a, const int *adj_ri,
+                          const int *adj_ci, const int *pgroup, const int *frames);
+void prepare(System &s);                 // mmpol_prepare
+void rotate_multipoles(System &s);
+void make_screening_lists(System &s);
+void check_coincident_sites(const System &s);
+void set_default_scales(System &s);
+
+// io.cpp
+void message(const std::string &msg, int level, const char *prefix = nullptr);
+[[noreturn]] void fatal(const std::string &msg);
+int verbosity();
+
+// engine.cpp
+void device_release(System &s);
+void prepare_polelec(System &s);                              // E_M2D on device
+void polarization(System &s, const double *d_field, int solver, int matv, int mask);
+double polelec_energy(System &s);
+void refresh_host_ipd(System &s);
+void set_external_field(System &s, const double *ext, bool ext_on_device, int solver, int matv,
+                        bool nomm);
+void invalidate_geometry(System &s);
+const double *device_ipd(System &s);
+void refresh_host_m2d(System &s);
+double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat);
+
+}  // namespace ommp

@@ -1,0 +1,246 @@
+"""Python face of the library, mirroring the reference's pyopenmmpol module for
+the polarization path (pythonapi/pyopenmmpol/pyommp_interface.cpp:9-30,
+176-266, 482-530, 888-890, 1351-1536): same class, method and property names,
+same argument meaning and the same ValueError checks.  Everything below calls
+the C-ABI of libopenmmpol_amd.so.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import c_dp, c_ip, lib
+
+OMMP_SOLVER_NONE, OMMP_SOLVER_CG, OMMP_SOLVER_DIIS, OMMP_SOLVER_INVERSION = 0, 1, 2, 3
+OMMP_MATV_NONE, OMMP_MATV_INCORE, OMMP_MATV_DIRECT = 0, 1, 2
+OMMP_FF_AMOEBA, OMMP_FF_AMBER = 1, 0
+PREC_FP64, PREC_MIXED = 0, 1
+
+available_solvers = {"none": OMMP_SOLVER_NONE, "default": OMMP_SOLVER_CG, "conjugate gradient": OMMP_SOLVER_CG,
+                     "cg": OMMP_SOLVER_CG, "inversion": OMMP_SOLVER_INVERSION, "diis": OMMP_SOLVER_DIIS}
+available_matrix_vector = {"none": OMMP_MATV_NONE, "default": OMMP_MATV_DIRECT, "incore": OMMP_MATV_INCORE,
+                           "direct": OMMP_MATV_DIRECT}
+verbosity = {"none": 0, "low": 1, "high": 2, "debug": 3}
+__version__ = "openmmpol-amd-0.1"
+
+
+def set_verbose(level):
+    lib().ommp_set_verbose(int(level))
+
+
+def time_push():
+    lib().ommp_time_push()
+
+
+def time_pull(label):
+    lib().ommp_time_pull(label.encode())
+
+
+def set_outputfile(path):
+    lib().ommp_set_outputfile(path.encode())
+
+
+def close_outputfile():
+    lib().ommp_close_outputfile()
+
+
+def message(msg, level, prefix=""):
+    lib().ommp_message(msg.encode(), int(level), prefix.encode())
+
+
+def _view(ptr, shape):
+    if not ptr:
+        return np.zeros(shape)
+    return np.ctypeslib.as_array(ptr, shape=shape)
+
+
+class OMMPSystem:
+    """System of OMMP library (polarization path)."""
+
+    def __init__(self, mmp_filename=None, _handle=None):
+        if _handle is not None:
+            self.handler = _handle
+        else:
+            if mmp_filename is None:
+                raise ValueError("mmp_filename is required")
+            self.handler = lib().ommp_init_mmp(str(mmp_filename).encode())
+
+    @classmethod
+    def from_arrays(cls, amoeba, cmm, q, pol, bonds_ri, bonds_ci, pgroup=None, frames=None, thole_a=None,
+                    ff_rules=0):
+        """Build a system from arrays (ommp_amd_init_arrays).  cmm in bohr,
+        q in the .mmp convention, pol in bohr^3, 1-2 connectivity as 0-based CSR."""
+        nmm = len(pol)
+        arrs = dict(cmm=np.ascontiguousarray(cmm, dtype=np.float64).reshape(nmm, 3),
+                    q=np.ascontiguousarray(q, dtype=np.float64).reshape(nmm, -1),
+                    pol=np.ascontiguousarray(pol, dtype=np.float64),
+                    ri=np.ascontiguousarray(bonds_ri, dtype=np.int32),
+                    ci=np.ascontiguousarray(bonds_ci if len(bonds_ci) else np.zeros(1), dtype=np.int32))
+        pg = None if pgroup is None else np.ascontiguousarray(pgroup, dtype=np.int32)
+        fr = None if frames is None else np.ascontiguousarray(frames, dtype=np.int32).reshape(nmm, 4)
+        ta = None if thole_a is None else np.ascontiguousarray(thole_a, dtype=np.float64)
+        h = lib().ommp_amd_init_arrays(OMMP_FF_AMOEBA if amoeba else OMMP_FF_AMBER, int(ff_rules), nmm,
+                                       arrs["cmm"].ctypes.data_as(c_dp), arrs["q"].ctypes.data_as(c_dp),
+                                       arrs["pol"].ctypes.data_as(c_dp),
+                                       None if ta is None else ta.ctypes.data_as(c_dp),
+                                       arrs["ri"].ctypes.data_as(c_ip), arrs["ci"].ctypes.data_as(c_ip),
+                                       None if pg is None else pg.ctypes.data_as(c_ip),
+                                       None if fr is None else fr.ctypes.data_as(c_ip))
+        return cls(_handle=h)
+
+    def __del__(self):
+        self.terminate()
+
+    def terminate(self):
+        if getattr(self, "handler", None):
+            lib().ommp_terminate(self.handler)
+            self.handler = None
+
+    @property
+    def is_init(self):
+        return bool(self.handler)
+
+    # ------------------------------------------------------------- properties
+    @property
+    def pol_atoms(self):
+        return lib().ommp_get_pol_atoms(self.handler)
+
+    @property
+    def mm_atoms(self):
+        return lib().ommp_get_mm_atoms(self.handler)
+
+    @property
+    def n_ipd(self):
+        return lib().ommp_get_n_ipd(self.handler)
+
+    @property
+    def _ld_cart(self):
+        return lib().ommp_get_ld_cart(self.handler)
+
+    @property
+    def is_amoeba(self):
+        return bool(lib().ommp_ff_is_amoeba(self.handler))
+
+    @property
+    def ipd(self):
+        return _view(lib().ommp_get_ipd(self.handler), (self.n_ipd, self.pol_atoms, 3))
+
+    @property
+    def cmm(self):
+        return _view(lib().ommp_get_cmm(self.handler), (self.mm_atoms, 3))
+
+    @property
+    def cpol(self):
+        return _view(lib().ommp_get_cpol(self.handler), (self.pol_atoms, 3))
+
+    @property
+    def _q(self):
+        return _view(lib().ommp_get_q(self.handler), (self.mm_atoms, self._ld_cart))
+
+    @property
+    def static_charges(self):
+        return self._q[:, 0]
+
+    @property
+    def polar_mm(self):
+        return _view(lib().ommp_get_polar_mm(self.handler), (self.pol_atoms,))
+
+    # --------------------------------------------------------------- methods
+    def set_external_field(self, external_field, nomm=False, solver="none", matv="none"):
+        ef = np.ascontiguousarray(external_field, dtype=np.float64)
+        if ef.ndim != 2 or ef.shape[0] != self.pol_atoms or ef.shape[1] != 3:
+            raise ValueError("ext_field should be shaped [pol_atoms, 3]")
+        if solver not in available_solvers:
+            raise ValueError("Selected solver is not available!")
+        if matv not in available_matrix_vector:
+            raise ValueError("Selected matrix-vector method is not available!")
+        fn = lib().ommp_set_external_field_nomm if nomm else lib().ommp_set_external_field
+        fn(self.handler, ef.ctypes.data_as(c_dp), available_solvers[solver], available_matrix_vector[matv])
+
+    def get_polelec_energy(self):
+        return lib().ommp_get_polelec_energy(self.handler)
+
+    def update_coordinates(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        if c.ndim != 2 or c.shape[0] != self.mm_atoms or c.shape[1] != 3:
+            raise ValueError("ext_field should be shaped [mm_atoms, 3]")
+        lib().ommp_update_coordinates(self.handler, c.ctypes.data_as(c_dp))
+
+    def set_default_solver(self, solver):
+        lib().ommp_set_default_solver(self.handler, available_solvers[solver])
+
+    def set_default_matv(self, matv):
+        lib().ommp_set_default_matv(self.handler, available_matrix_vector[matv])
+
+    # ------------------------------------------------------------ extensions
+    def set_screening_parameters(self, m, p, d, u, intra=None):
+        arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (m, p, d, u)]
+        it = None if intra is None else np.ascontiguousarray(intra, dtype=np.float64)
+        lib().ommp_amd_set_screening_parameters(self.handler, *[a.ctypes.data_as(c_dp) for a in arr],
+                                                None if it is None else it.ctypes.data_as(c_dp))
+
+    def set_solver_tolerance(self, tol, max_iter=200):
+        lib().ommp_amd_set_solver_tolerance(self.handler, float(tol), int(max_iter))
+
+    def set_precision(self, mode):
+        lib().ommp_amd_set_precision(self.handler, {"fp64": PREC_FP64, "mixed": PREC_MIXED}[mode])
+
+    @property
+    def last_iterations(self):
+        out = (ctypes.c_int32 * 4)()
+        lib().ommp_amd_get_last_iterations(self.handler, out)
+        return list(out)
+
+    @property
+    def last_status(self):
+        return lib().ommp_amd_get_last_status(self.handler)
+
+    @property
+    def last_timings_ms(self):
+        out = (ctypes.c_double * 3)()
+        lib().ommp_amd_get_last_timings(self.handler, out)
+        return list(out)
+
+    @property
+    def e_m2d(self):
+        return _view(lib().ommp_amd_get_e_m2d(self.handler), (self.n_ipd, self.pol_atoms, 3)).copy()
+
+    @property
+    def thole(self):
+        return _view(lib().ommp_amd_get_thole(self.handler), (self.mm_atoms,)).copy()
+
+    @property
+    def pol(self):
+        return _view(lib().ommp_amd_get_pol(self.handler), (self.pol_atoms,)).copy()
+
+    def get_list(self, which):
+        """Host-side screening/topology lists (see ommp_amd_get_list)."""
+        nnz = lib().ommp_amd_get_list(self.handler, int(which), None, None, None)
+        nrow = self.pol_atoms if which == 2 else self.mm_atoms
+        if which >= 7:
+            # group lists: rows = groups; size from the CSR itself
+            nrow = self.mm_atoms
+        ri = np.zeros(nrow + 2, dtype=np.int32)
+        ci = np.zeros(max(nnz, 1), dtype=np.int32)
+        val = np.zeros(max(nnz, 1))
+        lib().ommp_amd_get_list(self.handler, int(which), ri.ctypes.data_as(c_ip), ci.ctypes.data_as(c_ip),
+                                val.ctypes.data_as(c_dp))
+        return ri, ci[:nnz], val[:nnz]
+
+    def tmatvec(self, x, kind="otf64", repeat=1):
+        """y = T x for x [ncols, pol_atoms, 3]; returns (y, mean ms per launch)."""
+        kinds = {"otf64": 0, "otf32": 1, "incore32": 2, "incore64": 3}
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, self.pol_atoms, 3)
+        y = np.zeros_like(x)
+        ms = lib().ommp_amd_tmatvec(self.handler, kinds[kind], x.shape[0], x.ctypes.data_as(c_dp),
+                                    y.ctypes.data_as(c_dp), int(repeat))
+        return y, ms
+
+    def set_external_field_device(self, d_ext_ptr, solver="none"):
+        lib().ommp_amd_set_external_field_device(self.handler, ctypes.c_void_p(d_ext_ptr),
+                                                 available_solvers[solver])
+
+    def device_ipd_ptr(self):
+        return lib().ommp_amd_device_ipd(self.handler)
+
+    def reset_guess(self, also_static_field=False):
+        lib().ommp_amd_reset_guess(self.handler, bool(also_static_field))

@@ -20,3 +20,24 @@ def oracle():
     import oracle_api
     oracle_api.lib()
     return oracle_api
+
+
+@pytest.fixture(scope="session")
+def fixture_file(tmp_path_factory):
+    """Decompressed copy of a reference fixture file (the library reads plain files)."""
+    import gzip
+    import shutil
+
+    import mmp_io
+    base = tmp_path_factory.mktemp("fixtures")
+    cache = {}
+
+    def get(system, name):
+        key = (system, name)
+        if key not in cache:
+            dst = base / f"{system}_{name}"
+            with gzip.open(mmp_io.fixture_path(system, name), "rb") as fi, open(dst, "wb") as fo:
+                shutil.copyfileobj(fi, fo)
+            cache[key] = str(dst)
+        return cache[key]
+    return get
