@@ -1,0 +1,207 @@
+"""Polarization-solve benchmark on one MI355X.
+
+Workload: 15,000 polarizable AMOEBA sites uniform in a 33 A sphere (seeded
+chain topology, polarization groups of 3-12 sites, p-scale 0/0/0.4/0.8), D and
+P dipole sets.  One step = one complete polarization solve as after a geometry
+update: static multipole field (FP64), both dipole sets from a cold start to
+the tolerance, polarization energy.  Inputs stay resident in HBM; the field is
+passed as a device pointer and the dipoles stay on the device.
+
+Reported: solves/s of the mixed solver (FP32 T matvec inside FP64 iterative
+refinement), the same for the plain FP64 CG, p50/p99 latency, the energy
+error of both against the FP64 oracle's golden value (tight solve), live
+timing of the dominant kernel (the FP32 on-the-fly pair kernel) against the
+FP32 peak, and the oracle's CPU rate on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s @15k sites, mixed vs FP64 at ≤1e-9 rel E_pol err; p50/p99 solve latency"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+N_SITES, RADIUS, SEED = 15000, 33.0, 2
+
+
+def pair_flops(n, m):
+    """Ordered-pair on-the-fly matvec: N(N-1)(29 + 18m) FLOP per launch."""
+    return n * (n - 1) * (29 + 18 * m)
+
+
+def load_golden():
+    path = os.path.join(ROOT, "tests", "golden", f"sphere{N_SITES}_seed{SEED}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def run_mode(spec, mode, tol, steps, warmup, d_zero_ptr):
+    from openmmpol_amd import synth
+    s = synth.make_system(spec)
+    s.set_precision(mode)
+    s.set_solver_tolerance(tol, 500)
+    # first solve builds device state (upload, scratch) outside the timed region
+    s.reset_guess(True)
+    s.set_external_field_device(d_zero_ptr)
+    s.get_polelec_energy()
+    for _ in range(warmup):
+        s.reset_guess(True)
+        s.set_external_field_device(d_zero_ptr)
+        s.get_polelec_energy()
+    s.profile_kernels(True)
+    s.kernel_stats(reset=True)
+    lat = []
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        s.reset_guess(True)                   # drop guess and static field (new geometry)
+        s.set_external_field_device(d_zero_ptr)
+        e = s.get_polelec_energy()            # returns after the device work completed
+        lat.append((time.perf_counter() - t) * 1e3)
+        iters.append(s.last_iterations)
+    elapsed = time.perf_counter() - t0
+    stats = s.kernel_stats(reset=True)
+    s.profile_kernels(False)
+    ipd = s.ipd.copy()
+    s.terminate()
+    return dict(elapsed=elapsed, lat=np.array(lat), epol=e, ipd=ipd, iters=iters[-1], stats=stats)
+
+
+def cpu_baseline(spec, fp64_iters):
+    """Oracle (FP64 C/OpenMP restatement of the reference) on the host cores:
+    static field once + 8 T matvecs of the same system, extrapolated to one
+    full D+P CG solve with the FP64 iteration count."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as oa
+    from openmmpol_amd import synth
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    o = oa.OracleSystem(synth.oracle_prep(spec))
+    t = time.perf_counter()
+    o.m2d()
+    t_rhs = time.perf_counter() - t
+    x = np.random.default_rng(0).normal(size=(o.npol, 3))
+    nmv = 8
+    t = time.perf_counter()
+    for _ in range(nmv):
+        x = 1e-3 * o.tmatvec(x)
+    t_mv = (time.perf_counter() - t) / nmv
+    n_mv = int(fp64_iters[0]) + int(fp64_iters[1]) + 2
+    t_solve = t_rhs + n_mv * t_mv
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(oa.lib().oracle_num_threads()),
+            "kind": "port",
+            "sample": (f"oracle FP64 on {cores} host cores: static field once ({t_rhs:.2f} s) + {nmv} T matvecs "
+                       f"({t_mv * 1e3:.0f} ms each) of the 15k-site system, extrapolated to one D+P CG solve of "
+                       f"{n_mv} matvecs")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-9,
+                    help="solver tolerance on the preconditioned residual rms")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    from openmmpol_amd import synth
+    hip = ctypes.CDLL("libamdhip64.so")
+    spec = synth.sphere_system(N_SITES, RADIUS, seed=SEED)
+    n = len(spec["pol"])
+    d_zero = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d_zero), ctypes.c_size_t(3 * n * 8)) == 0
+    assert hip.hipMemset(d_zero, 0, ctypes.c_size_t(3 * n * 8)) == 0
+    print(f"[bench] system ready: {n} sites; running mixed", file=sys.stderr, flush=True)
+
+    mixed = run_mode(spec, "mixed", args.tol, args.steps, args.warmup, d_zero.value)
+    print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
+    fp64 = None
+    if not args.mixed_only:
+        fp64 = run_mode(spec, "fp64", args.tol, max(3, args.steps // 4), 1, d_zero.value)
+        print(f"[bench] fp64: {len(fp64['lat']) / fp64['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
+    hip.hipFree(d_zero)
+
+    golden = load_golden()
+    ref_e = golden["cases"]["zero"]["epol"] if golden else None
+
+    def mu_err(ipd):
+        if not golden:
+            return None
+        smp = golden["sample"]
+        return float(np.abs(ipd[:, smp, :] - np.array(golden["cases"]["zero"]["ipd_sample"])).max())
+    value = args.steps / mixed["elapsed"]
+    calls, ms = mixed["stats"]["otf32"]
+    avg_ms = ms / max(calls, 1)
+    flops = pair_flops(n, 2)
+    achieved = flops / (avg_ms * 1e-3) / 1e12 if calls else None
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["elapsed"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "15000-site AMOEBA sphere (R 33 A, seed 2), D+P dipoles, static field + "
+                               "cold-start solve + energy per step, mixed FP32 matvec / FP64 refinement",
+                   "n_sites": n, "tolerance": args.tol, "external_field": "zero", "parallelism": "single GPU"},
+        "p50_ms": float(np.percentile(mixed["lat"], 50)),
+        "p99_ms": float(np.percentile(mixed["lat"], 99)),
+        "iterations_mixed": {"outer_fp64": mixed["iters"][2], "inner_fp32_total": mixed["iters"][3]},
+        "epol_mixed": mixed["epol"],
+        "epol_oracle_golden": ref_e,
+        "epol_rel_err_mixed": abs(mixed["epol"] - ref_e) / abs(ref_e) if ref_e else None,
+        "mu_maxabs_err_mixed": mu_err(mixed["ipd"]),
+        "roofline": {"bound": "valu", "kernel": "k_tmv_pairs<float,2,AMOEBA> (on-the-fly FP32 pair kernel)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": None,
+                     "launches": calls, "avg_ms": avg_ms, "flop_per_launch": flops},
+        "kernel_time_ms": {k: v[1] for k, v in mixed["stats"].items()},
+        "kernel_launches": {k: v[0] for k, v in mixed["stats"].items()},
+    }
+    if fp64 is not None:
+        v64 = len(fp64["lat"]) / fp64["elapsed"]
+        c64, ms64 = fp64["stats"]["otf64"]
+        out.update({
+            "fp64_value": v64,
+            "speedup_vs_fp64": value / v64,
+            "fp64_p50_ms": float(np.percentile(fp64["lat"], 50)),
+            "fp64_p99_ms": float(np.percentile(fp64["lat"], 99)),
+            "iterations_fp64": {"D": fp64["iters"][0], "P": fp64["iters"][1]},
+            "epol_fp64": fp64["epol"],
+            "epol_rel_err_fp64": abs(fp64["epol"] - ref_e) / abs(ref_e) if ref_e else None,
+            "mu_maxabs_err_fp64": mu_err(fp64["ipd"]),
+            "mu_maxabs_mixed_vs_fp64": float(np.abs(mixed["ipd"] - fp64["ipd"]).max()),
+            "fp64_pair_kernel": {"launches": c64, "avg_ms": ms64 / max(c64, 1),
+                                 "achieved_tflops": pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12
+                                 if c64 else None, "peak": PEAK_FP64_TFLOPS},
+        })
+        if not args.no_cpu_baseline:
+            print("[bench] cpu baseline", file=sys.stderr, flush=True)
+            out["cpu_baseline"] = cpu_baseline(spec, fp64["iters"])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -199,6 +199,12 @@ double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const do
     return ommp::tmatvec_host(*sys(s), kind, ncols, x, y, repeat);
 }
 
+void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable) { ommp::profile_enable(*sys(s), enable); }
+
+void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset) {
+    ommp::profile_stats(*sys(s), stats, reset);
+}
+
 void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out) {
     for (int k = 0; k < 3; k++) out[k] = sys(s)->last_ms[k];
 }

@@ -48,10 +48,13 @@ struct CsrDev {
 };
 
 // ----------------------------------------------------------------- launchers
+// Optional events evb / eve are recorded on the stream right before and after
+// the pair (or packed-stream) kernel only, for live per-kernel timing.
 // Static field of the multipoles at the polarizable sites (E_M2D), FP64.
 void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double4 *ms64,
                     const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
-                    const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d);
+                    const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d,
+                    hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
 
 // T matvec on the fly: y[m] = T x[m] (dodiag) or (T - diag) x[m], m < M.
 // prec 64 or 32 selects the pair arithmetic; corrections and the final sum are FP64.
@@ -60,8 +63,10 @@ void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double
 void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, const double4 *ps64,
                     const double *pa64, const float4 *ps32, const float *pa32, const double *alpha,
                     const CsrDev &pp, const double *x, long xstride, double *y, long ystride,
-                    bool dodiag, double *dotpart, void *part, int nsplit, const int *skip_flag);
+                    bool dodiag, double *dotpart, void *part, int nsplit, const int *skip_flag,
+                    hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
 int tmv_split(int npol);
+int balanced_split(int tb, int nsrc, int num_cu);
 size_t tmv_part_bytes(int npol, int M, int prec, int nsplit);
 int vec_blocks(long n);     // blocks used by the element-wise kernels
 int site_blocks(int npol);  // blocks used by the per-site kernels
@@ -73,7 +78,8 @@ void launch_tpack_build(hipStream_t st, int prec, bool amoeba, int npol, const d
                         const double *pd_s);
 void launch_tpack_mv(hipStream_t st, int prec, int npol, int M, const void *tpack, const double *alpha,
                      const double *x, long xstride, double *y, long ystride, bool dodiag,
-                     double *dotpart, void *part, const int *skip_flag);
+                     double *dotpart, void *part, const int *skip_flag, hipEvent_t evb = nullptr,
+                     hipEvent_t eve = nullptr);
 size_t tpack_part_bytes(int npol, int M);
 
 // CG pieces (columns m < M, vectors [m * n + k]).

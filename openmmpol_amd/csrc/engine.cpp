@@ -70,6 +70,13 @@ struct DeviceState {
     double *diis_x = nullptr, *diis_e = nullptr, *diis_coef = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
+    // live kernel timing: event pairs in flight and accumulated totals per class
+    bool prof = false;
+    std::vector<hipEvent_t> prof_pool;
+    std::vector<int> prof_class;
+    int prof_used = 0;
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    long prof_count[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -100,6 +107,35 @@ template <typename T> void dfree(DeviceState &d, T *&p) {
 }
 
 template <typename T> T *dalloc(DeviceState &d, size_t count) { return (T *)dmalloc(d, count * sizeof(T)); }
+
+constexpr int kProfPool = 512;
+
+void prof_flush(DeviceState &d) {
+    if (d.prof_used == 0) return;
+    HCHECK(hipEventSynchronize(d.prof_pool[2 * (d.prof_used - 1) + 1]));
+    for (int i = 0; i < d.prof_used; i++) {
+        float ms = 0.0f;
+        HCHECK(hipEventElapsedTime(&ms, d.prof_pool[2 * i], d.prof_pool[2 * i + 1]));
+        d.prof_ms[d.prof_class[i]] += ms;
+        d.prof_count[d.prof_class[i]] += 1;
+    }
+    d.prof_used = 0;
+}
+
+// Returns the index of an event pair to bracket one launch, or -1 when off.
+int prof_begin(DeviceState &d, int cls) {
+    if (!d.prof) return -1;
+    if (d.prof_pool.empty()) {
+        d.prof_pool.resize(2 * kProfPool);
+        d.prof_class.resize(kProfPool);
+        for (auto &e : d.prof_pool) HCHECK(hipEventCreate(&e));
+    }
+    if (d.prof_used == kProfPool) prof_flush(d);
+    const int i = d.prof_used++;
+    d.prof_class[i] = cls;
+    HCHECK(hipEventRecord(d.prof_pool[2 * i], d.stream));
+    return i;
+}
 
 void upload(const void *h, void *d, size_t bytes) {
     if (bytes) HCHECK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
@@ -197,6 +233,7 @@ void upload_lists(System &s, DeviceState &d) {
 }
 
 void upload_geometry(System &s, DeviceState &d) {
+    check_coincident_sites(s);  // once per geometry, not per solve
     const double inf = std::numeric_limits<double>::infinity();
     std::vector<double4> ps(s.npol), ms(s.nmm);
     std::vector<float4> ps32(s.npol);
@@ -274,8 +311,11 @@ DeviceState &ensure_device(System &s) {
         d.nsplit = tmv_split(s.npol);
         d.mvpart_bytes = tmv_part_bytes(s.npol, 2, 64, d.nsplit);
         d.mvpart = dmalloc(d, d.mvpart_bytes);
-        d.rhs_split = std::max(1, std::min(64, (1024 + site_blocks(s.npol) - 1) / std::max(1, site_blocks(s.npol))));
-        d.rhs_split = std::min(d.rhs_split, std::max(1, (s.nmm + 2 * kBlock - 1) / (2 * kBlock)));
+        {
+            hipDeviceProp_t prop;
+            HCHECK(hipGetDeviceProperties(&prop, d.device));
+            d.rhs_split = balanced_split(site_blocks(s.npol), s.nmm, prop.multiProcessorCount);
+        }
         d.rhs_part = dalloc<double>(d, (size_t)d.rhs_split * n + 1);
         HCHECK(hipMemset(d.ipd, 0, sizeof(double) * 2 * n));
         HCHECK(hipMemset(d.e_m2d, 0, sizeof(double) * 2 * n));
@@ -292,9 +332,11 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
             double *dotpart, const CgState *skip) {
     const long n = d.n;
     if (kind == kOtf64 || kind == kOtf32) {
+        const int pi = prof_begin(d, kind);
         launch_tmv_otf(d.stream, kind == kOtf32 ? 32 : 64, s.amoeba, s.npol, M, d.ps64, d.pa64, d.ps32, d.pa32,
                        d.alpha, d.pp, x, n, y, n, dodiag, dotpart, d.mvpart, d.nsplit,
-                       reinterpret_cast<const int *>(skip));
+                       reinterpret_cast<const int *>(skip), pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
+                       pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
     } else {
         const int prec = kind == kPack32 ? 32 : 64;
         if (!d.tpack_valid || d.tpack_prec != prec) {
@@ -316,8 +358,10 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
             d.tpack_prec = prec;
             d.tpack_valid = true;
         }
+        const int pi = prof_begin(d, kind);
         launch_tpack_mv(d.stream, prec, s.npol, M, d.tpack, d.alpha, x, n, y, n, dodiag, dotpart, d.tpart,
-                        reinterpret_cast<const int *>(skip));
+                        reinterpret_cast<const int *>(skip), pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
+                        pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
     }
 }
 
@@ -343,7 +387,7 @@ CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, d
     launch_cg_init(d.stream, n, M, b, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_a, d.cs, zero_guess, d.tol_dev);
     const int nred_hp = site_blocks(s.npol);
     int it = 1;
-    int chunk = 2;
+    int chunk = 4;
     bool all_done = false;
     while (it <= maxit && !all_done) {
         const int end = std::min(maxit, it + chunk - 1);
@@ -543,6 +587,7 @@ void device_release(System &s) {
     if (d.scal_host) (void)hipHostFree(d.scal_host);
     for (auto &e : d.ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : d.prof_pool) (void)hipEventDestroy(e);
     if (d.own_stream) (void)hipStreamDestroy(d.stream);
     delete s.dev;
     s.dev = nullptr;
@@ -561,9 +606,10 @@ void prepare_polelec(System &s) {
     if (s.m2d_done) return;
     DeviceState &d = ensure_device(s);
     if (s.npol > 0) {
-        check_coincident_sites(s);
+        const int pi = prof_begin(d, 4);
         launch_rhs_m2d(d.stream, s.amoeba, s.nmm, s.npol, d.ms64, d.ma64, d.mq, d.ps64, d.pa64, d.polar_mm, d.sp,
-                       d.rhs_part, d.rhs_split, d.e_m2d);
+                       d.rhs_part, d.rhs_split, d.e_m2d, pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
+                       pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
     }
     s.m2d_done = true;
     s.host_m2d_stale = true;
@@ -717,6 +763,25 @@ double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, 
     float ms = 0;
     (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);
     return ms / repeat;
+}
+
+void profile_enable(System &s, bool on) {
+    DeviceState &d = ensure_device(s);
+    if (!on) prof_flush(d);
+    d.prof = on;
+}
+
+void profile_stats(System &s, double *out, bool reset) {
+    DeviceState &d = ensure_device(s);
+    prof_flush(d);
+    for (int k = 0; k < 5; k++) {
+        out[2 * k] = (double)d.prof_count[k];
+        out[2 * k + 1] = d.prof_ms[k];
+        if (reset) {
+            d.prof_count[k] = 0;
+            d.prof_ms[k] = 0.0;
+        }
+    }
 }
 
 // ommp_set_external_field[_nomm] (mod_interface.F90:184-233)

@@ -13,6 +13,7 @@
 // (fixed tree per block, fixed-order sum of the per-block partials).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -81,11 +82,13 @@ __device__ __forceinline__ void m2d_pair(double dx, double dy, double dz, double
         double k3 = k2 * ir2, k4 = k3 * ir2;
         const double a = fmin(ai, aj);
         const double au3 = a * u * u * u;
-        if (au3 < 50.0) {
+        const bool near = au3 < 50.0;
+        if (__ballot(near)) {
+            // au3 = +inf for a non-polarizable source: masked by the select
             const double e = exp(-au3);
-            k2 *= 1.0 - e;
-            k3 *= 1.0 - (1.0 + au3) * e;
-            k4 *= 1.0 - (1.0 + au3 + 0.6 * au3 * au3) * e;
+            k2 = near ? k2 * (1.0 - e) : k2;
+            k3 = near ? k3 * (1.0 - (1.0 + au3) * e) : k3;
+            k4 = near ? k4 * (1.0 - (1.0 + au3 + 0.6 * au3 * au3) * e) : k4;
         }
         const double md = q[1] * dx + q[2] * dy + q[3] * dz;
         // quadrupole storage xx, xy, yy, xz, yz, zz
@@ -98,9 +101,10 @@ __device__ __forceinline__ void m2d_pair(double dx, double dy, double dz, double
         ey = cd * dy - q[2] * k2 - 6.0 * qy * k3;
         ez = cd * dz - q[3] * k2 - 6.0 * qz * k3;
     } else {
-        if (u < 1.0) {
+        const bool near = u < 1.0;
+        if (__ballot(near)) {
             const double u3 = u * u * u;
-            k2 *= 4.0 * u3 - 3.0 * u3 * u;
+            k2 = near ? k2 * (4.0 * u3 - 3.0 * u3 * u) : k2;
         }
         ex = q[0] * k2 * dx;
         ey = q[0] * k2 * dy;
@@ -217,10 +221,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_finalize(int npol, int nsplit, c
 
 void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double4 *ms64,
                     const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
-                    const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d) {
+                    const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d,
+                    hipEvent_t evb, hipEvent_t eve) {
     if (npol == 0) return;
     const int chunk = (nmm + nsplit - 1) / nsplit;
     dim3 grid(site_blocks(npol), nsplit);
+    if (evb) HIP_CHECK(hipEventRecord(evb, st));
     if (amoeba)
         hipLaunchKernelGGL(k_rhs_pairs<true>, grid, dim3(kBlock), 0, st, nmm, npol, chunk, ms64, ma64, mq,
                            ps64, pa64, polar_mm, part);
@@ -228,6 +234,7 @@ void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double
         hipLaunchKernelGGL(k_rhs_pairs<false>, grid, dim3(kBlock), 0, st, nmm, npol, chunk, ms64, ma64, mq,
                            ps64, pa64, polar_mm, part);
     HIP_CHECK(hipGetLastError());
+    if (eve) HIP_CHECK(hipEventRecord(eve, st));
     if (amoeba)
         hipLaunchKernelGGL(k_rhs_finalize<true>, dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol, nsplit,
                            part, ms64, ma64, mq, ps64, pa64, sp.ri, sp.ci, sp.f2, e_m2d);
@@ -244,35 +251,99 @@ template <> struct V4<float> { using t = float4; };
 
 __device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
 __device__ __forceinline__ double fast_exp(double x) { return exp(x); }
-__device__ __forceinline__ float fast_rsqrt(float x) { return rsqrtf(x); }
+// r2 is either 0 (self pair, masked by the caller) or >= ~1 bohr^2, never
+// denormal, so the raw v_rsq_f32 (1 ulp) needs no range fix-up.
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double fast_rsqrt(double x) { return rsqrt(x); }
+__device__ __forceinline__ float fmaa(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmaa(double a, double b, double c) { return fma(a, b, c); }
 
 // Damped dipole-field kernel (maxder 2): returns k2 = lambda3 / r^3 and
 // k3 = 3 lambda5 / r^5 so that E = k3 (x.d) d - k2 x and T x = -E.
+// Damping only acts inside a few Angstrom; the exponential is evaluated under
+// a wave-uniform branch that waves without a near pair skip entirely.
 template <typename R, bool AMOEBA>
 __device__ __forceinline__ void dip_kernel(R r2, R ir, R ti, R tj, R ai, R aj, R &k2, R &k3) {
     const R ir2 = ir * ir;
     k2 = ir * ir2;
-    k3 = k2 * ir2;
-    const R u = r2 * ir * ti * tj;
+    k3 = (R(3) * k2) * ir2;
+    const R u = (r2 * ir) * (ti * tj);
     if (AMOEBA) {
         const R a = ai < aj ? ai : aj;
-        const R au3 = a * u * u * u;
-        if (au3 < R(50)) {
+        const R au3 = a * (u * u * u);
+        const bool near = au3 < R(50);
+        if (__ballot(near)) {
             const R e = fast_exp(-au3);
-            k2 *= R(1) - e;
-            k3 *= R(1) - (R(1) + au3) * e;
+            const R l3 = R(1) - e;
+            const R l5 = R(1) - (R(1) + au3) * e;
+            k2 = near ? k2 * l3 : k2;
+            k3 = near ? k3 * l5 : k3;
         }
     } else {
-        if (u < R(1)) {
+        const bool near = u < R(1);
+        if (__ballot(near)) {
             const R u3 = u * u * u, u4 = u3 * u;
-            k2 *= R(4) * u3 - R(3) * u4;
-            k3 *= u4;
+            k2 = near ? k2 * (R(4) * u3 - R(3) * u4) : k2;
+            k3 = near ? k3 * u4 : k3;
         }
     }
-    k3 *= R(3);
 }
 
+// Geometry of one pair: d = r_j - r_i, undamped k2 = 1/r^3 and k3 = 3/r^5,
+// the damping argument (a u^3 for AMOEBA, u = r/s for Wang) and whether the
+// pair needs damping at all.
+template <typename R, bool AMOEBA>
+__device__ __forceinline__ void pair_geom(const typename V4<R>::t &pj, R aj, const typename V4<R>::t &pi, R ai,
+                                          R &dx, R &dy, R &dz, R &k2, R &k3, R &arg, bool &near) {
+    dx = pj.x - pi.x;
+    dy = pj.y - pi.y;
+    dz = pj.z - pi.z;
+    const R r2 = fmaa(dz, dz, fmaa(dy, dy, dx * dx));
+    const R ir = r2 > R(0) ? fast_rsqrt(r2) : R(0);
+    const R ir2 = ir * ir;
+    k2 = ir * ir2;
+    k3 = (R(3) * k2) * ir2;
+    const R u = (r2 * ir) * (pi.w * pj.w);
+    if (AMOEBA) {
+        const R a = ai < aj ? ai : aj;
+        arg = a * (u * u * u);
+        near = arg < R(50);
+    } else {
+        arg = u;
+        near = u < R(1);
+    }
+}
+
+template <typename R, bool AMOEBA>
+__device__ __forceinline__ void pair_damp(R arg, bool near, R &k2, R &k3) {
+    if (AMOEBA) {
+        const R e = fast_exp(-arg);
+        const R l3 = R(1) - e;
+        const R l5 = R(1) - (R(1) + arg) * e;
+        k2 = near ? k2 * l3 : k2;
+        k3 = near ? k3 * l5 : k3;
+    } else {
+        const R u3 = arg * arg * arg, u4 = u3 * arg;
+        k2 = near ? k2 * (R(4) * u3 - R(3) * u4) : k2;
+        k3 = near ? k3 * u4 : k3;
+    }
+}
+
+template <typename R, int M>
+__device__ __forceinline__ void pair_accum(R (&acc)[M][3], const typename V4<R>::t *xv, R dx, R dy, R dz, R k2,
+                                           R k3) {
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const R c = k3 * fmaa(xv[m].z, dz, fmaa(xv[m].y, dy, xv[m].x * dx));
+        const R nk2 = -k2;
+        acc[m][0] = fmaa(c, dx, fmaa(nk2, xv[m].x, acc[m][0]));
+        acc[m][1] = fmaa(c, dy, fmaa(nk2, xv[m].y, acc[m][1]));
+        acc[m][2] = fmaa(c, dz, fmaa(nk2, xv[m].z, acc[m][2]));
+    }
+}
+
+// One target per thread, sources through LDS two at a time.  Tiles are padded
+// to an even count with far-away sources of zero dipole, which add exactly 0.
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kBlock) void k_tmv_pairs(int npol, int chunk,
                                                       const typename V4<R>::t *__restrict__ site,
@@ -287,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_tmv_pairs(int npol, int chunk,
         if (all) return;
     }
     __shared__ R4 s_pos[kBlock];
-    __shared__ R4 s_x[M][kBlock];  // x, y, z components of column m; w of column 0 holds a
+    __shared__ R4 s_x[M][kBlock];  // x, y, z of column m; w of column 0 holds the damping a
     const int tid = threadIdx.x;
     const int j = blockIdx.x * kBlock + tid;
     const bool valid = j < npol;
@@ -321,24 +392,38 @@ __global__ __launch_bounds__(kBlock) void k_tmv_pairs(int npol, int chunk,
                 v.w = m == 0 ? adamp[i] : R(0);
                 s_x[m][tid] = v;
             }
-        }
-        __syncthreads();
-        for (int k = 0; k < cnt; k++) {
-            const R4 pi = s_pos[k];
-            const R dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            const R r2 = dx * dx + dy * dy + dz * dz;
-            const R ir = r2 > R(0) ? fast_rsqrt(r2) : R(0);
-            const R4 x0 = s_x[0][k];
-            R k2, k3;
-            dip_kernel<R, AMOEBA>(r2, ir, pi.w, pj.w, x0.w, aj, k2, k3);
+        } else if (tid == cnt) {
+            R4 far;
+            far.x = far.y = far.z = R(1e15);
+            far.w = R(1);
+            s_pos[tid] = far;
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const R4 xv = m == 0 ? x0 : s_x[m][k];
-                const R c = k3 * (xv.x * dx + xv.y * dy + xv.z * dz);
-                acc[m][0] += c * dx - k2 * xv.x;
-                acc[m][1] += c * dy - k2 * xv.y;
-                acc[m][2] += c * dz - k2 * xv.z;
+                R4 z;
+                z.x = z.y = z.z = R(0);
+                z.w = R(0.39);
+                s_x[m][tid] = z;
             }
+        }
+        __syncthreads();
+        const int cnt2 = (cnt + 1) & ~1;
+        for (int k = 0; k < cnt2; k += 2) {
+            R4 xa[M], xb[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                xa[m] = s_x[m][k];
+                xb[m] = s_x[m][k + 1];
+            }
+            R dxa, dya, dza, k2a, k3a, arga, dxb, dyb, dzb, k2b, k3b, argb;
+            bool neara, nearb;
+            pair_geom<R, AMOEBA>(pj, aj, s_pos[k], xa[0].w, dxa, dya, dza, k2a, k3a, arga, neara);
+            pair_geom<R, AMOEBA>(pj, aj, s_pos[k + 1], xb[0].w, dxb, dyb, dzb, k2b, k3b, argb, nearb);
+            if (__ballot(neara || nearb)) {
+                pair_damp<R, AMOEBA>(arga, neara, k2a, k3a);
+                pair_damp<R, AMOEBA>(argb, nearb, k2b, k3b);
+            }
+            pair_accum<R, M>(acc, xa, dxa, dya, dza, k2a, k3a);
+            pair_accum<R, M>(acc, xb, dxb, dyb, dzb, k2b, k3b);
         }
     }
     if (valid) {
@@ -426,14 +511,39 @@ __global__ __launch_bounds__(kBlock) void k_tmv_finalize(int npol, int nsplit, c
     }
 }
 
+// Source splits for a pair kernel with tb target blocks: enough workgroups to
+// fill every CU several times, and a count that divides evenly over the CUs
+// (a partial last round of workgroups idles the rest of the chip).
+int balanced_split(int tb, int nsrc, int num_cu) {
+    const int max_s = std::max(1, (nsrc + kBlock - 1) / kBlock);  // >= one LDS tile per split
+    int best = 1;
+    double best_cost = 1e30;
+    for (int s = 1; s <= max_s; s++) {
+        const long blocks = (long)tb * s;
+        const double rounds = (double)blocks / num_cu;
+        if (rounds > 16.0) break;
+        const double eff = rounds / std::ceil(rounds);  // utilisation of the last round
+        // prefer 2..8 workgroups per CU, then balance
+        const double fill = rounds < 2.0 ? (2.0 - rounds) : 0.0;
+        const double cost = (1.0 - eff) + 0.5 * fill + 0.003 * rounds;
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = s;
+        }
+    }
+    return best;
+}
+
 int tmv_split(int npol) {
-    // aim for >= 4 workgroups per CU on 256 CUs while keeping >= 2 LDS tiles per split
-    const int tb = site_blocks(npol);
-    int s = (1024 + tb - 1) / tb;
-    const int max_s = (npol + 2 * kBlock - 1) / (2 * kBlock);
-    if (s > max_s) s = max_s;
-    if (s < 1) s = 1;
-    return s;
+    static int num_cu = 0;
+    if (num_cu == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            num_cu = prop.multiProcessorCount;
+        if (num_cu <= 0) num_cu = 256;
+    }
+    return balanced_split(site_blocks(npol), npol, num_cu);
 }
 
 size_t tmv_part_bytes(int npol, int M, int prec, int nsplit) {
@@ -444,9 +554,10 @@ template <typename R, int M>
 static void tmv_launch_t(hipStream_t st, bool amoeba, int npol, const typename V4<R>::t *site, const R *ad,
                          const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
                          const double *x, long xs, double *y, long ys, bool dodiag, double *dotpart,
-                         R *part, int nsplit, const CgState *skip, int skipM) {
+                         R *part, int nsplit, const CgState *skip, int skipM, hipEvent_t evb, hipEvent_t eve) {
     const int chunk = (npol + nsplit - 1) / nsplit;
     dim3 grid(site_blocks(npol), nsplit);
+    if (evb) HIP_CHECK(hipEventRecord(evb, st));
     if (amoeba)
         hipLaunchKernelGGL((k_tmv_pairs<R, M, true>), grid, dim3(kBlock), 0, st, npol, chunk, site, ad, x, xs,
                            part, skip, skipM);
@@ -454,6 +565,7 @@ static void tmv_launch_t(hipStream_t st, bool amoeba, int npol, const typename V
         hipLaunchKernelGGL((k_tmv_pairs<R, M, false>), grid, dim3(kBlock), 0, st, npol, chunk, site, ad, x, xs,
                            part, skip, skipM);
     HIP_CHECK(hipGetLastError());
+    if (eve) HIP_CHECK(hipEventRecord(eve, st));
     if (amoeba)
         hipLaunchKernelGGL((k_tmv_finalize<R, M, true>), dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol,
                            nsplit, part, ps64, pa64, alpha, pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag,
@@ -468,24 +580,25 @@ static void tmv_launch_t(hipStream_t st, bool amoeba, int npol, const typename V
 void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, const double4 *ps64,
                     const double *pa64, const float4 *ps32, const float *pa32, const double *alpha,
                     const CsrDev &pp, const double *x, long xstride, double *y, long ystride, bool dodiag,
-                    double *dotpart, void *part, int nsplit, const int *skip_flag) {
+                    double *dotpart, void *part, int nsplit, const int *skip_flag, hipEvent_t evb,
+                    hipEvent_t eve) {
     if (npol == 0) return;
     const CgState *skip = reinterpret_cast<const CgState *>(skip_flag);
     const int skipM = M;
     if (prec == 32) {
         if (M == 1)
             tmv_launch_t<float, 1>(st, amoeba, npol, ps32, pa32, ps64, pa64, alpha, pp, x, xstride, y, ystride,
-                                   dodiag, dotpart, (float *)part, nsplit, skip, skipM);
+                                   dodiag, dotpart, (float *)part, nsplit, skip, skipM, evb, eve);
         else
             tmv_launch_t<float, 2>(st, amoeba, npol, ps32, pa32, ps64, pa64, alpha, pp, x, xstride, y, ystride,
-                                   dodiag, dotpart, (float *)part, nsplit, skip, skipM);
+                                   dodiag, dotpart, (float *)part, nsplit, skip, skipM, evb, eve);
     } else {
         if (M == 1)
             tmv_launch_t<double, 1>(st, amoeba, npol, ps64, pa64, ps64, pa64, alpha, pp, x, xstride, y, ystride,
-                                    dodiag, dotpart, (double *)part, nsplit, skip, skipM);
+                                    dodiag, dotpart, (double *)part, nsplit, skip, skipM, evb, eve);
         else
             tmv_launch_t<double, 2>(st, amoeba, npol, ps64, pa64, ps64, pa64, alpha, pp, x, xstride, y, ystride,
-                                    dodiag, dotpart, (double *)part, nsplit, skip, skipM);
+                                    dodiag, dotpart, (double *)part, nsplit, skip, skipM, evb, eve);
     }
 }
 
@@ -752,11 +865,13 @@ void launch_tpack_build(hipStream_t st, int prec, bool amoeba, int npol, const d
 template <typename R, int M>
 static void tpack_mv_t(hipStream_t st, int npol, long nt, const void *tpack, const double *alpha, const double *x,
                        long xs, double *y, long ys, bool dodiag, double *dotpart, void *part,
-                       const CgState *skip) {
+                       const CgState *skip, hipEvent_t evb, hipEvent_t eve) {
     const long np = nt * (nt + 1) / 2;
+    if (evb) HIP_CHECK(hipEventRecord(evb, st));
     hipLaunchKernelGGL((k_tpack_mv<R, M>), dim3(np), dim3(kBlock), 0, st, npol, nt, (const R *)tpack, x, xs,
                        (R *)part, skip, M);
     HIP_CHECK(hipGetLastError());
+    if (eve) HIP_CHECK(hipEventRecord(eve, st));
     hipLaunchKernelGGL((k_tpack_finalize<R, M>), dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol, nt,
                        (const R *)part, alpha, x, xs, y, ys, (int)dodiag, dotpart, skip, M);
     HIP_CHECK(hipGetLastError());
@@ -764,16 +879,16 @@ static void tpack_mv_t(hipStream_t st, int npol, long nt, const void *tpack, con
 
 void launch_tpack_mv(hipStream_t st, int prec, int npol, int M, const void *tpack, const double *alpha,
                      const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
-                     void *part, const int *skip_flag) {
+                     void *part, const int *skip_flag, hipEvent_t evb, hipEvent_t eve) {
     if (npol == 0) return;
     const CgState *skip = reinterpret_cast<const CgState *>(skip_flag);
     const long nt = (npol + kTile - 1) / kTile;
     if (prec == 32) {
-        if (M == 1) tpack_mv_t<float, 1>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip);
-        else tpack_mv_t<float, 2>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip);
+        if (M == 1) tpack_mv_t<float, 1>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip, evb, eve);
+        else tpack_mv_t<float, 2>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip, evb, eve);
     } else {
-        if (M == 1) tpack_mv_t<double, 1>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip);
-        else tpack_mv_t<double, 2>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip);
+        if (M == 1) tpack_mv_t<double, 1>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip, evb, eve);
+        else tpack_mv_t<double, 2>(st, npol, nt, tpack, alpha, x, xstride, y, ystride, dodiag, dotpart, part, skip, evb, eve);
     }
 }
 

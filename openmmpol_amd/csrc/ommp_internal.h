@@ -108,5 +108,7 @@ void invalidate_geometry(System &s);
 const double *device_ipd(System &s);
 void refresh_host_m2d(System &s);
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat);
+void profile_enable(System &s, bool on);
+void profile_stats(System &s, double *out, bool reset);
 
 }  // namespace ommp

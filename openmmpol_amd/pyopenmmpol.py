@@ -235,6 +235,16 @@ class OMMPSystem:
                                     y.ctypes.data_as(c_dp), int(repeat))
         return y, ms
 
+    def profile_kernels(self, enable=True):
+        lib().ommp_amd_profile_kernels(self.handler, bool(enable))
+
+    def kernel_stats(self, reset=False):
+        """{class: (launches, total ms)} for otf64, otf32, incore32, incore64, static_field."""
+        out = (ctypes.c_double * 10)()
+        lib().ommp_amd_get_kernel_stats(self.handler, out, bool(reset))
+        names = ["otf64", "otf32", "incore32", "incore64", "static_field"]
+        return {nm: (int(out[2 * k]), out[2 * k + 1]) for k, nm in enumerate(names)}
+
     def set_external_field_device(self, d_ext_ptr, solver="none"):
         lib().ommp_amd_set_external_field_device(self.handler, ctypes.c_void_p(d_ext_ptr),
                                                  available_solvers[solver])

@@ -674,6 +674,12 @@ static double dot(long n, const double *a, const double *b)
     return acc;
 }
 
+/* Threshold of the |p.Tp| quick return of the CG loop.  The reference uses
+   eps_rp; golden-vector generation lowers it to converge far below the
+   reference tolerance (oracle_set_small_step_eps). */
+static double g_small_step_eps = EPS_RP;
+void oracle_set_small_step_eps(double e) { g_small_step_eps = e; }
+
 /* conjugate_gradient_solver (mod_solvers.F90:86-216).  Returns 0 on success,
    1 if not converged (the reference calls fatal_error). */
 static int cg_solve(mv_ctx *c, long n, const double *rhs, double *x, double tol, int maxit, int *iters)
@@ -694,7 +700,7 @@ static int cg_solve(mv_ctx *c, long n, const double *rhs, double *x, double tol,
     for (it = 1; it <= maxit; it++) {
         matvec(c, p, hh, 1);
         gama = dot(n, hh, p);
-        if (fabs(gama) < EPS_RP) break;
+        if (fabs(gama) < g_small_step_eps || gama == 0.0) break;
         alpha = gold / gama;
         for (long i = 0; i < n; i++) x[i] += alpha * p[i];
         for (long i = 0; i < n; i++) r[i] -= alpha * hh[i];
@@ -708,7 +714,7 @@ static int cg_solve(mv_ctx *c, long n, const double *rhs, double *x, double tol,
     }
     if (iters) *iters = it > maxit ? maxit : it;
     free(r); free(p); free(hh); free(z);
-    if (rms > tol && fabs(gama) > EPS_RP) return 1;
+    if (rms > tol && fabs(gama) > g_small_step_eps) return 1;
     return 0;
 }
 

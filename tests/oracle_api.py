@@ -59,6 +59,7 @@ def lib():
         L.oracle_energy_pol.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_charges_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_small_step_eps.argtypes = [ctypes.c_double]
         _lib = L
     return _lib
 
