@@ -1,7 +1,8 @@
 """Golden vectors for the 15,000-site sphere system (seed 2), from the oracle.
 
-Writes tests/golden/sphere15000_seed2.json: the polarization energy and
-dipole samples of a tight FP64 CG solve (tolerance 1e-12 on the reference's
+Writes tests/golden/sphere15000_seed2.json (polarization energy and dipole
+samples) and sphere15000_seed2_ipd_zero.npy (every dipole, zero external
+field) of a tight FP64 CG solve (tolerance 1e-12 on the reference's
 preconditioned rms, |p.Tp| quick return disabled), with zero external field
 and with a 30-point-charge field.  Run on the CPU (about 3 minutes on 8 cores):
 
@@ -43,6 +44,8 @@ def main():
             ipd_sum=[float(np.sum(ipd[k])) for k in range(2)],
             ipd_norm=[float(np.sqrt(np.sum(ipd[k] ** 2))) for k in range(2)])
         print(label, out["cases"][label]["epol"], o.iters, f"{time.time() - t:.1f}s", flush=True)
+        if label == "zero":
+            np.save(os.path.join(HERE, f"sphere{N}_seed{SEED}_ipd_zero.npy"), ipd)
     out["qm_charges"] = dict(pos_bohr=cq.tolist(), q=qq.tolist())
     with open(os.path.join(HERE, f"sphere{N}_seed{SEED}.json"), "w") as f:
         json.dump(out, f)
