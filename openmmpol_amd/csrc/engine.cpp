@@ -375,7 +375,7 @@ struct CgResult {
 };
 
 CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, double *x, const double *tol,
-                  int maxit, bool zero_guess, double small_step_eps) {
+                  int maxit, bool zero_guess, double small_step_eps, int slot) {
     const long n = d.n;
     CgResult res;
     double te[4] = {tol[0], M > 1 ? tol[1] : tol[0], small_step_eps, small_step_eps};
@@ -387,7 +387,9 @@ CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, d
     launch_cg_init(d.stream, n, M, b, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_a, d.cs, zero_guess, d.tol_dev);
     const int nred_hp = site_blocks(s.npol);
     int it = 1;
-    int chunk = 4;
+    // first batch: as many iterations as this call site needed last time
+    const int pred = (slot >= 0 && slot < 16) ? s.pred_iters[slot] : 0;
+    int chunk = pred > 0 ? pred : 4;
     bool all_done = false;
     while (it <= maxit && !all_done) {
         const int end = std::min(maxit, it + chunk - 1);
@@ -400,8 +402,11 @@ CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, d
         HCHECK(hipStreamSynchronize(d.stream));
         all_done = true;
         for (int m = 0; m < M; m++) all_done = all_done && d.cs_host[m].done;
-        chunk = std::min(8, chunk * 2);
+        chunk = 2;
     }
+    int most = 0;
+    for (int m = 0; m < M; m++) most = std::max(most, d.cs_host[m].done ? d.cs_host[m].iters : maxit);
+    if (slot >= 0 && slot < 16) s.pred_iters[slot] = most;
     for (int m = 0; m < M; m++) {
         res.iters[m] = d.cs_host[m].done ? d.cs_host[m].iters : maxit;
         res.status[m] = d.cs_host[m].done ? d.cs_host[m].status : 2;
@@ -540,7 +545,11 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
     const long n = d.n;
     MixedResult res;
     const int max_outer = 30;
-    const double eta = 2e-4;  // inner relative reduction per refinement step
+    // Inner target per refinement step: straight to the tolerance when the
+    // FP32 operator can deliver it, otherwise a 1e-5 reduction (the FP32 T
+    // matvec is accurate to ~1e-6 relative, so deeper inner solves would not
+    // lower the FP64 residual further).
+    const double eta_min = 1e-5;
     bool xzero = zero_guess;
     for (int outer = 0; outer < max_outer; outer++) {
         // FP64 residual r = b - T x and its preconditioned rms per column
@@ -559,14 +568,14 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
         for (int m = 0; m < M; m++) {
             const double rms = d.scal_host[m];
             conv = conv && rms < tol;
-            tin[m] = rms < tol ? 1e300 : std::max(eta * rms, 0.5 * tol);
+            tin[m] = rms < tol ? 1e300 : std::max(eta_min * rms, 0.5 * tol);
         }
         if (conv) {
             res.ok = true;
             return res;
         }
         // inner FP32 PCG on T delta = r from delta = 0
-        CgResult cr = cg_solve(s, d, kind32, M, d.vr2, d.vd, tin, maxit, true, 0.0);
+        CgResult cr = cg_solve(s, d, kind32, M, d.vr2, d.vd, tin, maxit, true, 0.0, 2 + std::min(outer, 13));
         res.inner += std::max(cr.iters[0], M > 1 ? cr.iters[1] : 0);
         launch_axpy(d.stream, n, M, 1.0, d.vd, x);
         xzero = false;
@@ -670,7 +679,8 @@ void polarization(System &s, const double *d_field, int solver, int matv, int ma
             CgState *cs_save = d.cs;
             d.cs = cs_save + k;
             double tol = s.tol;
-            CgResult r = cg_solve(s, d, kind, 1, d.vb + k * n, d.vx + k * n, &tol, s.max_iter, false, kEpsRp);
+            CgResult r = cg_solve(s, d, kind, 1, d.vb + k * n, d.vx + k * n, &tol, s.max_iter, false, kEpsRp,
+                                  s.ipd_use_guess ? -1 : k);
             d.cs = cs_save;
             s.last_iters[k] = r.iters[0];
             if (r.status[0] == 1) s.last_status = 1;

@@ -72,6 +72,10 @@ struct System {
     int max_iter = 200;
     int precision = 0;             // OMMP_AMD_PREC_*
     int last_iters[4] = {0, 0, 0, 0};
+    // iterations each CG call site took last time: the first batch of
+    // iterations is enqueued without a host round trip (slot: 0/1 FP64 D/P,
+    // 2.. mixed inner solves by refinement step)
+    int pred_iters[16] = {0};
     int last_status = 0;
     double last_ms[3] = {0, 0, 0};
     void *user_stream = nullptr;
