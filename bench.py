@@ -27,12 +27,43 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/s @15k sites, mixed vs FP64 at ≤1e-9 rel E_pol err; p50/p99 solve latency"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
 N_SITES, RADIUS, SEED = 15000, 33.0, 2
 
 
 def pair_flops(n, m):
     """Ordered-pair on-the-fly matvec: N(N-1)(29 + 18m) FLOP per launch."""
     return n * (n - 1) * (29 + 18 * m)
+
+
+def incore_bytes(n, m, s):
+    """Packed symmetric T streamed once per matvec plus the vectors in and out:
+    [N(N-1)/2 * 6 + N] * s + 2 * 3N * m * s bytes."""
+    return (n * (n - 1) // 2 * 6 + n) * s + 2 * 3 * n * m * s
+
+
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "pair_kernels_sphere15000.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch") and kernel in line:
+            return float(line.split()[-1])
+    return None
+
+
+def run_incore(spec, repeat):
+    """Packed in-core FP32 T (two columns) streamed from HBM: mean device time per launch."""
+    from openmmpol_amd import synth
+    s = synth.make_system(spec)
+    x = np.random.default_rng(1).normal(size=(2, s.pol_atoms, 3))
+    s.profile_kernels(True)
+    s.kernel_stats(reset=True)
+    _, ms = s.tmatvec(x, kind="incore32", repeat=repeat)
+    calls, kms = s.kernel_stats(reset=True)["incore32"]
+    s.terminate()
+    return ms, kms / max(calls, 1)
 
 
 def load_golden():
@@ -114,6 +145,7 @@ def main():
                     help="solver tolerance on the preconditioned residual rms")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
+    ap.add_argument("--no-incore", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -138,6 +170,10 @@ def main():
         fp64 = run_mode(spec, "fp64", args.tol, max(3, args.steps // 4), 1, d_zero.value)
         print(f"[bench] fp64: {len(fp64['lat']) / fp64['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
     hip.hipFree(d_zero)
+    incore = None
+    if not args.no_incore:
+        incore = run_incore(spec, 20)
+        print(f"[bench] incore32 matvec: {incore[1]:.3f} ms per stream launch", file=sys.stderr, flush=True)
 
     golden = load_golden()
     ref_e = golden["cases"]["zero"]["epol"] if golden else None
@@ -177,11 +213,19 @@ def main():
         "mu_maxabs_err_mixed": mu_err(mixed["ipd"]),
         "roofline": {"bound": "valu", "kernel": "k_tmv_pairs<float,2,AMOEBA> (on-the-fly FP32 pair kernel)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
+                     "traffic": profile_traffic("k_tmv_pairs<float,2,AMOEBA>"),
                      "launches": calls, "avg_ms": avg_ms, "flop_per_launch": flops},
         "kernel_time_ms": {k: v[1] for k, v in mixed["stats"].items()},
         "kernel_launches": {k: v[0] for k, v in mixed["stats"].items()},
     }
+    if incore is not None:
+        b = incore_bytes(n, 2, 4)
+        out["incore_fp32_matvec"] = {
+            "kernel": "k_tpack_mv<float,2> (packed symmetric T, 64x64 site tiles)", "bound": "hbm",
+            "avg_ms_stream_kernel": incore[1], "avg_ms_with_finalize": incore[0], "bytes_per_launch": b,
+            "achieved": b / (incore[1] * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": b / (incore[1] * 1e-3) / 1e9 / PEAK_HBM_GBS}
     if fp64 is not None:
         v64 = len(fp64["lat"]) / fp64["elapsed"]
         c64, ms64 = fp64["stats"]["otf64"]

@@ -603,12 +603,15 @@ void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, cons
 }
 
 // ===================================================== packed in-core T
-// Upper tile pairs (I <= J) of 64x64 sites; per pair 6 planes (xx, xy, xz, yy,
-// yz, zz) of 4096 entries; thread t of the 256-thread block owns the 4x4 sub-
-// block rows 4*(t>>4).., cols 4*(t&15).. stored as 16 contiguous entries, so
-// each thread streams 64 B (FP32) per plane with fully coalesced loads.
+// Upper tile pairs (I <= J) of 64x64 sites, 6 tensor planes (xx, xy, xz, yy,
+// yz, zz).  Thread t of the 256-thread block owns the 4x4 sub-block of rows
+// 4*(t>>4).. and columns 4*(t&15)..  Inside a tile pair the entries are ordered
+// [k][t][4], k = 6 * (row within the sub-block) + plane: one 16-byte (FP32)
+// vector per thread and step, so every wave-wide load of a step reads one
+// contiguous 1 KB (FP32) or 2 KB (FP64) run.
 constexpr int kTile = 64;
 constexpr int kTileEntries = kTile * kTile;
+constexpr int kTpackGroups = 16;  // source-tile groups of the first reduction stage
 
 __host__ __device__ inline long tile_pair_index(long I, long J, long nt) { return I * nt - I * (I - 1) / 2 + (J - I); }
 
@@ -624,6 +627,11 @@ __device__ inline void tile_pair_from_index(long p, long nt, int &I, int &J) {
     J = (int)(lo + (p - tile_pair_index(lo, lo, nt)));
 }
 
+__host__ __device__ inline long tpack_offset(int a, int b, int e) {
+    const int t = (a >> 2) * 16 + (b >> 2);
+    return ((long)((a & 3) * 6 + e) * kBlock + t) * 4 + (b & 3);
+}
+
 size_t tpack_bytes(int npol, int prec) {
     const long nt = (npol + kTile - 1) / kTile;
     return (size_t)(nt * (nt + 1) / 2) * 6 * kTileEntries * (prec == 32 ? 4 : 8);
@@ -632,16 +640,18 @@ size_t tpack_bytes(int npol, int prec) {
 template <typename R, bool AMOEBA>
 __global__ __launch_bounds__(kBlock) void k_tpack_build(int npol, long nt, const double4 *__restrict__ ps,
                                                         const double *__restrict__ pa, R *__restrict__ tp) {
+    using R4 = typename V4<R>::t;
     int I, J;
     tile_pair_from_index(blockIdx.x, nt, I, J);
     const int t = threadIdx.x;
     const int rg = t >> 4, cg = t & 15;
-    R *base = tp + (long)blockIdx.x * 6 * kTileEntries + t * 16;
-    for (int ai = 0; ai < 4; ai++)
+    R4 *base = reinterpret_cast<R4 *>(tp + (long)blockIdx.x * 6 * kTileEntries) + t;
+    for (int ai = 0; ai < 4; ai++) {
+        R v[6][4];
         for (int bj = 0; bj < 4; bj++) {
             const int i = I * kTile + rg * 4 + ai;
             const int j = J * kTile + cg * 4 + bj;
-            double v[6] = {0, 0, 0, 0, 0, 0};
+            for (int e = 0; e < 6; e++) v[e][bj] = R(0);
             if (i < npol && j < npol && i != j) {
                 const double4 pi = ps[i], pj = ps[j];
                 const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
@@ -649,15 +659,23 @@ __global__ __launch_bounds__(kBlock) void k_tpack_build(int npol, long nt, const
                 double k2, k3;
                 dip_kernel<double, AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, pa[i], pa[j], k2, k3);
                 // T_ij = k2 I - 3 k3 d d^T (k3 already carries the factor 3)
-                v[0] = k2 - k3 * dx * dx;
-                v[1] = -k3 * dx * dy;
-                v[2] = -k3 * dx * dz;
-                v[3] = k2 - k3 * dy * dy;
-                v[4] = -k3 * dy * dz;
-                v[5] = k2 - k3 * dz * dz;
+                v[0][bj] = (R)(k2 - k3 * dx * dx);
+                v[1][bj] = (R)(-k3 * dx * dy);
+                v[2][bj] = (R)(-k3 * dx * dz);
+                v[3][bj] = (R)(k2 - k3 * dy * dy);
+                v[4][bj] = (R)(-k3 * dy * dz);
+                v[5][bj] = (R)(k2 - k3 * dz * dz);
             }
-            for (int e = 0; e < 6; e++) base[e * kTileEntries + ai * 4 + bj] = (R)v[e];
         }
+        for (int e = 0; e < 6; e++) {
+            R4 w;
+            w.x = v[e][0];
+            w.y = v[e][1];
+            w.z = v[e][2];
+            w.w = v[e][3];
+            base[(ai * 6 + e) * kBlock] = w;
+        }
+    }
 }
 
 // Scale listed pairs (dense screening rule, create_tmat semantics).
@@ -670,21 +688,24 @@ __global__ void k_tpack_scale(int npol, long nt, const int *__restrict__ ri, con
         const int j = ci[k];
         const int I = i / kTile, J = j / kTile;
         if (I > J) continue;  // stored once, in tile (J, I), handled by row j
-        const int a = i % kTile, b = j % kTile;
-        const int t = (a >> 2) * 16 + (b >> 2);
-        R *p = tp + tile_pair_index(I, J, nt) * 6 * kTileEntries + t * 16 + (a & 3) * 4 + (b & 3);
-        for (int e = 0; e < 6; e++) p[e * kTileEntries] = (R)((double)p[e * kTileEntries] * sv[k]);
+        R *p = tp + tile_pair_index(I, J, nt) * 6 * kTileEntries;
+        for (int e = 0; e < 6; e++) {
+            const long o = tpack_offset(i % kTile, j % kTile, e);
+            p[o] = (R)((double)p[o] * sv[k]);
+        }
     }
 }
 
 size_t tpack_part_bytes(int npol, int M) {
     const long nt = (npol + kTile - 1) / kTile;
-    return (size_t)nt * M * 3 * (size_t)(nt * kTile) * 8;  // sized for FP64 partials
+    // per-source-tile partials (sized for FP64) + the grouped FP64 first stage
+    return (size_t)nt * M * 3 * (size_t)(nt * kTile) * 8 + (size_t)kTpackGroups * M * 3 * (size_t)npol * 8;
 }
 
 // One tile pair per block.  Row sums go to part[J][.][rows of I], column sums
 // to part[I][.][rows of J]: every (source tile, target) slot is written by
-// exactly one block, so the finalize sum is deterministic.
+// exactly one block, so the reduction is deterministic.  The next sub-block
+// row of T is loaded while the current one is used.
 template <typename R, int M>
 __global__ __launch_bounds__(kBlock) void k_tpack_mv(int npol, long nt, const R *__restrict__ tp,
                                                      const double *__restrict__ x, long xstride,
@@ -702,6 +723,10 @@ __global__ __launch_bounds__(kBlock) void k_tpack_mv(int npol, long nt, const R 
     __shared__ R scol[4][M * 3][kTile];
     const int t = threadIdx.x;
     const long npad = nt * kTile;
+    const R4 *tv = reinterpret_cast<const R4 *>(tp + (long)blockIdx.x * 6 * kTileEntries) + t;
+    R4 cur[6];
+#pragma unroll
+    for (int e = 0; e < 6; e++) cur[e] = tv[e * kBlock];
     if (t < kTile) {
         const int i = I * kTile + t, j = J * kTile + t;
         for (int m = 0; m < M; m++)
@@ -712,7 +737,18 @@ __global__ __launch_bounds__(kBlock) void k_tpack_mv(int npol, long nt, const R 
     }
     __syncthreads();
     const int rg = t >> 4, cg = t & 15;
-    const R *base = tp + (long)blockIdx.x * 6 * kTileEntries + t * 16;
+    // FP32 keeps the four source dipoles of the thread's columns in registers;
+    // FP64 reads them from LDS (register budget).
+    constexpr bool kCacheX = sizeof(R) == 4;
+    R xb[kCacheX ? 4 : 1][M][3];
+    if constexpr (kCacheX) {
+#pragma unroll
+        for (int bj = 0; bj < 4; bj++)
+#pragma unroll
+            for (int m = 0; m < M; m++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) xb[bj][m][c] = sxJ[m][c][cg * 4 + bj];
+    }
     R col[4][M][3];
 #pragma unroll
     for (int b = 0; b < 4; b++)
@@ -720,30 +756,47 @@ __global__ __launch_bounds__(kBlock) void k_tpack_mv(int npol, long nt, const R 
         for (int m = 0; m < M; m++)
 #pragma unroll
             for (int c = 0; c < 3; c++) col[b][m][c] = R(0);
-#pragma unroll 1
+    // FP32: fully unrolled (all loads of the tile issued early, ~160 VGPRs);
+    // FP64: one row at a time with the next row in flight (bounded registers).
+    constexpr int kUnroll = sizeof(R) == 4 ? 4 : 1;
+#pragma unroll kUnroll
     for (int ai = 0; ai < 4; ai++) {
-        R4 tv[6];
+        R4 nxt[6];
+        if (ai < 3) {
 #pragma unroll
-        for (int e = 0; e < 6; e++) tv[e] = *reinterpret_cast<const R4 *>(base + e * kTileEntries + ai * 4);
+            for (int e = 0; e < 6; e++) nxt[e] = tv[((ai + 1) * 6 + e) * kBlock];
+        }
         const int a = rg * 4 + ai;
+        R xa[M][3];
+#pragma unroll
+        for (int m = 0; m < M; m++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) xa[m][c] = sxI[m][c][a];
         R row[M][3];
 #pragma unroll
         for (int m = 0; m < M; m++) row[m][0] = row[m][1] = row[m][2] = R(0);
 #pragma unroll
         for (int bj = 0; bj < 4; bj++) {
-            const R txx = (&tv[0].x)[bj], txy = (&tv[1].x)[bj], txz = (&tv[2].x)[bj];
-            const R tyy = (&tv[3].x)[bj], tyz = (&tv[4].x)[bj], tzz = (&tv[5].x)[bj];
-            const int b = cg * 4 + bj;
+            const R txx = (&cur[0].x)[bj], txy = (&cur[1].x)[bj], txz = (&cur[2].x)[bj];
+            const R tyy = (&cur[3].x)[bj], tyz = (&cur[4].x)[bj], tzz = (&cur[5].x)[bj];
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                const R xb0 = sxJ[m][0][b], xb1 = sxJ[m][1][b], xb2 = sxJ[m][2][b];
-                row[m][0] += txx * xb0 + txy * xb1 + txz * xb2;
-                row[m][1] += txy * xb0 + tyy * xb1 + tyz * xb2;
-                row[m][2] += txz * xb0 + tyz * xb1 + tzz * xb2;
-                const R xa0 = sxI[m][0][a], xa1 = sxI[m][1][a], xa2 = sxI[m][2][a];
-                col[bj][m][0] += txx * xa0 + txy * xa1 + txz * xa2;
-                col[bj][m][1] += txy * xa0 + tyy * xa1 + tyz * xa2;
-                col[bj][m][2] += txz * xa0 + tyz * xa1 + tzz * xa2;
+                R xb0, xb1, xb2;
+                if constexpr (kCacheX) {
+                    xb0 = xb[bj][m][0];
+                    xb1 = xb[bj][m][1];
+                    xb2 = xb[bj][m][2];
+                } else {
+                    xb0 = sxJ[m][0][cg * 4 + bj];
+                    xb1 = sxJ[m][1][cg * 4 + bj];
+                    xb2 = sxJ[m][2][cg * 4 + bj];
+                }
+                row[m][0] = fmaa(txz, xb2, fmaa(txy, xb1, fmaa(txx, xb0, row[m][0])));
+                row[m][1] = fmaa(tyz, xb2, fmaa(tyy, xb1, fmaa(txy, xb0, row[m][1])));
+                row[m][2] = fmaa(tzz, xb2, fmaa(tyz, xb1, fmaa(txz, xb0, row[m][2])));
+                col[bj][m][0] = fmaa(txz, xa[m][2], fmaa(txy, xa[m][1], fmaa(txx, xa[m][0], col[bj][m][0])));
+                col[bj][m][1] = fmaa(tyz, xa[m][2], fmaa(tyy, xa[m][1], fmaa(txy, xa[m][0], col[bj][m][1])));
+                col[bj][m][2] = fmaa(tzz, xa[m][2], fmaa(tyz, xa[m][1], fmaa(txz, xa[m][0], col[bj][m][2])));
             }
         }
         // row sum over the 16 contiguous lanes of the row group
@@ -762,6 +815,10 @@ __global__ __launch_bounds__(kBlock) void k_tpack_mv(int npol, long nt, const R 
             const long tgt = (long)I * kTile + a;
             for (int m = 0; m < M; m++)
                 for (int c = 0; c < 3; c++) part[((long)J * M + m) * 3 * npad + 3 * tgt + c] = row[m][c];
+        }
+        if (ai < 3) {
+#pragma unroll
+            for (int e = 0; e < 6; e++) cur[e] = nxt[e];
         }
     }
     if (I == J) return;  // diagonal tiles hold both orders: rows only (uniform per block)
@@ -792,8 +849,37 @@ __global__ __launch_bounds__(kBlock) void k_tpack_mv(int npol, long nt, const R 
     }
 }
 
+// First reduction stage: source tiles of group blockIdx.y summed in order, in FP64.
 template <typename R, int M>
-__global__ __launch_bounds__(kBlock) void k_tpack_finalize(int npol, long nt, const R *__restrict__ part,
+__global__ __launch_bounds__(kBlock) void k_tpack_reduce(int npol, long nt, const R *__restrict__ part,
+                                                         double *__restrict__ red, const CgState *__restrict__ skip,
+                                                         int skipM) {
+    if (skip) {
+        bool all = true;
+        for (int m = 0; m < skipM; m++) all = all && skip[m].done;
+        if (all) return;
+    }
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= npol) return;
+    const long npad = nt * kTile;
+    const long chunk = (nt + kTpackGroups - 1) / kTpackGroups;
+    const long s0 = blockIdx.y * chunk, s1 = s0 + chunk < nt ? s0 + chunk : nt;
+    double acc[M][3];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
+    for (long s = s0; s < s1; s++)
+#pragma unroll
+        for (int m = 0; m < M; m++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[m][c] += (double)part[(s * M + m) * 3 * npad + 3L * j + c];
+#pragma unroll
+    for (int m = 0; m < M; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) red[((long)blockIdx.y * M + m) * 3 * npol + 3L * j + c] = acc[m][c];
+}
+
+template <int M>
+__global__ __launch_bounds__(kBlock) void k_tpack_finalize(int npol, const double *__restrict__ red,
                                                            const double *__restrict__ alpha,
                                                            const double *__restrict__ x, long xstride,
                                                            double *__restrict__ y, long ystride, int dodiag,
@@ -807,15 +893,15 @@ __global__ __launch_bounds__(kBlock) void k_tpack_finalize(int npol, long nt, co
     __shared__ double sh[4];
     const int j = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = j < npol;
-    const long npad = nt * kTile;
     double acc[M][3];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
     if (valid) {
-        for (long s = 0; s < nt; s++)
+        for (int g = 0; g < kTpackGroups; g++)
 #pragma unroll
             for (int m = 0; m < M; m++)
-                for (int c = 0; c < 3; c++) acc[m][c] += (double)part[(s * M + m) * 3 * npad + 3L * j + c];
+#pragma unroll
+                for (int c = 0; c < 3; c++) acc[m][c] += red[((long)g * M + m) * 3 * npol + 3L * j + c];
         if (dodiag) {
             const double ia = 1.0 / alpha[j];
             for (int m = 0; m < M; m++)
@@ -867,13 +953,17 @@ static void tpack_mv_t(hipStream_t st, int npol, long nt, const void *tpack, con
                        long xs, double *y, long ys, bool dodiag, double *dotpart, void *part,
                        const CgState *skip, hipEvent_t evb, hipEvent_t eve) {
     const long np = nt * (nt + 1) / 2;
+    double *red = reinterpret_cast<double *>(static_cast<char *>(part) +
+                                             (size_t)nt * M * 3 * (size_t)(nt * kTile) * 8);
     if (evb) HIP_CHECK(hipEventRecord(evb, st));
     hipLaunchKernelGGL((k_tpack_mv<R, M>), dim3(np), dim3(kBlock), 0, st, npol, nt, (const R *)tpack, x, xs,
                        (R *)part, skip, M);
     HIP_CHECK(hipGetLastError());
     if (eve) HIP_CHECK(hipEventRecord(eve, st));
-    hipLaunchKernelGGL((k_tpack_finalize<R, M>), dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol, nt,
-                       (const R *)part, alpha, x, xs, y, ys, (int)dodiag, dotpart, skip, M);
+    hipLaunchKernelGGL((k_tpack_reduce<R, M>), dim3(site_blocks(npol), kTpackGroups), dim3(kBlock), 0, st, npol,
+                       nt, (const R *)part, red, skip, M);
+    hipLaunchKernelGGL((k_tpack_finalize<M>), dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol, red, alpha, x,
+                       xs, y, ys, (int)dodiag, dotpart, skip, M);
     HIP_CHECK(hipGetLastError());
 }
 
