@@ -115,6 +115,63 @@ void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset);
    out[1] solver, out[2] energy/copy-back.  out must hold 3 entries. */
 void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out);
 
+/* ---------------------------------------------------------------- batched */
+/* Several external fields of one system solved together: nrhs right-hand sides
+   E_M2D + ext[k], each with its D and P sets (AMOEBA) or its single set, by the
+   mixed solver (FP32 block T matvec, FP64 refinement per column) with every
+   launch covering all columns.  The reference has no such call; it is the
+   batched form of ommp_set_external_field (mod_interface.F90:184-223) for QM
+   codes that need the response to many fields (e.g. several excited states or
+   finite-difference displacements).  The guess of every column is the
+   system's current dipoles when a previous solve left them, else zero; the
+   system's own dipoles are not changed.  Tolerance: ommp_amd_set_solver_tolerance.
+     ext   [nrhs][pol_atoms][3]
+     ipd   [nrhs][n_ipd][pol_atoms][3] output, or NULL
+     epol  [nrhs] polarization energies (host), or NULL */
+void ommp_amd_set_external_fields_multi(OMMP_SYSTEM_PRT s, int32_t nrhs, const double *ext, double *ipd,
+                                        double *epol);
+/* Same with ext and ipd in device memory (epol stays a host array). */
+void ommp_amd_set_external_fields_multi_device(OMMP_SYSTEM_PRT s, int32_t nrhs, const double *d_ext,
+                                               double *d_ipd, double *epol);
+/* Field of nq point charges (x, y, z, q in bohr and e; [nq][4]) at the
+   polarizable sites, undamped and unscreened, as the reference's E_n2p
+   (mod_qm_helper.F90:376-391); out [pol_atoms][3] on the host. */
+void ommp_amd_charges_field(OMMP_SYSTEM_PRT s, int32_t nq, const double *charges, double *out);
+
+/* ---------------------------------------------------------------- serving */
+/* A serving loop on the current GPU: a worker thread that admits requests
+   while batch slots (max_active) and the HBM budget (bytes, counted as the
+   device memory of each admitted system plus its batch state) allow, and runs
+   all admitted solves together in ticks of the batched mixed solver.  A request
+   is one system and an SCF sequence of nsteps solves; the external field of
+   step k is that of the point charges charges[k][nq][4] (x, y, z, q).  Each
+   solve starts from the previous step's dipoles.  Solves that converge leave
+   the batch at the next tick and the request's next solve takes their place;
+   requests that arrive while others run are admitted as slots and memory free
+   up.  All systems of one server must use the same force field.  Tolerance
+   and iteration limits are each system's (ommp_amd_set_solver_tolerance).
+   While a request is pending or active its system must not be used by other
+   calls; when it completes, the system's dipoles are those of its last solve. */
+typedef void *OMMP_AMD_SERVER_PTR;
+OMMP_AMD_SERVER_PTR ommp_amd_server_create(double hbm_budget_bytes, int32_t max_active);
+/* Returns the request id. */
+int32_t ommp_amd_server_submit(OMMP_AMD_SERVER_PTR srv, OMMP_SYSTEM_PRT s, int32_t nsteps, int32_t nq,
+                               const double *charges);
+/* Blocks until request id has completed (id < 0: every submitted request). */
+void ommp_amd_server_wait(OMMP_AMD_SERVER_PTR srv, int32_t id);
+/* Results of a completed request: energies[nsteps] (Hartree), latency_ms[nsteps]
+   (from the time the solve could start -- arrival for the first -- to the time
+   its convergence was seen), iters[nsteps][2] (FP64 refinement steps, FP32
+   iterations); any pointer may be NULL.  Returns 0 when every solve
+   converged, 2 otherwise, -1 if the request has not completed. */
+int32_t ommp_amd_server_result(OMMP_AMD_SERVER_PTR srv, int32_t id, double *energies, double *latency_ms,
+                               int32_t *iters);
+/* out[6]: ticks, solves, mean columns in flight per tick, most requests admitted
+   at once, peak admitted bytes, admission deferrals. */
+void ommp_amd_server_stats(OMMP_AMD_SERVER_PTR srv, double *out);
+/* Waits for every submitted request, then stops the worker. */
+void ommp_amd_server_destroy(OMMP_AMD_SERVER_PTR srv);
+
 #ifdef __cplusplus
 }
 #endif

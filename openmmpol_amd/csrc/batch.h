@@ -1,0 +1,139 @@
+// Batched solver: many right-hand-side columns, of one system or of several
+// systems of different sizes, solved together by the mixed-precision scheme
+// (FP32 T matvec inside FP64 iterative refinement) with every launch covering
+// every column of the batch.
+//
+// A column is one dipole set (D or P) of one solve.  Columns of a system are
+// stored [column][3 * npol] in that system's segment.  Each column carries its
+// own CG scalars and a phase:
+//   OUTER  next tick computes r = b - T x in FP64 and tests convergence;
+//   INNER  next tick runs one FP32-matvec PCG iteration on T d = r;
+//   DONE   converged (or stopped), skipped by every kernel.
+// One tick = [FP64 matvec, residual, decide] for OUTER columns followed by
+// [FP32 matvec, update, direction] for INNER columns.  A column's arithmetic
+// depends only on its own data and on its segment's fixed launch geometry, so
+// its result is the same whatever else is in the batch.
+//
+// Phase and scalars are double-buffered: the two kernels that change them
+// (decide and direction) read copy 0 / write copy 1 and read copy 1 / write
+// copy 0, so no block ever sees a value written by another block of the same
+// launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "device.h"
+
+namespace ommp {
+
+constexpr int kPhaseDone = 0, kPhaseOuter = 1, kPhaseInner = 2;
+constexpr int kBatchMaxOuter = 30;
+
+struct BColSub {
+    double g;        // inner r.z
+    double tin;      // inner target rms
+    double rms;      // last outer (FP64) preconditioned residual rms
+    int phase;
+    int it;          // iteration within the current inner solve
+    int outer;       // outer steps taken
+    int inner_total; // inner iterations summed over outer steps
+    int status;      // 0 converged, 2 not converged
+    int pad;
+};
+
+struct BCol {
+    BColSub s[2];
+    double tol;
+    int maxit;       // inner iterations per refinement step
+    int xzero;       // x is exactly zero (first step of a cold start)
+    int seg;
+    int col;         // column within the segment
+};
+
+// One system in a batch; device pointers only.
+struct BSeg {
+    int npol, ncols, nsplit, chunk;
+    int tblocks;     // ceil(npol / kBlock): pair/finalize target blocks
+    int vblocks;     // ceil(3 npol / kBlock): vector blocks per column
+    int col0;        // first batch column of this segment
+    int amoeba;
+    const float4 *ps32;
+    const float *pa32;
+    const double4 *ps64;
+    const double *pa64;
+    const double *alpha;
+    const int *pp_ri;
+    const int *pp_ci;
+    const double *pp_f;
+    double *b, *x, *r, *z, *p, *h, *d;  // [ncols][3 npol]
+    void *part;                          // [nsplit][ncols][3 npol], FP64-sized
+    double *red_hp;                      // [ncols][tblocks]
+    double *red_rz;                      // [ncols][vblocks]
+};
+
+// Work lists (built on the host when the batch composition changes):
+//   pair items     int4 {segment, target block, source split, column group}
+//   finalize items int4 {segment, target block, column group, 0}
+//   vector items   int4 {segment, column in segment, vector block, batch column}
+struct BatchTables {
+    const BSeg *segs;
+    BCol *cols;
+    const int4 *pair32;
+    int npair32;
+    const int4 *fin32;
+    int nfin32;
+    const int4 *pair64;
+    int npair64;
+    const int4 *fin64;
+    int nfin64;
+    const int4 *vec;
+    int nvec;
+};
+
+constexpr int kBatchM32 = 8;  // columns per FP32 pair work item (multi-RHS)
+constexpr int kBatchM64 = 4;  // columns per FP64 pair work item (multi-RHS)
+
+// Column-group width used for a segment of ncols columns.
+inline int batch_m32(int ncols) { return ncols <= 2 ? 2 : kBatchM32; }
+inline int batch_m64(int ncols) { return ncols <= 2 ? 2 : kBatchM64; }
+
+// One tick over every column of the batch (see the file comment).
+// m32/m64: column-group widths of this batch (all segments share them).
+void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m32, int m64, double eta);
+
+// b[col] = E_M2D[set of col] + ext[rhs of col] for columns [c0, c0 + ncols) of a
+// segment whose columns are ordered (rhs, set); ext [nrhs][3 npol] on device.
+void launch_batch_rhs(hipStream_t st, long n, int nsets, int ncols, const double *e_m2d, const double *ext,
+                      double *b);
+// Field of point charges (x, y, z, q) at the polarizable sites, undamped and
+// unscreened (E_n2p, mod_qm_helper.F90:376-391); ext[3 npol].
+void launch_charges_field(hipStream_t st, int npol, const double4 *ps64, int nq, const double4 *charges,
+                          double *ext);
+// Start (or restart) columns [c0, c0 + nc) of the batch column array: phase
+// OUTER, counters cleared, tolerance set; xzero marks a zero initial guess.
+void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int xzero);
+// E_pol per right-hand side, -1/2 mu_D . E_M2D,P (AMOEBA) or -1/2 mu . E_M2D:
+// x columns ordered (rhs, set); red scratch [nrhs][vblocks]; out[nrhs] on device.
+void launch_batch_energy(hipStream_t st, long n, int nsets, int nrhs, const double *x, const double *e_m2d,
+                         double *red, double *out);
+
+// Device arrays of one system that the batched engine reads (engine.cpp).
+struct DeviceView {
+    int device;
+    hipStream_t stream;
+    int npol, amoeba, n_ipd;
+    const double4 *ps64;
+    const double *pa64;
+    const float4 *ps32;
+    const float *pa32;
+    const double *alpha;
+    CsrDev pp;
+    const double *e_m2d;
+    double *ipd;
+    size_t bytes;    // device memory held by the system
+};
+
+}  // namespace ommp

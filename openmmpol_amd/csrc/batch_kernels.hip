@@ -1,0 +1,491 @@
+// Kernels of the batched solver (batch.h): grouped pair kernels whose work
+// items span several systems and column groups, and per-column PCG vector
+// kernels driven by work lists.  Arithmetic per pair and per element is that of
+// the single-system kernels (pair_math.h), so a column gives the same numbers
+// in any batch.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "batch.h"
+#include "pair_math.h"
+
+namespace ommp {
+
+#define HIP_CHECK(x)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                      \
+            std::abort();                                                                \
+        }                                                                                \
+    } while (0)
+
+template <typename R> struct SegSites;
+template <> struct SegSites<float> {
+    __device__ static const float4 *site(const BSeg &s) { return s.ps32; }
+    __device__ static const float *damp(const BSeg &s) { return s.pa32; }
+};
+template <> struct SegSites<double> {
+    __device__ static const double4 *site(const BSeg &s) { return s.ps64; }
+    __device__ static const double *damp(const BSeg &s) { return s.pa64; }
+};
+
+// Does column c take part in this matvec?  FP32 (use_p): inner PCG columns in
+// slot 1.  FP64: outer columns in slot 0 whose x is not identically zero.
+__device__ __forceinline__ bool mv_active(const BCol &c, int use_p) {
+    return use_p ? c.s[1].phase == kPhaseInner : (c.s[0].phase == kPhaseOuter && !c.xzero);
+}
+
+// Grouped on-the-fly pair kernel: one target per thread, sources of one split
+// staged through LDS, M columns per work item (as k_tmv_pairs).
+template <typename R, int M, bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                   const int4 *__restrict__ work, int use_p) {
+    using R4 = typename V4<R>::t;
+    const int4 w = work[blockIdx.x];
+    const BSeg &S = segs[w.x];
+    const int npol = S.npol, ncols = S.ncols;
+    const long n = 3L * npol;
+    const int c0 = w.w * M;
+    const double *xin = use_p ? S.p : S.x;
+    bool act[M];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        act[m] = c0 + m < ncols && mv_active(cols[S.col0 + c0 + m], use_p);
+        any = any || act[m];
+    }
+    if (!any) return;
+    const R4 *__restrict__ site = SegSites<R>::site(S);
+    const R *__restrict__ adamp = SegSites<R>::damp(S);
+    __shared__ R4 s_pos[kBlock];
+    __shared__ R4 s_x[M][kBlock];
+    const int tid = threadIdx.x;
+    const int j = w.y * kBlock + tid;
+    const bool valid = j < npol;
+    R4 pj;
+    R aj;
+    if (valid) {
+        pj = site[j];
+        aj = adamp[j];
+    } else {
+        pj.x = pj.y = pj.z = R(0);
+        pj.w = R(1);
+        aj = R(0.39);
+    }
+    R acc[M][3];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = R(0);
+    const int i0 = w.z * S.chunk;
+    const int i1 = min(npol, i0 + S.chunk);
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            const int i = base + tid;
+            s_pos[tid] = site[i];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                R4 v;
+                if (act[m]) {
+                    const double *xc = xin + (long)(c0 + m) * n + 3L * i;
+                    v.x = (R)xc[0];
+                    v.y = (R)xc[1];
+                    v.z = (R)xc[2];
+                } else {
+                    v.x = v.y = v.z = R(0);
+                }
+                v.w = m == 0 ? adamp[i] : R(0);
+                s_x[m][tid] = v;
+            }
+        } else if (tid == cnt) {
+            R4 far;
+            far.x = far.y = far.z = R(1e15);
+            far.w = R(1);
+            s_pos[tid] = far;
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                R4 z;
+                z.x = z.y = z.z = R(0);
+                z.w = R(0.39);
+                s_x[m][tid] = z;
+            }
+        }
+        __syncthreads();
+        const int cnt2 = (cnt + 1) & ~1;
+        for (int k = 0; k < cnt2; k += 2) {
+            R4 xa[M], xb[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                xa[m] = s_x[m][k];
+                xb[m] = s_x[m][k + 1];
+            }
+            R dxa, dya, dza, k2a, k3a, arga, dxb, dyb, dzb, k2b, k3b, argb;
+            bool neara, nearb;
+            pair_geom<R, AMOEBA>(pj, aj, s_pos[k], xa[0].w, dxa, dya, dza, k2a, k3a, arga, neara);
+            pair_geom<R, AMOEBA>(pj, aj, s_pos[k + 1], xb[0].w, dxb, dyb, dzb, k2b, k3b, argb, nearb);
+            if (__ballot(neara || nearb)) {
+                pair_damp<R, AMOEBA>(arga, neara, k2a, k3a);
+                pair_damp<R, AMOEBA>(argb, nearb, k2b, k3b);
+            }
+            pair_accum<R, M>(acc, xa, dxa, dya, dza, k2a, k3a);
+            pair_accum<R, M>(acc, xb, dxb, dyb, dzb, k2b, k3b);
+        }
+    }
+    if (valid) {
+        R *part = static_cast<R *>(S.part);
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            if (!act[m]) continue;
+            R *o = part + ((long)w.z * ncols + c0 + m) * n + 3L * j;
+            o[0] = acc[m][0];
+            o[1] = acc[m][1];
+            o[2] = acc[m][2];
+        }
+    }
+}
+
+// h = -(sum of split partials) + (1 - s) corrections + x / alpha for the
+// active columns of one target block; FP32 passes also store h.p partials.
+template <typename R, int M, bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_bfinal(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                   const int4 *__restrict__ work, int use_p) {
+    const int4 w = work[blockIdx.x];
+    const BSeg &S = segs[w.x];
+    const int npol = S.npol, ncols = S.ncols;
+    const long n = 3L * npol;
+    const int c0 = w.z * M;
+    bool act[M];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        act[m] = c0 + m < ncols && mv_active(cols[S.col0 + c0 + m], use_p);
+        any = any || act[m];
+    }
+    if (!any) return;
+    __shared__ double sh[4];
+    const double *xin = use_p ? S.p : S.x;
+    const R *part = static_cast<const R *>(S.part);
+    const int j = w.y * kBlock + threadIdx.x;
+    const bool valid = j < npol;
+    double acc[M][3];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
+    if (valid) {
+        for (int s = 0; s < S.nsplit; s++)
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                if (!act[m]) continue;
+                const R *p = part + ((long)s * ncols + c0 + m) * n + 3L * j;
+                acc[m][0] -= (double)p[0];
+                acc[m][1] -= (double)p[1];
+                acc[m][2] -= (double)p[2];
+            }
+        const double4 pj = S.ps64[j];
+        const double aj = S.pa64[j];
+        for (int k = S.pp_ri[j]; k < S.pp_ri[j + 1]; k++) {
+            const int i = S.pp_ci[k];
+            const double f = S.pp_f[k];
+            const double4 pi = S.ps64[i];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double k2, k3;
+            dip_kernel<double, AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, S.pa64[i], aj, k2, k3);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                if (!act[m]) continue;
+                const double *xi = xin + (long)(c0 + m) * n + 3L * i;
+                const double c = k3 * (xi[0] * dx + xi[1] * dy + xi[2] * dz);
+                acc[m][0] += f * (c * dx - k2 * xi[0]);
+                acc[m][1] += f * (c * dy - k2 * xi[1]);
+                acc[m][2] += f * (c * dz - k2 * xi[2]);
+            }
+        }
+        const double ia = 1.0 / S.alpha[j];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            if (!act[m]) continue;
+            const double *xj = xin + (long)(c0 + m) * n + 3L * j;
+            double *yj = S.h + (long)(c0 + m) * n + 3L * j;
+            for (int c = 0; c < 3; c++) {
+                acc[m][c] += xj[c] * ia;
+                yj[c] = acc[m][c];
+            }
+        }
+    }
+    if (use_p) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            if (!act[m]) continue;  // uniform over the block
+            double v = 0.0;
+            if (valid) {
+                const double *xj = xin + (long)(c0 + m) * n + 3L * j;
+                for (int c = 0; c < 3; c++) v += acc[m][c] * xj[c];
+            }
+            v = block_sum_all(v, sh);
+            if (threadIdx.x == 0) S.red_hp[(long)(c0 + m) * S.tblocks + w.y] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------ vector kernels
+// Work item {segment, column, vector block, batch column}; element k of the
+// column belongs to site k / 3.
+
+// Outer step, part 1: r = b - T x (x = 0: r = b), z = alpha r, r.z partials.
+__global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                   const int4 *__restrict__ work) {
+    __shared__ double sh[4];
+    const int4 w = work[blockIdx.x];
+    const BCol &C = cols[w.w];
+    if (C.s[0].phase != kPhaseOuter) return;
+    const BSeg &S = segs[w.x];
+    const long n = 3L * S.npol;
+    const long k = (long)w.z * kBlock + threadIdx.x;
+    const long o = (long)w.y * n + k;
+    double v = 0.0;
+    if (k < n) {
+        const double rr = C.xzero ? S.b[o] : S.b[o] - S.h[o];
+        const double zz = S.alpha[k / 3] * rr;
+        S.r[o] = rr;
+        S.z[o] = zz;
+        v = rr * zz;
+    }
+    v = block_sum_all(v, sh);
+    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = v;
+}
+
+// Outer step, part 2 (state copy 0 -> 1): rms test; if not converged, start an
+// inner PCG on T d = r from d = 0 (p = z, g = r.z).
+__global__ __launch_bounds__(kBlock) void k_bdecide(const BSeg *__restrict__ segs, BCol *__restrict__ cols,
+                                                    const int4 *__restrict__ work, double eta) {
+    __shared__ double sh[4];
+    const int4 w = work[blockIdx.x];
+    const BCol &C = cols[w.w];
+    const BColSub st = C.s[0];
+    BColSub nst = st;
+    if (st.phase == kPhaseOuter) {
+        const BSeg &S = segs[w.x];
+        const long n = 3L * S.npol;
+        const double rz = reduce_parts(S.red_rz + (long)w.y * S.vblocks, S.vblocks, sh);
+        const double rms = sqrt(rz / (double)n);
+        nst.rms = rms;
+        if (rms < C.tol) {
+            nst.phase = kPhaseDone;
+            nst.status = 0;
+        } else if (st.outer >= kBatchMaxOuter) {
+            nst.phase = kPhaseDone;
+            nst.status = 2;
+        } else {
+            nst.phase = kPhaseInner;
+            nst.tin = fmax(eta * rms, 0.5 * C.tol);
+            nst.g = rz;
+            nst.it = 0;
+            nst.outer = st.outer + 1;
+            const long k = (long)w.z * kBlock + threadIdx.x;
+            if (k < n) {
+                const long o = (long)w.y * n + k;
+                S.d[o] = 0.0;
+                S.p[o] = S.z[o];
+            }
+        }
+    }
+    if (w.z == 0 && threadIdx.x == 0) cols[w.w].s[1] = nst;
+}
+
+// Inner PCG, update: alpha = g / p.Tp; d += alpha p; r -= alpha h; z = alpha_i r.
+__global__ __launch_bounds__(kBlock) void k_bupdate(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                    const int4 *__restrict__ work) {
+    __shared__ double sh[4];
+    const int4 w = work[blockIdx.x];
+    const BCol &C = cols[w.w];
+    if (C.s[1].phase != kPhaseInner) return;
+    const BSeg &S = segs[w.x];
+    const long n = 3L * S.npol;
+    const double gama = reduce_parts(S.red_hp + (long)w.y * S.tblocks, S.tblocks, sh);
+    if (gama == 0.0) return;  // the direction kernel ends this inner solve
+    const double a = C.s[1].g / gama;
+    const long k = (long)w.z * kBlock + threadIdx.x;
+    const long o = (long)w.y * n + k;
+    double v = 0.0;
+    if (k < n) {
+        S.d[o] += a * S.p[o];
+        const double rr = S.r[o] - a * S.h[o];
+        const double zz = S.alpha[k / 3] * rr;
+        S.r[o] = rr;
+        S.z[o] = zz;
+        v = rr * zz;
+    }
+    v = block_sum_all(v, sh);
+    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = v;
+}
+
+// Inner PCG, direction (state copy 1 -> 0): inner convergence ends the
+// refinement step with x += d; otherwise p = z + beta p.
+__global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ segs, BCol *__restrict__ cols,
+                                                       const int4 *__restrict__ work) {
+    __shared__ double sh[4];
+    const int4 w = work[blockIdx.x];
+    const BCol &C = cols[w.w];
+    const BColSub st = C.s[1];
+    BColSub nst = st;
+    bool ended = false;
+    if (st.phase == kPhaseInner) {
+        const BSeg &S = segs[w.x];
+        const long n = 3L * S.npol;
+        const double gama = reduce_parts(S.red_hp + (long)w.y * S.tblocks, S.tblocks, sh);
+        const double gnew = gama == 0.0 ? st.g : reduce_parts(S.red_rz + (long)w.y * S.vblocks, S.vblocks, sh);
+        const double rms_in = sqrt(gnew / (double)n);
+        ended = gama == 0.0 || rms_in < st.tin || st.it + 1 >= C.maxit;
+        const long k = (long)w.z * kBlock + threadIdx.x;
+        const long o = (long)w.y * n + k;
+        if (ended) {
+            if (k < n) S.x[o] += S.d[o];
+            nst.phase = kPhaseOuter;
+            nst.inner_total = st.inner_total + st.it + 1;
+        } else {
+            const double beta = gnew / st.g;
+            if (k < n) S.p[o] = S.z[o] + beta * S.p[o];
+            nst.g = gnew;
+            nst.it = st.it + 1;
+        }
+    }
+    if (w.z == 0 && threadIdx.x == 0) {
+        cols[w.w].s[0] = nst;
+        if (ended) cols[w.w].xzero = 0;
+    }
+}
+
+template <typename R, int M, bool AMOEBA>
+static void mv_launch(hipStream_t st, const BatchTables &t, const int4 *pair, int npair, const int4 *fin, int nfin,
+                      int use_p) {
+    if (npair == 0) return;
+    hipLaunchKernelGGL((k_bpairs<R, M, AMOEBA>), dim3(npair), dim3(kBlock), 0, st, t.segs, t.cols, pair, use_p);
+    hipLaunchKernelGGL((k_bfinal<R, M, AMOEBA>), dim3(nfin), dim3(kBlock), 0, st, t.segs, t.cols, fin, use_p);
+}
+
+template <bool AMOEBA>
+static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, double eta) {
+    if (m64 == 2) mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
+    else mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
+    hipLaunchKernelGGL(k_bresid, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
+    hipLaunchKernelGGL(k_bdecide, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec, eta);
+    if (m32 == 2) mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
+    else mv_launch<float, kBatchM32, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
+    hipLaunchKernelGGL(k_bupdate, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
+    hipLaunchKernelGGL(k_bdirection, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m32, int m64, double eta) {
+    if (t.nvec == 0) return;
+    if (amoeba) tick_t<true>(st, t, m32, m64, eta);
+    else tick_t<false>(st, t, m32, m64, eta);
+}
+
+// ------------------------------------------------------------------ helpers
+__global__ void k_batch_rhs(long n, int nsets, int ncols, const double *__restrict__ e_m2d,
+                            const double *__restrict__ ext, double *__restrict__ b) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= n || c >= ncols) return;
+    const int rhs = c / nsets, set = c % nsets;
+    b[(long)c * n + k] = e_m2d[(long)set * n + k] + ext[(long)rhs * n + k];
+}
+
+void launch_batch_rhs(hipStream_t st, long n, int nsets, int ncols, const double *e_m2d, const double *ext,
+                      double *b) {
+    if (n == 0 || ncols == 0) return;
+    hipLaunchKernelGGL(k_batch_rhs, dim3((unsigned)((n + kBlock - 1) / kBlock), ncols), dim3(kBlock), 0, st, n,
+                       nsets, ncols, e_m2d, ext, b);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_charges_field(int npol, const double4 *__restrict__ ps, int nq,
+                                                          const double4 *__restrict__ qc, double *__restrict__ ext) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= npol) return;
+    const double4 p = ps[j];
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    for (int q = 0; q < nq; q++) {
+        const double4 c = qc[q];
+        const double dx = p.x - c.x, dy = p.y - c.y, dz = p.z - c.z;
+        const double ir = rsqrt(dx * dx + dy * dy + dz * dz);
+        const double f = c.w * ir * ir * ir;
+        ex += f * dx;
+        ey += f * dy;
+        ez += f * dz;
+    }
+    ext[3L * j] = ex;
+    ext[3L * j + 1] = ey;
+    ext[3L * j + 2] = ez;
+}
+
+void launch_charges_field(hipStream_t st, int npol, const double4 *ps64, int nq, const double4 *charges,
+                          double *ext) {
+    if (npol == 0) return;
+    hipLaunchKernelGGL(k_charges_field, dim3((npol + kBlock - 1) / kBlock), dim3(kBlock), 0, st, npol, ps64, nq,
+                       charges, ext);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit, int xzero) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nc) return;
+    BCol &C = cols[c0 + c];
+    BColSub s;
+    s.g = 0.0;
+    s.tin = 0.0;
+    s.rms = 0.0;
+    s.phase = kPhaseOuter;
+    s.it = 0;
+    s.outer = 0;
+    s.inner_total = 0;
+    s.status = 2;
+    s.pad = 0;
+    C.s[0] = s;
+    C.s[1] = s;
+    C.tol = tol;
+    C.maxit = maxit;
+    C.xzero = xzero;
+}
+
+void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int xzero) {
+    if (nc == 0) return;
+    hipLaunchKernelGGL(k_batch_start, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cols, c0, nc, tol,
+                       maxit, xzero);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_benergy_part(long n, int nsets, const double *__restrict__ x,
+                                                         const double *__restrict__ e_m2d,
+                                                         double *__restrict__ red) {
+    __shared__ double sh[4];
+    const int rhs = blockIdx.y;
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    // mu_D of this right-hand side against the P-set static field
+    double v = k < n ? x[(long)rhs * nsets * n + k] * e_m2d[(long)(nsets - 1) * n + k] : 0.0;
+    v = block_sum_all(v, sh);
+    if (threadIdx.x == 0) red[(long)rhs * gridDim.x + blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_benergy_final(const double *__restrict__ red, int nred,
+                                                          double *__restrict__ out) {
+    __shared__ double sh[4];
+    const double s = reduce_parts(red + (long)blockIdx.x * nred, nred, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = -0.5 * s;
+}
+
+void launch_batch_energy(hipStream_t st, long n, int nsets, int nrhs, const double *x, const double *e_m2d,
+                         double *red, double *out) {
+    if (nrhs == 0) return;
+    const int nb = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_benergy_part, dim3(nb, nrhs), dim3(kBlock), 0, st, n, nsets, x, e_m2d, red);
+    hipLaunchKernelGGL(k_benergy_final, dim3(nrhs), dim3(kBlock), 0, st, red, nb, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ommp

@@ -209,4 +209,41 @@ void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out) {
     for (int k = 0; k < 3; k++) out[k] = sys(s)->last_ms[k];
 }
 
+void ommp_amd_set_external_fields_multi(OMMP_SYSTEM_PRT s, int32_t nrhs, const double *ext, double *ipd,
+                                        double *epol) {
+    ommp::solve_multi(*sys(s), nrhs, ext, false, ipd, false, epol);
+}
+
+void ommp_amd_set_external_fields_multi_device(OMMP_SYSTEM_PRT s, int32_t nrhs, const double *d_ext,
+                                               double *d_ipd, double *epol) {
+    ommp::solve_multi(*sys(s), nrhs, d_ext, true, d_ipd, true, epol);
+}
+
+void ommp_amd_charges_field(OMMP_SYSTEM_PRT s, int32_t nq, const double *charges, double *out) {
+    ommp::charges_field_host(*sys(s), nq, charges, out);
+}
+
+OMMP_AMD_SERVER_PTR ommp_amd_server_create(double hbm_budget_bytes, int32_t max_active) {
+    return ommp::server_create(hbm_budget_bytes, max_active);
+}
+
+int32_t ommp_amd_server_submit(OMMP_AMD_SERVER_PTR srv, OMMP_SYSTEM_PRT s, int32_t nsteps, int32_t nq,
+                               const double *charges) {
+    return ommp::server_submit(srv, sys(s), nsteps, nq, charges);
+}
+
+void ommp_amd_server_wait(OMMP_AMD_SERVER_PTR srv, int32_t id) { ommp::server_wait(srv, id); }
+
+int32_t ommp_amd_server_result(OMMP_AMD_SERVER_PTR srv, int32_t id, double *energies, double *latency_ms,
+                               int32_t *iters) {
+    return ommp::server_result(srv, id, energies, latency_ms, iters);
+}
+
+void ommp_amd_server_stats(OMMP_AMD_SERVER_PTR srv, double *out) { ommp::server_stats(srv, out); }
+
+void ommp_amd_server_destroy(OMMP_AMD_SERVER_PTR srv) {
+    ommp::server_wait(srv, -1);
+    ommp::server_destroy(srv);
+}
+
 }  // extern "C"

@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
+#include "batch_engine.h"
 #include "device.h"
 #include "ommp_internal.h"
 
@@ -70,6 +72,9 @@ struct DeviceState {
     double *diis_x = nullptr, *diis_e = nullptr, *diis_coef = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
+    std::vector<size_t> alloc_bytes;
+    // batched multi-RHS solver state (cached between calls)
+    BatchEngine *batch = nullptr;
     // live kernel timing: event pairs in flight and accumulated totals per class
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
@@ -95,6 +100,7 @@ void *dmalloc(DeviceState &d, size_t bytes) {
         fatal(buf);
     }
     d.allocs.push_back(p);
+    d.alloc_bytes.push_back(bytes);
     return p;
 }
 
@@ -102,7 +108,12 @@ template <typename T> void dfree(DeviceState &d, T *&p) {
     if (!p) return;
     void *v = (void *)p;
     (void)hipFree(v);
-    d.allocs.erase(std::remove(d.allocs.begin(), d.allocs.end(), v), d.allocs.end());
+    for (size_t i = 0; i < d.allocs.size(); i++)
+        if (d.allocs[i] == v) {
+            d.allocs.erase(d.allocs.begin() + i);
+            d.alloc_bytes.erase(d.alloc_bytes.begin() + i);
+            break;
+        }
     p = nullptr;
 }
 
@@ -178,6 +189,7 @@ void free_csr(DeviceState &d, CsrDev &c) {
 }
 
 void upload_lists(System &s, DeviceState &d) {
+    batch_release(d.batch);  // its segment points at the old lists
     free_csr(d, d.sp);
     free_csr(d, d.pp);
     free_csr(d, d.pd);
@@ -590,8 +602,11 @@ void device_release(System &s) {
     if (!s.dev) return;
     DeviceState &d = *s.dev;
     (void)hipStreamSynchronize(d.stream);
+    batch_release(d.batch);
+    d.batch = nullptr;
     for (void *p : d.allocs) (void)hipFree(p);
     d.allocs.clear();
+    d.alloc_bytes.clear();
     if (d.cs_host) (void)hipHostFree(d.cs_host);
     if (d.scal_host) (void)hipHostFree(d.scal_host);
     for (auto &e : d.ev)
@@ -749,6 +764,143 @@ void refresh_host_ipd(System &s) {
     HCHECK(hipMemcpyAsync(s.ipd.data(), d.ipd, sizeof(double) * s.n_ipd * d.n, hipMemcpyDeviceToHost, d.stream));
     HCHECK(hipStreamSynchronize(d.stream));
     s.host_ipd_stale = false;
+}
+
+// Device bytes ensure_device() allocates for a system (packed T, dense T and
+// the batched multi-RHS state come on top, on demand).
+size_t device_footprint_estimate(const System &s) {
+    const size_t np = s.npol, nm = s.nmm, n = 3 * np;
+    size_t b = np * (sizeof(double4) + sizeof(float4) + 3 * sizeof(double) + sizeof(float) + sizeof(int));
+    b += nm * (sizeof(double4) + sizeof(double) + sizeof(double) * s.ld_cart);
+    b += sizeof(double) * (2 * n * 2 + n + 8 * 2 * n);     // e_m2d, ipd, ext, work vectors
+    b += 3 * 24 * sizeof(double) * (n / kBlock + 16);      // reduction scratch
+    const int nsplit = tmv_split(s.npol);
+    b += tmv_part_bytes(s.npol, 2, 64, nsplit) + sizeof(double) * n * 16;  // pair and static-field partials
+    b += (size_t)(s.list_S_P_P.nnz() + s.list_S_P_D.nnz() + 2 * s.list_P_P.nnz() + s.list_P_P_dense.nnz()) * 24;
+    return b + (1 << 20);
+}
+
+DeviceView device_view(System &s) {
+    DeviceState &d = ensure_device(s);
+    prepare_polelec(s);
+    HCHECK(hipStreamSynchronize(d.stream));
+    DeviceView v;
+    v.device = d.device;
+    v.stream = d.stream;
+    v.npol = s.npol;
+    v.amoeba = s.amoeba ? 1 : 0;
+    v.n_ipd = s.n_ipd;
+    v.ps64 = d.ps64;
+    v.pa64 = d.pa64;
+    v.ps32 = d.ps32;
+    v.pa32 = d.pa32;
+    v.alpha = d.alpha;
+    v.pp = d.pp;
+    v.e_m2d = d.e_m2d;
+    v.ipd = d.ipd;
+    v.bytes = 0;
+    for (size_t b : d.alloc_bytes) v.bytes += b;
+    return v;
+}
+
+// Several external fields of one system solved together (no reference
+// counterpart: the reference solves one field per polarization() call).  Each
+// right-hand side is the system's E_M2D plus its own field, D and P sets as two
+// columns; all columns share every launch of the batched mixed solver.  The
+// guess of every column is the system's current dipoles when a previous solve
+// left them (ipd_use_guess), else zero.  The system's own dipoles and energy
+// state are left as they were.
+void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *ipd_out, bool out_dev,
+                 double *epol) {
+    if (nrhs < 1) fatal("The number of right-hand sides must be positive");
+    const long n = 3L * s.npol;
+    if (s.npol == 0) {
+        if (epol)
+            for (int k = 0; k < nrhs; k++) epol[k] = 0.0;
+        return;
+    }
+    DeviceView v = device_view(s);
+    DeviceState &d = *s.dev;
+    const int nsets = s.n_ipd, ncols = nrhs * nsets;
+    if (d.batch && (int)d.batch->col_owner.size() != ncols) batch_release(d.batch);
+    if (!d.batch) {
+        d.batch = new BatchEngine(d.stream, s.amoeba, ncols > 2, 1, ncols);
+        if (d.batch->add(v, nrhs) < 0) {
+            batch_release(d.batch);
+            fatal("Cannot allocate device memory for the multi-RHS solve");
+        }
+    }
+    BatchEngine &E = *d.batch;
+    SegInfo &S = E.segs[0];
+    HCHECK(hipEventRecord(d.ev[0], d.stream));
+    HCHECK(hipMemcpyAsync(S.ext, ext, sizeof(double) * nrhs * n,
+                          ext_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, d.stream));
+    E.set_rhs(0);
+    const bool guess = s.ipd_use_guess;
+    if (guess) {
+        for (int r = 0; r < nrhs; r++)
+            HCHECK(hipMemcpyAsync(S.dev.x + (long)r * nsets * n, d.ipd, sizeof(double) * nsets * n,
+                                  hipMemcpyDeviceToDevice, d.stream));
+    } else {
+        HCHECK(hipMemsetAsync(S.dev.x, 0, sizeof(double) * ncols * n, d.stream));
+    }
+    E.start(0, s.tol, s.max_iter, !guess);
+    const double eta = 1e-5;
+    const long max_ticks = (long)(kBatchMaxOuter + 1) * (s.max_iter + 2);
+    long prev = -1;
+    const BCol *snap = nullptr;
+    for (long t = 0;; t++) {
+        E.tick(eta);
+        const long seq = E.enqueue_readback();
+        if (prev >= 0) {
+            snap = E.wait_readback(prev);
+            bool all = true;
+            for (int c = 0; c < ncols && all; c++) all = snap[S.col0 + c].s[0].phase == kPhaseDone;
+            if (all) break;
+        }
+        prev = seq;
+        if (t > max_ticks) fatal("Iterative solver did not converged");
+    }
+    int outer = 0, inner = 0;
+    for (int c = 0; c < ncols; c++) {
+        const BColSub &cs = snap[S.col0 + c].s[0];
+        if (cs.status != 0) fatal("Iterative solver did not converged");
+        outer = std::max(outer, cs.outer + 1);
+        inner = std::max(inner, cs.inner_total);
+    }
+    s.last_iters[0] = s.last_iters[1] = inner;
+    s.last_iters[2] = outer;
+    s.last_iters[3] = inner;
+    s.last_status = 0;
+    HCHECK(hipEventRecord(d.ev[1], d.stream));
+    E.energies(0);
+    if (ipd_out)
+        HCHECK(hipMemcpyAsync(ipd_out, S.dev.x, sizeof(double) * ncols * n,
+                              out_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, d.stream));
+    if (epol) HCHECK(hipMemcpyAsync(epol, S.eout, sizeof(double) * nrhs, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipEventRecord(d.ev[3], d.stream));
+    HCHECK(hipEventSynchronize(d.ev[3]));
+    float ms = 0.0f;
+    s.last_ms[0] = 0.0;
+    (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);
+    s.last_ms[1] = ms;
+    (void)hipEventElapsedTime(&ms, d.ev[1], d.ev[3]);
+    s.last_ms[2] = ms;
+}
+
+// E_n2p (mod_qm_helper.F90:376-391) of nq charges [nq][4] at the polarizable sites.
+void charges_field_host(System &s, int nq, const double *charges, double *out) {
+    if (s.npol == 0) return;
+    if (nq < 0) fatal("The number of charges must not be negative");
+    DeviceState &d = ensure_device(s);
+    double4 *dq = nullptr;
+    double *dout = d.vr2;
+    if (nq > 0) HCHECK(hipMalloc((void **)&dq, sizeof(double4) * nq));
+    if (nq > 0) HCHECK(hipMemcpyAsync(dq, charges, sizeof(double4) * nq, hipMemcpyHostToDevice, d.stream));
+    launch_charges_field(d.stream, s.npol, d.ps64, nq, dq, dout);
+    HCHECK(hipMemcpyAsync(out, dout, sizeof(double) * d.n, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    if (dq) HCHECK(hipFree(dq));
 }
 
 const double *device_ipd(System &s) {

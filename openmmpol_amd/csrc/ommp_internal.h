@@ -114,5 +114,16 @@ void refresh_host_m2d(System &s);
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat);
 void profile_enable(System &s, bool on);
 void profile_stats(System &s, double *out, bool reset);
+void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *ipd_out, bool out_dev,
+                 double *epol);
+void charges_field_host(System &s, int nq, const double *charges, double *out);
+
+// server.cpp
+void *server_create(double budget, int max_active);
+void server_destroy(void *srv);
+int server_submit(void *srv, System *s, int nsteps, int nq, const double *charges);
+void server_wait(void *srv, int id);
+int server_result(void *srv, int id, double *energies, double *latency_ms, int *iters);
+void server_stats(void *srv, double *out);
 
 }  // namespace ommp

@@ -1,0 +1,142 @@
+// Device-side arithmetic shared by the single-system and the batched kernels:
+// block reductions with a fixed summation order, and the per-pair geometry,
+// Thole damping and dipole-field accumulation of the T matvec
+// (damped_coulomb_kernel + mu_elec_prop, mod_electrostatics.F90:1018-1284).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace ommp {
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum over the block; the result is valid in every thread.
+__device__ __forceinline__ double block_sum_all(double v, double *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    return t;
+}
+
+// Fixed-order sum of np per-block partials, valid in every thread.
+__device__ __forceinline__ double reduce_parts(const double *p, int np, double *sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += kBlock) v += p[i];
+    return block_sum_all(v, sh);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+
+template <typename R> struct V4;
+template <> struct V4<double> { using t = double4; };
+template <> struct V4<float> { using t = float4; };
+
+__device__ __forceinline__ float fast_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double fast_exp(double x) { return exp(x); }
+// r2 is either 0 (self pair, masked by the caller) or >= ~1 bohr^2, never
+// denormal, so the raw v_rsq_f32 (1 ulp) needs no range fix-up.
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double fast_rsqrt(double x) { return rsqrt(x); }
+__device__ __forceinline__ float fmaa(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double fmaa(double a, double b, double c) { return fma(a, b, c); }
+
+// Damped dipole-field kernel (maxder 2): returns k2 = lambda3 / r^3 and
+// k3 = 3 lambda5 / r^5 so that E = k3 (x.d) d - k2 x and T x = -E.
+// Damping only acts inside a few Angstrom; the exponential is evaluated under
+// a wave-uniform branch that waves without a near pair skip entirely.
+template <typename R, bool AMOEBA>
+__device__ __forceinline__ void dip_kernel(R r2, R ir, R ti, R tj, R ai, R aj, R &k2, R &k3) {
+    const R ir2 = ir * ir;
+    k2 = ir * ir2;
+    k3 = (R(3) * k2) * ir2;
+    const R u = (r2 * ir) * (ti * tj);
+    if (AMOEBA) {
+        const R a = ai < aj ? ai : aj;
+        const R au3 = a * (u * u * u);
+        const bool near = au3 < R(50);
+        if (__ballot(near)) {
+            const R e = fast_exp(-au3);
+            const R l3 = R(1) - e;
+            const R l5 = R(1) - (R(1) + au3) * e;
+            k2 = near ? k2 * l3 : k2;
+            k3 = near ? k3 * l5 : k3;
+        }
+    } else {
+        const bool near = u < R(1);
+        if (__ballot(near)) {
+            const R u3 = u * u * u, u4 = u3 * u;
+            k2 = near ? k2 * (R(4) * u3 - R(3) * u4) : k2;
+            k3 = near ? k3 * u4 : k3;
+        }
+    }
+}
+
+// Geometry of one pair: d = r_j - r_i, undamped k2 = 1/r^3 and k3 = 3/r^5,
+// the damping argument (a u^3 for AMOEBA, u = r/s for Wang) and whether the
+// pair needs damping at all.
+template <typename R, bool AMOEBA>
+__device__ __forceinline__ void pair_geom(const typename V4<R>::t &pj, R aj, const typename V4<R>::t &pi, R ai,
+                                          R &dx, R &dy, R &dz, R &k2, R &k3, R &arg, bool &near) {
+    dx = pj.x - pi.x;
+    dy = pj.y - pi.y;
+    dz = pj.z - pi.z;
+    const R r2 = fmaa(dz, dz, fmaa(dy, dy, dx * dx));
+    const R ir = r2 > R(0) ? fast_rsqrt(r2) : R(0);
+    const R ir2 = ir * ir;
+    k2 = ir * ir2;
+    k3 = (R(3) * k2) * ir2;
+    const R u = (r2 * ir) * (pi.w * pj.w);
+    if (AMOEBA) {
+        const R a = ai < aj ? ai : aj;
+        arg = a * (u * u * u);
+        near = arg < R(50);
+    } else {
+        arg = u;
+        near = u < R(1);
+    }
+}
+
+template <typename R, bool AMOEBA>
+__device__ __forceinline__ void pair_damp(R arg, bool near, R &k2, R &k3) {
+    if (AMOEBA) {
+        const R e = fast_exp(-arg);
+        const R l3 = R(1) - e;
+        const R l5 = R(1) - (R(1) + arg) * e;
+        k2 = near ? k2 * l3 : k2;
+        k3 = near ? k3 * l5 : k3;
+    } else {
+        const R u3 = arg * arg * arg, u4 = u3 * arg;
+        k2 = near ? k2 * (R(4) * u3 - R(3) * u4) : k2;
+        k3 = near ? k3 * u4 : k3;
+    }
+}
+
+template <typename R, int M>
+__device__ __forceinline__ void pair_accum(R (&acc)[M][3], const typename V4<R>::t *xv, R dx, R dy, R dz, R k2,
+                                           R k3) {
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const R c = k3 * fmaa(xv[m].z, dz, fmaa(xv[m].y, dy, xv[m].x * dx));
+        const R nk2 = -k2;
+        acc[m][0] = fmaa(c, dx, fmaa(nk2, xv[m].x, acc[m][0]));
+        acc[m][1] = fmaa(c, dy, fmaa(nk2, xv[m].y, acc[m][1]));
+        acc[m][2] = fmaa(c, dz, fmaa(nk2, xv[m].z, acc[m][2]));
+    }
+}
+
+}  // namespace ommp

@@ -1,0 +1,383 @@
+// Serving loop: continuous batching of SCF-style polarization solves of many
+// systems on one GPU.
+//
+// A request is one system plus an SCF sequence of external fields, each the
+// field of a set of point charges (a QM region) at the polarizable sites.  The
+// worker thread admits requests while batch slots and the HBM budget allow,
+// keeps every admitted system resident (geometry, lists, static field, last
+// dipoles), and runs ticks of the batched mixed solver (batch.h) over all
+// admitted systems at once.  A solve that converges is retired at the next
+// tick boundary; the request's next solve starts right away from the converged
+// dipoles (warm start), and a request whose last solve finished releases its
+// slot and memory to the next waiting request.  Column states are read back one
+// tick late, so the GPU always has the next tick queued.
+//
+// Each solve's arithmetic depends only on its own system (batch.h), so a
+// request gives the same dipoles and energies alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "batch_engine.h"
+#include "ommp_internal.h"
+
+namespace ommp {
+
+#define SCHECK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess)                                                                 \
+            fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
+    } while (0)
+
+DeviceView device_view(System &s);
+size_t device_footprint_estimate(const System &s);
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct Request {
+    int id = 0;
+    System *sys = nullptr;
+    int nsteps = 0, nq = 0;
+    std::vector<double> charges;  // [nsteps][nq][4]
+    double *charges_pin = nullptr;  // pinned copy, uploaded in stream order at admission
+    double t_arrival = 0.0;
+    // progress (worker thread only)
+    int seg = -1;
+    int step = 0;
+    long valid_from = 0;
+    double t_ready = 0.0;
+    size_t footprint = 0;
+    double *epin = nullptr;  // pinned [nsteps] energies
+    hipEvent_t ev = nullptr;
+    // results
+    std::vector<double> epin_copy;
+    std::vector<double> latency_ms, t_done;
+    std::vector<int> outer, inner;
+    int status = 0;
+    bool done = false;
+};
+
+class Server {
+  public:
+    Server(double budget, int max_active) : budget_(budget), max_active_(std::max(1, max_active)) {
+        SCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            num_cu_ = prop.multiProcessorCount;
+        worker_ = std::thread([this] { run(); });
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        worker_.join();
+        delete engine_;
+        for (auto &kv : reqs_) delete kv.second;
+        (void)hipStreamDestroy(stream_);
+    }
+
+    int submit(System *s, int nsteps, int nq, const double *charges) {
+        if (nsteps < 1) fatal("A request needs at least one solve");
+        if (nq < 0 || (long)nq * nsteps > kMaxCharges)
+            fatal("Too many point charges in a request (at most " + std::to_string(kMaxCharges) + " in all steps)");
+        Request *r = new Request();
+        r->sys = s;
+        r->nsteps = nsteps;
+        r->nq = nq;
+        r->charges.assign(charges, charges + 4L * nq * nsteps);
+        r->latency_ms.assign(nsteps, 0.0);
+        r->t_done.assign(nsteps, 0.0);
+        r->outer.assign(nsteps, 0);
+        r->inner.assign(nsteps, 0);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            if (in_service_.count(s)) fatal("A system can be in only one pending or active request");
+            in_service_.insert(s);
+            r->id = next_id_++;
+            r->t_arrival = now_s();
+            reqs_[r->id] = r;
+            pending_.push_back(r);
+        }
+        cv_.notify_all();
+        return r->id;
+    }
+
+    void wait(int id) {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] {
+            if (id >= 0) {
+                auto it = reqs_.find(id);
+                return it == reqs_.end() || it->second->done;
+            }
+            for (auto &kv : reqs_)
+                if (!kv.second->done) return false;
+            return true;
+        });
+    }
+
+    int result(int id, double *energies, double *latency_ms, int *iters) {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = reqs_.find(id);
+        if (it == reqs_.end() || !it->second->done) return -1;
+        Request &r = *it->second;
+        for (int k = 0; k < r.nsteps; k++) {
+            if (energies) energies[k] = r.epin_copy[k];
+            if (latency_ms) latency_ms[k] = r.latency_ms[k];
+            if (iters) {
+                iters[2 * k] = r.outer[k];
+                iters[2 * k + 1] = r.inner[k];
+            }
+        }
+        return r.status;
+    }
+
+    void stats(double *out) {
+        std::lock_guard<std::mutex> g(mu_);
+        out[0] = (double)ticks_;
+        out[1] = (double)solves_;
+        out[2] = ticks_ ? (double)col_ticks_ / (double)ticks_ : 0.0;  // mean columns in flight per tick
+        out[3] = (double)max_active_seen_;
+        out[4] = (double)peak_bytes_;
+        out[5] = (double)deferred_;
+    }
+
+  private:
+    void admit_locked(std::unique_lock<std::mutex> &lk) {
+        while (!pending_.empty() && (int)active_.size() < max_active_) {
+            Request *r = pending_.front();
+            const int nsets = r->sys->n_ipd;
+            const size_t fp = device_footprint_estimate(*r->sys) +
+                              BatchEngine::segment_bytes(r->sys->npol, 1, nsets, false, num_cu_);
+            if (used_ + fp > budget_ && !active_.empty()) {
+                deferred_++;
+                return;  // wait for memory to be released
+            }
+            pending_.pop_front();
+            lk.unlock();
+            const bool ok = admit(r, fp);
+            lk.lock();
+            if (!ok) {
+                if (active_.empty()) fatal("Cannot allocate device memory for a request of the serving loop");
+                pending_.push_front(r);
+                return;
+            }
+        }
+    }
+
+    bool admit(Request *r, size_t fp_estimate) {
+        System &s = *r->sys;
+        if (!engine_) engine_ = new BatchEngine(stream_, s.amoeba, false, max_active_, 2 * max_active_);
+        if (s.amoeba != engine_->amoeba) fatal("All systems of a server must use the same force field");
+        if (s.npol == 0) {
+            // nothing to solve: every step has zero energy
+            r->epin_copy.assign(r->nsteps, 0.0);
+            finish_now(r);
+            return true;
+        }
+        DeviceView v = device_view(s);  // upload, static field (system stream, synchronized)
+        const int seg = engine_->add(v, 1);
+        if (seg < 0) return false;
+        SegInfo &S = engine_->segs[seg];
+        r->seg = seg;
+        r->footprint = std::max(fp_estimate, v.bytes + S.bytes);
+        used_ += r->footprint;
+        if (used_ > peak_bytes_) peak_bytes_ = used_;
+        SCHECK(hipHostMalloc((void **)&r->epin, sizeof(double) * r->nsteps, hipHostMallocDefault));
+        SCHECK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
+        // pinned source: the copy stays in stream order behind earlier users of
+        // this (pooled) memory
+        SCHECK(hipHostMalloc((void **)&r->charges_pin, sizeof(double) * std::max<size_t>(r->charges.size(), 1),
+                             hipHostMallocDefault));
+        std::memcpy(r->charges_pin, r->charges.data(), sizeof(double) * r->charges.size());
+        SCHECK(hipMemcpyAsync(S.charges, r->charges_pin, sizeof(double) * r->charges.size(),
+                              hipMemcpyHostToDevice, stream_));
+        const long n = 3L * s.npol;
+        SCHECK(hipMemsetAsync(S.dev.x, 0, sizeof(double) * S.ncols * n, stream_));
+        start_step(r, true);
+        r->t_ready = r->t_arrival;
+        active_.push_back(r);
+        if ((int)active_.size() > max_active_seen_) max_active_seen_ = (int)active_.size();
+        return true;
+    }
+
+    void start_step(Request *r, bool xzero) {
+        SegInfo &S = engine_->segs[r->seg];
+        System &s = *r->sys;
+        launch_charges_field(stream_, s.npol, S.view.ps64, r->nq,
+                             S.charges + (long)r->step * r->nq, S.ext);
+        engine_->set_rhs(r->seg);
+        engine_->start(r->seg, s.tol, s.max_iter, xzero);
+        r->valid_from = engine_->next_readback();
+    }
+
+    // Solve r->step converged (seen in a snapshot): energy, next step or finish.
+    void complete_step(Request *r, const BCol *snap, double t) {
+        SegInfo &S = engine_->segs[r->seg];
+        int outer = 0, inner = 0;
+        for (int c = 0; c < S.ncols; c++) {
+            const BColSub &cs = snap[S.col0 + c].s[0];
+            if (cs.status != 0) r->status = 2;
+            outer = std::max(outer, cs.outer + 1);
+            inner = std::max(inner, cs.inner_total);
+        }
+        const int k = r->step;
+        r->outer[k] = outer;
+        r->inner[k] = inner;
+        r->t_done[k] = t;
+        r->latency_ms[k] = (t - r->t_ready) * 1e3;
+        engine_->energies(r->seg);
+        SCHECK(hipMemcpyAsync(r->epin + k, S.eout, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        solves_++;
+        r->step++;
+        if (r->step < r->nsteps) {
+            start_step(r, false);
+            r->t_ready = t;
+            return;
+        }
+        // last solve: dipoles back into the system, slot and memory released
+        const long n = 3L * r->sys->npol;
+        SCHECK(hipMemcpyAsync(S.view.ipd, S.dev.x, sizeof(double) * S.ncols * n, hipMemcpyDeviceToDevice,
+                              stream_));
+        SCHECK(hipEventRecord(r->ev, stream_));
+        engine_->remove(r->seg);
+        r->seg = -1;
+        used_ -= r->footprint;
+        finishing_.push_back(r);
+    }
+
+    void finish_now(Request *r) {
+        std::lock_guard<std::mutex> g(mu_);
+        r->done = true;
+        in_service_.erase(r->sys);
+        cv_.notify_all();
+    }
+
+    void reap_finished() {
+        for (size_t i = 0; i < finishing_.size();) {
+            Request *r = finishing_[i];
+            if (hipEventQuery(r->ev) != hipSuccess) {
+                i++;
+                continue;
+            }
+            r->epin_copy.assign(r->epin, r->epin + r->nsteps);
+            (void)hipHostFree(r->epin);
+            r->epin = nullptr;
+            (void)hipHostFree(r->charges_pin);
+            r->charges_pin = nullptr;
+            (void)hipEventDestroy(r->ev);
+            r->ev = nullptr;
+            System &s = *r->sys;
+            s.ipd_done = true;
+            s.ipd_use_guess = true;
+            s.host_ipd_stale = true;
+            finishing_.erase(finishing_.begin() + i);
+            finish_now(r);
+        }
+    }
+
+    void run() {
+        long prev = -1;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            admit_locked(lk);
+            if (active_.empty()) {
+                lk.unlock();
+                if (!finishing_.empty()) SCHECK(hipStreamSynchronize(stream_));
+                reap_finished();
+                lk.lock();
+                prev = -1;
+                if (stop_ && pending_.empty() && finishing_.empty()) break;
+                if (pending_.empty()) cv_.wait(lk, [&] { return stop_ || !pending_.empty(); });
+                continue;
+            }
+            lk.unlock();
+            engine_->tick(kEta);
+            const long seq = engine_->enqueue_readback();
+            int live = 0;
+            if (prev >= 0) {
+                const BCol *snap = engine_->wait_readback(prev);
+                const double t = now_s();
+                std::vector<Request *> still;
+                for (Request *r : active_) {
+                    if (prev < r->valid_from) {
+                        still.push_back(r);
+                        continue;
+                    }
+                    const SegInfo &S = engine_->segs[r->seg];
+                    bool all = true;
+                    for (int c = 0; c < S.ncols && all; c++) all = snap[S.col0 + c].s[0].phase == kPhaseDone;
+                    if (!all) {
+                        live += S.ncols;
+                        still.push_back(r);
+                        continue;
+                    }
+                    complete_step(r, snap, t);
+                    if (r->seg >= 0) still.push_back(r);
+                }
+                active_.swap(still);
+            }
+            prev = seq;
+            reap_finished();
+            lk.lock();
+            ticks_++;
+            col_ticks_ += live;
+        }
+    }
+
+    static constexpr double kEta = 1e-5;
+    double budget_;
+    int max_active_;
+    int num_cu_ = 256;
+    hipStream_t stream_ = nullptr;
+    BatchEngine *engine_ = nullptr;
+    std::thread worker_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    bool stop_ = false;
+    int next_id_ = 0;
+    std::map<int, Request *> reqs_;
+    std::deque<Request *> pending_;
+    std::set<System *> in_service_;
+    std::vector<Request *> active_;     // worker thread
+    std::vector<Request *> finishing_;  // worker thread
+    size_t used_ = 0;  // worker thread
+    std::atomic<size_t> peak_bytes_{0};
+    std::atomic<long> ticks_{0}, solves_{0}, col_ticks_{0}, deferred_{0};
+    std::atomic<int> max_active_seen_{0};
+};
+
+// C-ABI glue (capi.cpp)
+void *server_create(double budget, int max_active) { return new Server(budget, max_active); }
+void server_destroy(void *p) { delete static_cast<Server *>(p); }
+int server_submit(void *p, System *s, int nsteps, int nq, const double *charges) {
+    return static_cast<Server *>(p)->submit(s, nsteps, nq, charges);
+}
+void server_wait(void *p, int id) { static_cast<Server *>(p)->wait(id); }
+int server_result(void *p, int id, double *e, double *lat, int *iters) {
+    return static_cast<Server *>(p)->result(id, e, lat, iters);
+}
+void server_stats(void *p, double *out) { static_cast<Server *>(p)->stats(out); }
+
+}  // namespace ommp

@@ -249,6 +249,34 @@ class OMMPSystem:
         lib().ommp_amd_set_external_field_device(self.handler, ctypes.c_void_p(d_ext_ptr),
                                                  available_solvers[solver])
 
+    def set_external_fields_multi(self, ext):
+        """Solve nrhs external fields together (batched mixed solver).
+
+        ext: [nrhs, pol_atoms, 3].  Returns (ipd [nrhs, n_ipd, pol_atoms, 3],
+        epol [nrhs]).  The system's own dipoles are not changed."""
+        ext = np.ascontiguousarray(ext, dtype=np.float64).reshape(-1, self.pol_atoms, 3)
+        nrhs = ext.shape[0]
+        ipd = np.zeros((nrhs, self.n_ipd, self.pol_atoms, 3))
+        ep = np.zeros(nrhs)
+        lib().ommp_amd_set_external_fields_multi(self.handler, nrhs, ext.ctypes.data_as(c_dp),
+                                                 ipd.ctypes.data_as(c_dp), ep.ctypes.data_as(c_dp))
+        return ipd, ep
+
+    def set_external_fields_multi_device(self, nrhs, d_ext_ptr, d_ipd_ptr=None):
+        """Device-pointer variant; returns the energies [nrhs]."""
+        ep = np.zeros(nrhs)
+        lib().ommp_amd_set_external_fields_multi_device(self.handler, int(nrhs), ctypes.c_void_p(d_ext_ptr),
+                                                        ctypes.c_void_p(d_ipd_ptr or 0),
+                                                        ep.ctypes.data_as(c_dp))
+        return ep
+
+    def charges_field(self, charges):
+        """Field of point charges [nq, 4] (x, y, z, q) at the polarizable sites."""
+        q = np.ascontiguousarray(charges, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros((self.pol_atoms, 3))
+        lib().ommp_amd_charges_field(self.handler, q.shape[0], q.ctypes.data_as(c_dp), out.ctypes.data_as(c_dp))
+        return out
+
     def device_ipd_ptr(self):
         return lib().ommp_amd_device_ipd(self.handler)
 
