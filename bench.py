@@ -109,6 +109,207 @@ def run_mode(spec, mode, tol, steps, warmup, d_zero_ptr):
                 status=status)
 
 
+class DeviceBuffers:
+    """Raw device allocations through the HIP runtime (inputs resident in HBM)."""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.ptrs = []
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p = self.alloc(a.nbytes)
+        assert self.hip.hipMemcpy(self.ct.c_void_p(p), a.ctypes.data_as(self.ct.c_void_p),
+                                  self.ct.c_size_t(a.nbytes), 1) == 0
+        return p
+
+    def alloc(self, nbytes):
+        p = self.ct.c_void_p()
+        assert self.hip.hipMalloc(self.ct.byref(p), self.ct.c_size_t(max(nbytes, 8))) == 0
+        self.ptrs.append(p)
+        return p.value
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
+MULTI_NRHS, MULTI_QSEED = 64, 21
+
+
+def run_multi(tol, steps, bufs):
+    """64 QM-region fields on the 15k-site system with a 6 A cavity: one batched
+    call (all columns share every launch) against the same fields solved one
+    after another with the single-field entry (each warm-started from the
+    previous one), mixed and plain FP64."""
+    from openmmpol_amd import synth
+    spec = synth.sphere_system(N_SITES, RADIUS, seed=SEED, cavity=6.0)
+    charges = synth.qm_charge_sets(MULTI_NRHS, MULTI_QSEED)
+    s = synth.make_system(spec)
+    cpol = spec["cmm"][np.asarray(s.polar_mm)]
+    ext = np.stack([synth.point_charge_field(cpol, q[:, :3], q[:, 3]) for q in charges])
+    n = s.pol_atoms
+    d_ext = bufs.upload(ext)
+    d_ipd = bufs.alloc(MULTI_NRHS * 2 * 3 * n * 8)
+    s.set_precision("mixed")
+    s.set_solver_tolerance(tol, 500)
+    # the previous solve: the undisplaced QM field; its dipoles are every column's guess
+    s.set_external_field_device(d_ext)
+    s.get_polelec_energy()
+    ep = s.set_external_fields_multi_device(MULTI_NRHS, d_ext, d_ipd)  # warm-up (allocates batch state)
+    t = time.perf_counter()
+    for _ in range(steps):
+        ep = s.set_external_fields_multi_device(MULTI_NRHS, d_ext, d_ipd)
+    t_multi = (time.perf_counter() - t) / steps
+    it_multi = s.last_iterations
+    ipd0 = np.zeros((2, n, 3))
+    bufs.hip.hipMemcpy(ipd0.ctypes.data_as(bufs.ct.c_void_p), bufs.ct.c_void_p(d_ipd),
+                       bufs.ct.c_size_t(ipd0.nbytes), 2)
+
+    def sequential(mode, count):
+        s.set_precision(mode)
+        s.reset_guess(False)
+        s.set_external_field_device(d_ext)  # same starting point as the batched call
+        s.get_polelec_energy()
+        e = []
+        t0 = time.perf_counter()
+        for k in range(count):
+            s.set_external_field_device(d_ext + k * 3 * n * 8)
+            e.append(s.get_polelec_energy())
+        return (time.perf_counter() - t0) / count, np.array(e)
+
+    t_seq_mixed, e_seq_mixed = sequential("mixed", MULTI_NRHS)
+    t_seq_fp64, e_seq_fp64 = sequential("fp64", 16)
+    s.terminate()
+    gpath = os.path.join(ROOT, "tests", "golden", "multi15000_cavity.json")
+    err_e, err_mu = None, None
+    if os.path.exists(gpath):
+        with open(gpath) as f:
+            g = json.load(f)
+        err_e = max(abs(ep[k] - r["epol"]) / abs(r["epol"]) for k, r in enumerate(g["rhs"]))
+        npy = os.path.join(ROOT, "tests", "golden", "multi15000_cavity_ipd0.npy")
+        if os.path.exists(npy):
+            err_mu = float(np.abs(ipd0 - np.load(npy)).max())
+    return {
+        "workload": f"15000-site AMOEBA sphere with a 6 A cavity, {MULTI_NRHS} fields of a 30-charge QM region "
+                    "displaced N(0, 0.1 A) per field; guess = dipoles of the undisplaced-field solve",
+        "rhs_solves_per_s": MULTI_NRHS / t_multi, "ms_per_call": t_multi * 1e3,
+        "iterations": {"outer_fp64_max": it_multi[2], "inner_fp32_max": it_multi[3]},
+        "sequential_mixed_solves_per_s": 1.0 / t_seq_mixed,
+        "sequential_fp64_solves_per_s": 1.0 / t_seq_fp64,
+        "speedup_vs_sequential_mixed": t_seq_mixed / t_multi * MULTI_NRHS,
+        "speedup_vs_sequential_fp64": t_seq_fp64 / t_multi * MULTI_NRHS,
+        "epol_rel_err_vs_golden_max": err_e, "mu_maxabs_err_rhs0": err_mu,
+        "epol_max_rel_diff_vs_sequential_fp64": float(np.max(np.abs(ep[:16] - e_seq_fp64) / np.abs(e_seq_fp64))),
+    }
+
+
+def run_serving(tol, nreq, seed, mean_gap_ms, max_active, bufs):
+    """Poisson arrivals of SCF-like requests on systems of log-uniform size
+    500-40,000 sites; the continuous-batching server against the same requests
+    run one after another through the single-field entry in plain FP64."""
+    from openmmpol_amd import Server, synth
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(40000), size=nreq)).astype(int)
+    nsteps = rng.integers(10, 21, size=nreq)
+    gaps = rng.exponential(mean_gap_ms * 1e-3, size=nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    specs, charges = [], []
+    for k in range(nreq):
+        radius = (3.0 * sizes[k] / (4.0 * np.pi * 0.1)) ** (1.0 / 3.0) + 3.0
+        specs.append(synth.sphere_system(int(sizes[k]), radius, seed=1000 + k, cavity=6.0))
+        base = synth.qm_charge_sets(1, 2000 + k)[0]
+        steps = np.repeat(base[None], nsteps[k], axis=0)
+        steps[:, :, :3] += rng.normal(0.0, 0.02 * 1.8897261245650, size=(nsteps[k], base.shape[0], 3))
+        charges.append(steps)
+    total = int(nsteps.sum())
+    # ---- server
+    systems = [synth.make_system(sp) for sp in specs]
+    for x in systems:
+        x.set_solver_tolerance(tol, 500)
+    srv = Server(hbm_budget=200e9, max_active=max_active)
+    t0 = time.perf_counter()
+    ids = []
+    for k in range(nreq):
+        dt = t0 + arrivals[k] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(systems[k], charges[k]))
+    srv.wait()
+    makespan = time.perf_counter() - t0
+    lat, e_srv, its = [], [], []
+    for rid in ids:
+        st, e, la, it = srv.result(rid)
+        assert st == 0
+        lat.extend(la.tolist())
+        e_srv.append(e)
+        its.append(it)
+    its = np.concatenate(its)
+    stats = srv.stats()
+    srv.close()
+    for x in systems:
+        x.terminate()
+    # ---- sequential plain FP64, one solve after another (fields precomputed in HBM)
+    systems = [synth.make_system(sp) for sp in specs]
+    fields = []
+    for k in range(nreq):
+        cpol = specs[k]["cmm"][np.asarray(systems[k].polar_mm)]
+        f = np.stack([synth.point_charge_field(cpol, q[:, :3], q[:, 3]) for q in charges[k]])
+        fields.append((bufs.upload(f), f.shape[1]))
+    seq_lat, e_seq, seq_it = [], [], []
+    t = time.perf_counter()
+    for k in range(nreq):
+        x = systems[k]
+        x.set_solver_tolerance(tol, 500)
+        ee = []
+        d_f, npol = fields[k]
+        for j in range(nsteps[k]):
+            t1 = time.perf_counter()
+            x.set_external_field_device(d_f + j * 3 * npol * 8)
+            ee.append(x.get_polelec_energy())
+            seq_lat.append((time.perf_counter() - t1) * 1e3)
+            seq_it.append(x.last_iterations[:2])
+        e_seq.append(np.array(ee))
+    t_seq = time.perf_counter() - t
+    # accuracy of both against tight solves (mixed solver, tolerance 1e-12), untimed
+    e_tight = []
+    for k in range(nreq):
+        x = systems[k]
+        x.set_precision("mixed")
+        x.set_solver_tolerance(1e-12, 1000)
+        x.reset_guess(False)
+        d_f, npol = fields[k]
+        ee = []
+        for j in range(nsteps[k]):
+            x.set_external_field_device(d_f + j * 3 * npol * 8)
+            ee.append(x.get_polelec_energy())
+        e_tight.append(np.array(ee))
+        x.terminate()
+
+    def max_rel(ea):
+        return max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(ea, e_tight))
+    lat = np.array(lat)
+    return {
+        "workload": (f"{nreq} requests, sizes log-uniform 500-40000 sites (seed {seed}), 10-20 solves each "
+                     f"({total} solves), 30-charge QM field perturbed N(0, 0.02 A) per solve, Poisson arrivals "
+                     f"every {mean_gap_ms} ms on average, up to {max_active} requests admitted"),
+        "solves_per_s": total / makespan, "makespan_s": makespan,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "sequential_fp64_solves_per_s": total / t_seq,
+        "sequential_fp64_p50_ms": float(np.percentile(seq_lat, 50)),
+        "sequential_fp64_p99_ms": float(np.percentile(seq_lat, 99)),
+        "speedup_vs_sequential_fp64": t_seq / makespan,
+        "epol_max_rel_err_server": max_rel(e_srv),
+        "epol_max_rel_err_sequential_fp64": max_rel(e_seq),
+        "iterations_server_mean": {"outer_fp64": float(its[:, 0].mean()), "inner_fp32": float(its[:, 1].mean())},
+        "iterations_sequential_fp64_mean": float(np.mean(seq_it)),
+        "server_stats": stats,
+    }
+
+
 def cpu_baseline(spec, fp64_iters):
     """Oracle (FP64 C/OpenMP restatement of the reference) on the host cores:
     static field once + 8 T matvecs of the same system, extrapolated to one
@@ -146,6 +347,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
     ap.add_argument("--no-incore", action="store_true")
+    ap.add_argument("--no-multi", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=16)
+    ap.add_argument("--serving-gap-ms", type=float, default=10.0)
+    ap.add_argument("--serving-slots", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -170,6 +376,17 @@ def main():
         fp64 = run_mode(spec, "fp64", args.tol, max(3, args.steps // 4), 1, d_zero.value)
         print(f"[bench] fp64: {len(fp64['lat']) / fp64['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
     hip.hipFree(d_zero)
+    bufs = DeviceBuffers()
+    multi = None
+    if not args.no_multi:
+        multi = run_multi(args.tol, 3, bufs)
+        print(f"[bench] multi-RHS: {multi['rhs_solves_per_s']:.1f} rhs solves/s", file=sys.stderr, flush=True)
+    bufs.free()
+    serving = None
+    if not args.no_serving:
+        serving = run_serving(args.tol, args.serving_requests, 5, args.serving_gap_ms, args.serving_slots, bufs)
+        bufs.free()
+        print(f"[bench] serving: {serving['solves_per_s']:.1f} solves/s", file=sys.stderr, flush=True)
     incore = None
     if not args.no_incore:
         incore = run_incore(spec, 20)
@@ -219,6 +436,10 @@ def main():
         "kernel_time_ms": {k: v[1] for k, v in mixed["stats"].items()},
         "kernel_launches": {k: v[0] for k, v in mixed["stats"].items()},
     }
+    if multi is not None:
+        out["multi_rhs"] = multi
+    if serving is not None:
+        out["serving"] = serving
     if incore is not None:
         b = incore_bytes(n, 2, 4)
         out["incore_fp32_matvec"] = {

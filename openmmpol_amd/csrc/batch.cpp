@@ -317,9 +317,14 @@ void BatchEngine::set_rhs(int seg) {
     launch_batch_rhs(stream, 3L * S.dev.npol, S.nsets, S.ncols, S.view.e_m2d, S.ext, S.dev.b);
 }
 
-void BatchEngine::start(int seg, double tol, int maxit, bool xzero) {
+void BatchEngine::shift_rhs(int seg) {
     const SegInfo &S = segs[seg];
-    launch_batch_start(stream, d_cols, S.col0, S.ncols, tol, maxit, xzero ? 1 : 0);
+    launch_batch_shift_rhs(stream, 3L * S.dev.npol, S.nsets, S.ncols, S.view.e_m2d, S.ext, S.dev.b, S.dev.r);
+}
+
+void BatchEngine::start(int seg, double tol, int maxit, int rinit) {
+    const SegInfo &S = segs[seg];
+    launch_batch_start(stream, d_cols, S.col0, S.ncols, tol, maxit, rinit);
 }
 
 void BatchEngine::energies(int seg) {

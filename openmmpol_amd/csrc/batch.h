@@ -31,6 +31,10 @@ namespace ommp {
 
 constexpr int kPhaseDone = 0, kPhaseOuter = 1, kPhaseInner = 2;
 constexpr int kBatchMaxOuter = 30;
+// Where the first FP64 residual of a solve comes from:
+constexpr int kResMatvec = 0;  // r = b - T x (FP64 matvec)
+constexpr int kResXZero = 1;   // x = 0: r = b
+constexpr int kResKnown = 2;   // r already holds b - T x (same T, x, updated b)
 
 struct BColSub {
     double g;        // inner r.z
@@ -48,7 +52,7 @@ struct BCol {
     BColSub s[2];
     double tol;
     int maxit;       // inner iterations per refinement step
-    int xzero;       // x is exactly zero (first step of a cold start)
+    int rinit;       // first residual: kResMatvec, kResXZero or kResKnown
     int seg;
     int col;         // column within the segment
 };
@@ -113,8 +117,14 @@ void launch_batch_rhs(hipStream_t st, long n, int nsets, int ncols, const double
 void launch_charges_field(hipStream_t st, int npol, const double4 *ps64, int nq, const double4 *charges,
                           double *ext);
 // Start (or restart) columns [c0, c0 + nc) of the batch column array: phase
-// OUTER, counters cleared, tolerance set; xzero marks a zero initial guess.
-void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int xzero);
+// OUTER, counters cleared, tolerance set; rinit says how the first residual
+// is formed (kResMatvec, kResXZero, kResKnown).
+void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit);
+// New right-hand sides for columns whose x and FP64 residual r = b - T x are
+// those of their last solve (same geometry): b' = E_M2D + ext, r += b' - b,
+// b = b'.  The next solve then starts without an FP64 matvec (kResKnown).
+void launch_batch_shift_rhs(hipStream_t st, long n, int nsets, int ncols, const double *e_m2d, const double *ext,
+                            double *b, double *r);
 // E_pol per right-hand side, -1/2 mu_D . E_M2D,P (AMOEBA) or -1/2 mu . E_M2D:
 // x columns ordered (rhs, set); red scratch [nrhs][vblocks]; out[nrhs] on device.
 void launch_batch_energy(hipStream_t st, long n, int nsets, int nrhs, const double *x, const double *e_m2d,

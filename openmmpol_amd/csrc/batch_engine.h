@@ -54,8 +54,10 @@ class BatchEngine {
 
     // b = E_M2D + ext for every column of the segment.
     void set_rhs(int seg);
-    // Starts every column of the segment (phase OUTER).
-    void start(int seg, double tol, int maxit, bool xzero);
+    // Same, keeping r = b - T x of the last solve in step (see launch_batch_shift_rhs).
+    void shift_rhs(int seg);
+    // Starts every column of the segment (phase OUTER); rinit as launch_batch_start.
+    void start(int seg, double tol, int maxit, int rinit);
     // Energies of the segment's right-hand sides into seg.eout (device).
     void energies(int seg);
 

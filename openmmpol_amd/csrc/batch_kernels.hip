@@ -34,9 +34,9 @@ template <> struct SegSites<double> {
 };
 
 // Does column c take part in this matvec?  FP32 (use_p): inner PCG columns in
-// slot 1.  FP64: outer columns in slot 0 whose x is not identically zero.
+// slot 1.  FP64: outer columns in slot 0 whose residual needs T x.
 __device__ __forceinline__ bool mv_active(const BCol &c, int use_p) {
-    return use_p ? c.s[1].phase == kPhaseInner : (c.s[0].phase == kPhaseOuter && !c.xzero);
+    return use_p ? c.s[1].phase == kPhaseInner : (c.s[0].phase == kPhaseOuter && c.rinit == kResMatvec);
 }
 
 // Grouped on-the-fly pair kernel: one target per thread, sources of one split
@@ -235,7 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const BSeg *__restrict__ segs
 // Work item {segment, column, vector block, batch column}; element k of the
 // column belongs to site k / 3.
 
-// Outer step, part 1: r = b - T x (x = 0: r = b), z = alpha r, r.z partials.
+// Outer step, part 1: r = b - T x (x = 0: r = b; known: r kept), z = alpha r,
+// r.z partials.
 __global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                    const int4 *__restrict__ work) {
     __shared__ double sh[4];
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs
     const long o = (long)w.y * n + k;
     double v = 0.0;
     if (k < n) {
-        const double rr = C.xzero ? S.b[o] : S.b[o] - S.h[o];
+        const double rr = C.rinit == kResXZero ? S.b[o] : (C.rinit == kResKnown ? S.r[o] : S.b[o] - S.h[o]);
         const double zz = S.alpha[k / 3] * rr;
         S.r[o] = rr;
         S.z[o] = zz;
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ 
     }
     if (w.z == 0 && threadIdx.x == 0) {
         cols[w.w].s[0] = nst;
-        if (ended) cols[w.w].xzero = 0;
+        if (ended) cols[w.w].rinit = kResMatvec;
     }
 }
 
@@ -432,7 +433,7 @@ void launch_charges_field(hipStream_t st, int npol, const double4 *ps64, int nq,
     HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit, int xzero) {
+__global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit, int rinit) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
     BCol &C = cols[c0 + c];
@@ -450,13 +451,33 @@ __global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit,
     C.s[1] = s;
     C.tol = tol;
     C.maxit = maxit;
-    C.xzero = xzero;
+    C.rinit = rinit;
 }
 
-void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int xzero) {
+void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit) {
     if (nc == 0) return;
     hipLaunchKernelGGL(k_batch_start, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cols, c0, nc, tol,
-                       maxit, xzero);
+                       maxit, rinit);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_batch_shift_rhs(long n, int nsets, int ncols, const double *__restrict__ e_m2d,
+                                  const double *__restrict__ ext, double *__restrict__ b, double *__restrict__ r) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= n || c >= ncols) return;
+    const int rhs = c / nsets, set = c % nsets;
+    const long o = (long)c * n + k;
+    const double bn = e_m2d[(long)set * n + k] + ext[(long)rhs * n + k];
+    r[o] += bn - b[o];
+    b[o] = bn;
+}
+
+void launch_batch_shift_rhs(hipStream_t st, long n, int nsets, int ncols, const double *e_m2d, const double *ext,
+                            double *b, double *r) {
+    if (n == 0 || ncols == 0) return;
+    hipLaunchKernelGGL(k_batch_shift_rhs, dim3((unsigned)((n + kBlock - 1) / kBlock), ncols), dim3(kBlock), 0, st,
+                       n, nsets, ncols, e_m2d, ext, b, r);
     HIP_CHECK(hipGetLastError());
 }
 

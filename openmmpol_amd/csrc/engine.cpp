@@ -844,7 +844,7 @@ void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *i
     } else {
         HCHECK(hipMemsetAsync(S.dev.x, 0, sizeof(double) * ncols * n, d.stream));
     }
-    E.start(0, s.tol, s.max_iter, !guess);
+    E.start(0, s.tol, s.max_iter, guess ? kResMatvec : kResXZero);
     const double eta = 1e-5;
     const long max_ticks = (long)(kBatchMaxOuter + 1) * (s.max_iter + 2);
     long prev = -1;

@@ -222,13 +222,17 @@ class Server {
         return true;
     }
 
-    void start_step(Request *r, bool xzero) {
+    // First step: x = 0.  Later steps: x is the last solve's dipoles and the
+    // geometry is unchanged, so r = b - T x follows from the last FP64
+    // residual and the change of b without another FP64 matvec.
+    void start_step(Request *r, bool first) {
         SegInfo &S = engine_->segs[r->seg];
         System &s = *r->sys;
         launch_charges_field(stream_, s.npol, S.view.ps64, r->nq,
                              S.charges + (long)r->step * r->nq, S.ext);
-        engine_->set_rhs(r->seg);
-        engine_->start(r->seg, s.tol, s.max_iter, xzero);
+        if (first) engine_->set_rhs(r->seg);
+        else engine_->shift_rhs(r->seg);
+        engine_->start(r->seg, s.tol, s.max_iter, first ? kResXZero : kResKnown);
         r->valid_from = engine_->next_readback();
     }
 

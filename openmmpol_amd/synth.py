@@ -275,6 +275,20 @@ def qm_charges(rng, nq=30, cavity=6.0, sigma_q=0.5):
     return np.array(pos) * ANG2AU, qq
 
 
+def qm_charge_sets(nrhs, seed, nq=30, cavity=6.0, sigma_q=0.5, sigma_disp=0.1):
+    """A 'QM region' of nq charges in the central cavity and nrhs copies of it:
+    copy 0 as drawn, copy k > 0 with every charge displaced N(0, sigma_disp)
+    Angstrom.  Returns [nrhs, nq, 4] (x, y, z in bohr, q in e)."""
+    rng = np.random.default_rng(seed)
+    cq, qq = qm_charges(rng, nq, cavity=cavity, sigma_q=sigma_q)
+    out = np.zeros((nrhs, nq, 4))
+    for k in range(nrhs):
+        d = rng.normal(0.0, sigma_disp, size=(nq, 3)) * ANG2AU if k > 0 else 0.0
+        out[k, :, :3] = cq + d
+        out[k, :, 3] = qq
+    return out
+
+
 def point_charge_field(cpol_au, cq_au, qq):
     """E_j = sum_c q_c (r_j - r_c) / |r_j - r_c|^3 at every polarizable site (a.u.)."""
     E = np.zeros_like(cpol_au)
