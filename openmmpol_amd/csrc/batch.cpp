@@ -293,9 +293,9 @@ void BatchEngine::commit() {
     dirty_ = false;
 }
 
-void BatchEngine::tick(double eta) {
+void BatchEngine::tick(double eta, double frac) {
     commit();
-    launch_batch_tick(stream, tables_, amoeba, m32, m64, eta);
+    launch_batch_tick(stream, tables_, amoeba, m32, m64, eta, frac);
 }
 
 long BatchEngine::enqueue_readback() {

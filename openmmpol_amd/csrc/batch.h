@@ -106,7 +106,9 @@ inline int batch_m64(int ncols) { return ncols <= 2 ? 2 : kBatchM64; }
 
 // One tick over every column of the batch (see the file comment).
 // m32/m64: column-group widths of this batch (all segments share them).
-void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m32, int m64, double eta);
+// Inner solves stop at max(eta * outer rms, frac * tol).
+void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m32, int m64, double eta,
+                       double frac);
 
 // b[col] = E_M2D[set of col] + ext[rhs of col] for columns [c0, c0 + ncols) of a
 // segment whose columns are ordered (rhs, set); ext [nrhs][3 npol] on device.

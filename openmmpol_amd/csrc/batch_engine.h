@@ -43,7 +43,7 @@ class BatchEngine {
 
     // Uploads descriptors and work lists when the composition changed.
     void commit();
-    void tick(double eta);
+    void tick(double eta, double frac);
     // Copies the column states to the next pinned snapshot; returns its number.
     long enqueue_readback();
     // Waits for snapshot number seq and returns its column array.

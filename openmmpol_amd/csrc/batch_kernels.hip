@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs
 // Outer step, part 2 (state copy 0 -> 1): rms test; if not converged, start an
 // inner PCG on T d = r from d = 0 (p = z, g = r.z).
 __global__ __launch_bounds__(kBlock) void k_bdecide(const BSeg *__restrict__ segs, BCol *__restrict__ cols,
-                                                    const int4 *__restrict__ work, double eta) {
+                                                    const int4 *__restrict__ work, double eta, double frac) {
     __shared__ double sh[4];
     const int4 w = work[blockIdx.x];
     const BCol &C = cols[w.w];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_bdecide(const BSeg *__restrict__ seg
             nst.status = 2;
         } else {
             nst.phase = kPhaseInner;
-            nst.tin = fmax(eta * rms, 0.5 * C.tol);
+            nst.tin = fmax(eta * rms, frac * C.tol);
             nst.g = rz;
             nst.it = 0;
             nst.outer = st.outer + 1;
@@ -369,11 +369,11 @@ static void mv_launch(hipStream_t st, const BatchTables &t, const int4 *pair, in
 }
 
 template <bool AMOEBA>
-static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, double eta) {
+static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, double eta, double frac) {
     if (m64 == 2) mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
     else mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
     hipLaunchKernelGGL(k_bresid, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
-    hipLaunchKernelGGL(k_bdecide, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec, eta);
+    hipLaunchKernelGGL(k_bdecide, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec, eta, frac);
     if (m32 == 2) mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
     else mv_launch<float, kBatchM32, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
     hipLaunchKernelGGL(k_bupdate, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
@@ -381,10 +381,11 @@ static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, doubl
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m32, int m64, double eta) {
+void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m32, int m64, double eta,
+                       double frac) {
     if (t.nvec == 0) return;
-    if (amoeba) tick_t<true>(st, t, m32, m64, eta);
-    else tick_t<false>(st, t, m32, m64, eta);
+    if (amoeba) tick_t<true>(st, t, m32, m64, eta, frac);
+    else tick_t<false>(st, t, m32, m64, eta, frac);
 }
 
 // ------------------------------------------------------------------ helpers

@@ -557,11 +557,8 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
     const long n = d.n;
     MixedResult res;
     const int max_outer = 30;
-    // Inner target per refinement step: straight to the tolerance when the
-    // FP32 operator can deliver it, otherwise a 1e-5 reduction (the FP32 T
-    // matvec is accurate to ~1e-6 relative, so deeper inner solves would not
-    // lower the FP64 residual further).
-    const double eta_min = 1e-5;
+    // Inner target per refinement step: max(refine_eta() * rms, refine_frac() * tol).
+    const double eta_min = refine_eta(), frac = refine_frac();
     bool xzero = zero_guess;
     for (int outer = 0; outer < max_outer; outer++) {
         // FP64 residual r = b - T x and its preconditioned rms per column
@@ -580,7 +577,7 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
         for (int m = 0; m < M; m++) {
             const double rms = d.scal_host[m];
             conv = conv && rms < tol;
-            tin[m] = rms < tol ? 1e300 : std::max(eta_min * rms, 0.5 * tol);
+            tin[m] = rms < tol ? 1e300 : std::max(eta_min * rms, frac * tol);
         }
         if (conv) {
             res.ok = true;
@@ -596,6 +593,13 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
 }
 
 }  // namespace
+
+// The FP32 T matvec is accurate to ~1e-7 relative, so an inner solve can
+// reduce the FP64 residual by ~1e-6 before its recursive residual drifts from
+// the true one; asking for that much (instead of stopping at the tolerance)
+// makes one refinement step enough for most warm-started solves.
+double refine_eta() { return 1e-6; }
+double refine_frac() { return 0.5; }
 
 // ===================================================================== API
 void device_release(System &s) {
@@ -845,12 +849,11 @@ void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *i
         HCHECK(hipMemsetAsync(S.dev.x, 0, sizeof(double) * ncols * n, d.stream));
     }
     E.start(0, s.tol, s.max_iter, guess ? kResMatvec : kResXZero);
-    const double eta = 1e-5;
     const long max_ticks = (long)(kBatchMaxOuter + 1) * (s.max_iter + 2);
     long prev = -1;
     const BCol *snap = nullptr;
     for (long t = 0;; t++) {
-        E.tick(eta);
+        E.tick(refine_eta(), refine_frac());
         const long seq = E.enqueue_readback();
         if (prev >= 0) {
             snap = E.wait_readback(prev);

@@ -101,6 +101,9 @@ void message(const std::string &msg, int level, const char *prefix = nullptr);
 int verbosity();
 
 // engine.cpp
+// Inner-solve target of the mixed refinement: max(eta * outer rms, frac * tol).
+double refine_eta();
+double refine_frac();
 void device_release(System &s);
 void prepare_polelec(System &s);                              // E_M2D on device
 void polarization(System &s, const double *d_field, int solver, int matv, int mask);

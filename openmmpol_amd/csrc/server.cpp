@@ -317,7 +317,7 @@ class Server {
                 continue;
             }
             lk.unlock();
-            engine_->tick(kEta);
+            engine_->tick(refine_eta(), refine_frac());
             const long seq = engine_->enqueue_readback();
             int live = 0;
             if (prev >= 0) {
@@ -350,7 +350,6 @@ class Server {
         }
     }
 
-    static constexpr double kEta = 1e-5;
     double budget_;
     int max_active_;
     int num_cu_ = 256;
