@@ -24,7 +24,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct SegLayout {
     int nsplit, chunk, tblocks, vblocks;
-    size_t vec, part, red_hp, red_rz, ext, ered, eout, charges, total;
+    size_t vec, pf, part, red_hp, red_rz, ext, ered, eout, charges, total;
 };
 
 SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu) {
@@ -33,10 +33,17 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu) {
     const int ncols = nrhs * nsets;
     L.tblocks = site_blocks(npol);
     L.vblocks = vec_blocks(n);
-    const int groups = (ncols + m32 - 1) / m32;
-    L.nsplit = balanced_split(L.tblocks * groups, npol, num_cu);
+    if (m32 == kBatchM32) {
+        // matrix-core FP32 matvec: 32-site target tiles x 64-column groups
+        const int items = ((npol + kMfmaSites - 1) / kMfmaSites) * ((ncols + kMfmaCols - 1) / kMfmaCols);
+        L.nsplit = balanced_split(items, npol, num_cu);
+    } else {
+        const int groups = (ncols + m32 - 1) / m32;
+        L.nsplit = balanced_split(L.tblocks * groups, npol, num_cu);
+    }
     L.chunk = (npol + L.nsplit - 1) / L.nsplit;
     L.vec = align256(sizeof(double) * ncols * n);
+    L.pf = m32 == kBatchM32 ? align256(sizeof(float) * ncols * n) : 0;
     L.part = align256(sizeof(double) * L.nsplit * ncols * n);
     L.red_hp = align256(sizeof(double) * ncols * L.tblocks);
     L.red_rz = align256(sizeof(double) * ncols * L.vblocks);
@@ -44,7 +51,7 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu) {
     L.ered = align256(sizeof(double) * nrhs * L.vblocks);
     L.eout = align256(sizeof(double) * nrhs);
     L.charges = align256(sizeof(double4) * kMaxCharges);
-    L.total = 7 * L.vec + L.part + L.red_hp + L.red_rz + L.ext + L.ered + L.eout + L.charges;
+    L.total = 7 * L.vec + L.pf + L.part + L.red_hp + L.red_rz + L.ext + L.ered + L.eout + L.charges;
     return L;
 }
 
@@ -179,6 +186,8 @@ int BatchEngine::add(const DeviceView &v, int nrhs) {
         *pv = reinterpret_cast<double *>(mem + off);
         off += L.vec;
     }
+    d.pf = L.pf ? reinterpret_cast<float *>(mem + off) : nullptr;
+    off += L.pf;
     d.part = mem + off;
     off += L.part;
     d.red_hp = reinterpret_cast<double *>(mem + off);
@@ -229,9 +238,17 @@ void BatchEngine::commit() {
     for (int si : order) {
         const BSeg &d = segs[si].dev;
         const int g32 = (d.ncols + m32 - 1) / m32, g64 = (d.ncols + m64 - 1) / m64;
-        for (int g = 0; g < g32; g++)
-            for (int sp = 0; sp < d.nsplit; sp++)
-                for (int tb = 0; tb < d.tblocks; tb++) p32.push_back(make_int4(si, tb, sp, g));
+        if (m32 == kBatchM32) {
+            const int ttiles = (d.npol + kMfmaSites - 1) / kMfmaSites;
+            const int gm = (d.ncols + kMfmaCols - 1) / kMfmaCols;
+            for (int g = 0; g < gm; g++)
+                for (int sp = 0; sp < d.nsplit; sp++)
+                    for (int tt = 0; tt < ttiles; tt++) p32.push_back(make_int4(si, tt, sp, g));
+        } else {
+            for (int g = 0; g < g32; g++)
+                for (int sp = 0; sp < d.nsplit; sp++)
+                    for (int tb = 0; tb < d.tblocks; tb++) p32.push_back(make_int4(si, tb, sp, g));
+        }
         for (int g = 0; g < g32; g++)
             for (int tb = 0; tb < d.tblocks; tb++) f32.push_back(make_int4(si, tb, g, 0));
         for (int g = 0; g < g64; g++)

@@ -73,6 +73,7 @@ struct BSeg {
     const int *pp_ci;
     const double *pp_f;
     double *b, *x, *r, *z, *p, *h, *d;  // [ncols][3 npol]
+    float *pf;                           // [ncols][3 npol] FP32 copy of p (MFMA path)
     void *part;                          // [nsplit][ncols][3 npol], FP64-sized
     double *red_hp;                      // [ncols][tblocks]
     double *red_rz;                      // [ncols][vblocks]
@@ -97,7 +98,11 @@ struct BatchTables {
     int nvec;
 };
 
-constexpr int kBatchM32 = 8;  // columns per FP32 pair work item (multi-RHS)
+constexpr int kBatchM32 = 8;  // columns per FP32 finalize work item (multi-RHS)
+// Multi-RHS FP32 matvec on the matrix cores: work item = 32 target sites x 128
+// columns x one source split; T tiles of 32 x 16 sites built in LDS.
+constexpr int kMfmaSites = 32;
+constexpr int kMfmaCols = 128;
 constexpr int kBatchM64 = 4;  // columns per FP64 pair work item (multi-RHS)
 
 // Column-group width used for a segment of ncols columns.

@@ -148,6 +148,169 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
     }
 }
 
+// ------------------------------------------------- matrix-core FP32 matvec
+// Multi-RHS form of the on-the-fly matvec as a dense contraction on the matrix
+// cores: for a tile of 32 target sites (96 rows) and 128 columns, every tile of
+// 16 source sites contributes A (96 x 48, the damped dipole tensors k3 d d^T -
+// k2 I built in FP32 in LDS) times X (48 x 128, the columns' source dipoles in
+// LDS), accumulated by v_mfma_f32_16x16x4f32.  Wave w owns columns 32w..32w+31
+// (two 16-column blocks) and all six 16-row blocks.  Inside one instruction,
+// lane group g = lane / 16 supplies k = 12 g + kk, so each lane reads four
+// consecutive k with one 16-byte LDS load.  The LDS footprint (~47 KB) lets
+// three workgroups share a CU, so one workgroup's tile staging overlaps the
+// others' matrix-core work.  Partial fields go to part[split][column][3 npol]
+// as in k_bpairs; the finalize (k_bfinal) is shared.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaRows = 3 * kMfmaSites;  // 96
+constexpr int kMfmaSrc = 16;               // source sites per tile
+constexpr int kMfmaK = 3 * kMfmaSrc;       // 48
+constexpr int kMfmaLd = kMfmaK + 4;        // LDS row stride (floats): conflict-free 16-byte reads
+
+constexpr int kMfmaStage = kMfmaCols * kMfmaK / kBlock;  // X-tile elements per thread (24)
+
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_bpairs_mfma(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                        const int4 *__restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    const BSeg &S = segs[w.x];
+    const int npol = S.npol, ncols = S.ncols;
+    const long n = 3L * npol;
+    const int c0 = w.w * kMfmaCols;
+    const float4 *__restrict__ ps32 = S.ps32;
+    const float *__restrict__ pa32 = S.pa32;
+    const float *__restrict__ pf = S.pf + (long)c0 * n;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, l16 = lane & 15;
+    __shared__ float As[kMfmaRows][kMfmaLd];
+    __shared__ float Bs[kMfmaCols][kMfmaLd];
+    __shared__ float4 s_tp[kMfmaSites], s_sp[kMfmaSrc];
+    __shared__ float s_ta[kMfmaSites], s_sa[kMfmaSrc];
+    __shared__ int s_act[kMfmaCols];
+    int mine = 0;
+    if (tid < kMfmaCols) {
+        mine = c0 + tid < ncols && cols[S.col0 + c0 + tid].s[1].phase == kPhaseInner;
+        s_act[tid] = mine;
+    }
+    if (!__syncthreads_or(mine)) return;
+    const int t0 = w.y * kMfmaSites;
+    if (tid < kMfmaSites) {
+        const int t = t0 + tid;
+        float4 p;
+        float a;
+        if (t < npol) {
+            p = ps32[t];
+            a = pa32[t];
+        } else {
+            p = make_float4(0.f, 0.f, 0.f, 1.f);
+            a = 0.39f;
+        }
+        s_tp[tid] = p;
+        s_ta[tid] = a;
+    }
+    // Staging of one source tile (positions and the X tile) goes through
+    // registers: the loads of tile t + 1 are issued before the generation and
+    // matrix-core work of tile t and land in LDS after it.
+    const int i0 = w.z * S.chunk;
+    const int i1 = min(npol, i0 + S.chunk);
+    float xr[kMfmaStage];
+    float4 spr = make_float4(1e15f, 1e15f, 1e15f, 1.f);
+    float sar = 0.39f;
+    // thread t stages column t / 2, k = 24 (t % 2) .. 24 (t % 2) + 23
+    const int sc = tid >> 1, sk0 = (tid & 1) * kMfmaStage;
+    const bool sact = s_act[sc];
+    auto fetch = [&](int base) {
+        const int cnt = min(kMfmaSrc, i1 - base);
+        const int lim = 3 * cnt - sk0;
+        const float *src = pf + (long)sc * n + 3L * base + sk0;
+#pragma unroll
+        for (int j = 0; j < kMfmaStage; j++) xr[j] = (sact && j < lim) ? src[j] : 0.f;
+        if (tid < kMfmaSrc) {
+            if (tid < cnt) {
+                spr = ps32[base + tid];
+                sar = pa32[base + tid];
+            } else {
+                spr = make_float4(1e15f, 1e15f, 1e15f, 1.f);
+                sar = 0.39f;
+            }
+        }
+    };
+    mfma_f4 acc[2][6];
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+        for (int rb = 0; rb < 6; rb++) acc[cb][rb] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    if (i0 < i1) fetch(i0);
+    for (int base = i0; base < i1; base += kMfmaSrc) {
+        __syncthreads();  // previous tile's LDS reads are done
+#pragma unroll
+        for (int j = 0; j < kMfmaStage; j += 4)
+            *reinterpret_cast<float4 *>(&Bs[sc][sk0 + j]) = make_float4(xr[j], xr[j + 1], xr[j + 2], xr[j + 3]);
+        if (tid < kMfmaSrc) {
+            s_sp[tid] = spr;
+            s_sa[tid] = sar;
+        }
+        __syncthreads();
+        if (base + kMfmaSrc < i1) fetch(base + kMfmaSrc);
+        // A tile: damped dipole tensors of the 32 x 16 site pairs
+        for (int pidx = tid; pidx < kMfmaSites * kMfmaSrc; pidx += kBlock) {
+            const int ti = pidx / kMfmaSrc, sj = pidx - ti * kMfmaSrc;
+            float dx, dy, dz, k2, k3, arg;
+            bool near;
+            pair_geom<float, AMOEBA>(s_tp[ti], s_ta[ti], s_sp[sj], s_sa[sj], dx, dy, dz, k2, k3, arg, near);
+            pair_damp<float, AMOEBA>(arg, near, k2, k3);
+            const float d[3] = {dx, dy, dz};
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) As[3 * ti + a][3 * sj + b] = k3 * d[a] * d[b] - (a == b ? k2 : 0.f);
+        }
+        __syncthreads();
+        const float *arow = &As[l16][12 * g];
+        const float *bcol = &Bs[32 * wv + l16][12 * g];
+#pragma unroll
+        for (int kk = 0; kk < 12; kk += 4) {
+            const float4 b0 = *reinterpret_cast<const float4 *>(bcol + kk);
+            const float4 b1 = *reinterpret_cast<const float4 *>(bcol + 16 * kMfmaLd + kk);
+#pragma unroll
+            for (int rb = 0; rb < 6; rb++) {
+                const float4 av = *reinterpret_cast<const float4 *>(arow + rb * 16 * kMfmaLd + kk);
+                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, b0.x, acc[0][rb], 0, 0, 0);
+                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, b1.x, acc[1][rb], 0, 0, 0);
+                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, b0.y, acc[0][rb], 0, 0, 0);
+                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, b1.y, acc[1][rb], 0, 0, 0);
+                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, b0.z, acc[0][rb], 0, 0, 0);
+                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, b1.z, acc[1][rb], 0, 0, 0);
+                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, b0.w, acc[0][rb], 0, 0, 0);
+                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, b1.w, acc[1][rb], 0, 0, 0);
+            }
+        }
+    }
+    // lane holds D[16 rb + 4 g + i][l16] of column blocks 2 wv, 2 wv + 1
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        const int c = 32 * wv + 16 * cb + l16;
+        if (!s_act[c]) continue;
+        float *part = static_cast<float *>(S.part) + ((long)w.z * ncols + c0 + c) * n + 3L * t0;
+#pragma unroll
+        for (int rb = 0; rb < 6; rb++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = 16 * rb + 4 * g + i;
+                if (t0 + row / 3 < npol) part[row] = acc[cb][rb][i];
+            }
+    }
+}
+
+// FP32 copy of the inner search directions for the matrix-core matvec.
+__global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                     const int4 *__restrict__ work) {
+    const int4 w = work[blockIdx.x];
+    if (cols[w.w].s[1].phase != kPhaseInner) return;
+    const BSeg &S = segs[w.x];
+    const long n = 3L * S.npol;
+    const long k = (long)w.z * kBlock + threadIdx.x;
+    if (k < n) S.pf[(long)w.y * n + k] = (float)S.p[(long)w.y * n + k];
+}
+
 // h = -(sum of split partials) + (1 - s) corrections + x / alpha for the
 // active columns of one target block; FP32 passes also store h.p partials.
 template <typename R, int M, bool AMOEBA>
@@ -374,8 +537,14 @@ static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, doubl
     else mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
     hipLaunchKernelGGL(k_bresid, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
     hipLaunchKernelGGL(k_bdecide, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec, eta, frac);
-    if (m32 == 2) mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
-    else mv_launch<float, kBatchM32, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
+    if (m32 == 2) {
+        mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
+    } else if (t.npair32) {
+        hipLaunchKernelGGL(k_bconvert, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
+        hipLaunchKernelGGL(k_bpairs_mfma<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);
+        hipLaunchKernelGGL((k_bfinal<float, kBatchM32, AMOEBA>), dim3(t.nfin32), dim3(kBlock), 0, st, t.segs,
+                           t.cols, t.fin32, 1);
+    }
     hipLaunchKernelGGL(k_bupdate, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
     hipLaunchKernelGGL(k_bdirection, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
     HIP_CHECK(hipGetLastError());

@@ -87,3 +87,27 @@ def test_warm_started_scf_sequence(sphere):
     assert np.abs(s.ipd - ipd_warm).max() <= 1e-7
     assert abs(s.get_polelec_energy() - e_warm) <= 1e-9 * abs(e_warm)
     assert inner[-1] < inner[0]
+
+
+def test_multi_rhs_64_fields_matches_golden():
+    """64 QM-region fields on the 15,000-site cavity system in one batched call
+    (matrix-core FP32 block matvec, FP64 refinement per column) against the
+    oracle's tight solves of the first four (tests/golden/make_multi_golden.py)."""
+    from openmmpol_amd import synth
+    with open(os.path.join(GOLDEN, "multi15000_cavity.json")) as f:
+        g = json.load(f)
+    ref0 = np.load(os.path.join(GOLDEN, "multi15000_cavity_ipd0.npy"))
+    spec = synth.sphere_system(g["n"], g["radius"], seed=g["seed"], cavity=g["cavity"])
+    charges = synth.qm_charge_sets(g["nrhs"], g["qm_seed"])
+    s = synth.make_system(spec)
+    s.set_solver_tolerance(3e-10, 500)
+    cpol = spec["cmm"][np.asarray(s.polar_mm)]
+    ext = np.stack([synth.point_charge_field(cpol, q[:, :3], q[:, 3]) for q in charges])
+    ipd, ep = s.set_external_fields_multi(ext)
+    for k, r in enumerate(g["rhs"]):
+        assert abs(ep[k] - r["epol"]) <= 1e-9 * abs(r["epol"]), k
+        assert np.abs(ipd[k][:, g["sample"], :] - np.array(r["ipd_sample"])).max() < 1e-7, k
+    assert np.abs(ipd[0] - ref0).max() < 1e-7
+    # every other column is a converged solve of its own field: energies close to RHS 0's
+    assert np.all(np.abs(ep - ep[0]) < 1e-3 * abs(ep[0]))
+    s.terminate()
