@@ -310,6 +310,65 @@ def run_serving(tol, nreq, seed, mean_gap_ms, max_active, bufs):
     }
 
 
+def run_large(tol, bufs):
+    """60,000 sites in a 52 A sphere: mixed solves with the on-the-fly FP32
+    matvec and with the packed FP32 T in HBM (43 GB), plain FP64 CG, and the
+    packed-T stream rate."""
+    from openmmpol_amd import synth
+    n_sites, radius, seed = 60000, 52.0, 3
+    spec = synth.sphere_system(n_sites, radius, seed=seed)
+    s = synth.make_system(spec)
+    n = s.pol_atoms
+    d_zero = bufs.alloc(3 * n * 8)
+    bufs.hip.hipMemset(bufs.ct.c_void_p(d_zero), 0, bufs.ct.c_size_t(3 * n * 8))
+    s.set_solver_tolerance(tol, 500)
+
+    def solve(mode, matv, steps):
+        s.set_precision(mode)
+        s.set_default_matv(matv)
+        out = []
+        for k in range(steps + 1):  # the first (untimed) builds device state / packed T
+            t = time.perf_counter()
+            s.reset_guess(True)
+            s.set_external_field_device(d_zero)
+            e = s.get_polelec_energy()
+            if k:
+                out.append(time.perf_counter() - t)
+        return float(np.mean(out)), e, s.ipd.copy(), s.last_iterations
+
+    t_otf, e_otf, ipd_otf, it_otf = solve("mixed", "direct", 2)
+    t_inc, e_inc, ipd_inc, it_inc = solve("mixed", "incore", 2)
+    x = np.random.default_rng(1).normal(size=(2, n, 3))
+    s.profile_kernels(True)
+    s.kernel_stats(reset=True)
+    _, ms_inc = s.tmatvec(x, kind="incore32", repeat=5)
+    st = s.kernel_stats(reset=True)
+    s.profile_kernels(False)
+    t_64, e_64, ipd_64, it_64 = solve("fp64", "direct", 1)
+    s.terminate()
+    b = incore_bytes(n, 2, 4)
+    stream_ms = st["incore32"][1] / max(st["incore32"][0], 1)
+    res = {
+        "workload": f"{n_sites}-site AMOEBA sphere (R {radius} A, seed {seed}), static field + cold-start D+P "
+                    "solve + energy",
+        "mixed_otf_solves_per_s": 1.0 / t_otf, "mixed_incore_solves_per_s": 1.0 / t_inc,
+        "fp64_solves_per_s": 1.0 / t_64, "speedup_mixed_otf_vs_fp64": t_64 / t_otf,
+        "iterations": {"mixed_otf": it_otf, "mixed_incore": it_inc, "fp64": it_64[:2]},
+        "incore_fp32_stream": {"packed_T_bytes": b, "avg_ms": stream_ms,
+                               "achieved_GBs": b / (stream_ms * 1e-3) / 1e9, "peak_GBs": PEAK_HBM_GBS},
+        "mu_maxabs_mixed_otf_vs_incore": float(np.abs(ipd_otf - ipd_inc).max()),
+    }
+    gpath = os.path.join(ROOT, "tests", "golden", f"sphere{n_sites}_seed{seed}.json")
+    if os.path.exists(gpath):
+        with open(gpath) as f:
+            g = json.load(f)
+        smp = g["sample"]
+        for label, e, ipd in (("mixed_otf", e_otf, ipd_otf), ("mixed_incore", e_inc, ipd_inc), ("fp64", e_64, ipd_64)):
+            res[f"epol_rel_err_{label}"] = abs(e - g["epol"]) / abs(g["epol"])
+            res[f"mu_sample_maxabs_err_{label}"] = float(np.abs(ipd[:, smp, :] - np.array(g["ipd_sample"])).max())
+    return res
+
+
 def cpu_baseline(spec, fp64_iters):
     """Oracle (FP64 C/OpenMP restatement of the reference) on the host cores:
     static field once + 8 T matvecs of the same system, extrapolated to one
@@ -352,7 +411,14 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=16)
     ap.add_argument("--serving-gap-ms", type=float, default=10.0)
     ap.add_argument("--serving-slots", type=int, default=16)
+    ap.add_argument("--legs", default="headline,fp64,multi,serving,incore,large,cpu",
+                    help="comma-separated subset to run (profiling one leg); the default runs all")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    for flag, leg in ((args.no_cpu_baseline, "cpu"), (args.mixed_only, "fp64"), (args.no_incore, "incore"),
+                      (args.no_multi, "multi"), (args.no_serving, "serving")):
+        if flag:
+            legs.discard(leg)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
@@ -367,28 +433,36 @@ def main():
     d_zero = ctypes.c_void_p()
     assert hip.hipMalloc(ctypes.byref(d_zero), ctypes.c_size_t(3 * n * 8)) == 0
     assert hip.hipMemset(d_zero, 0, ctypes.c_size_t(3 * n * 8)) == 0
-    print(f"[bench] system ready: {n} sites; running mixed", file=sys.stderr, flush=True)
+    print(f"[bench] system ready: {n} sites", file=sys.stderr, flush=True)
 
-    mixed = run_mode(spec, "mixed", args.tol, args.steps, args.warmup, d_zero.value)
-    print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
+    mixed = None
+    if "headline" in legs:
+        mixed = run_mode(spec, "mixed", args.tol, args.steps, args.warmup, d_zero.value)
+        print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
     fp64 = None
-    if not args.mixed_only:
+    if "fp64" in legs:
         fp64 = run_mode(spec, "fp64", args.tol, max(3, args.steps // 4), 1, d_zero.value)
         print(f"[bench] fp64: {len(fp64['lat']) / fp64['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
     hip.hipFree(d_zero)
     bufs = DeviceBuffers()
     multi = None
-    if not args.no_multi:
+    if "multi" in legs:
         multi = run_multi(args.tol, 3, bufs)
         print(f"[bench] multi-RHS: {multi['rhs_solves_per_s']:.1f} rhs solves/s", file=sys.stderr, flush=True)
     bufs.free()
     serving = None
-    if not args.no_serving:
+    if "serving" in legs:
         serving = run_serving(args.tol, args.serving_requests, 5, args.serving_gap_ms, args.serving_slots, bufs)
         bufs.free()
         print(f"[bench] serving: {serving['solves_per_s']:.1f} solves/s", file=sys.stderr, flush=True)
+    large = None
+    if "large" in legs:
+        large = run_large(args.tol, bufs)
+        bufs.free()
+        print(f"[bench] 60k sites: mixed {large['mixed_otf_solves_per_s']:.2f} solves/s", file=sys.stderr,
+              flush=True)
     incore = None
-    if not args.no_incore:
+    if "incore" in legs:
         incore = run_incore(spec, 20)
         print(f"[bench] incore32 matvec: {incore[1]:.3f} ms per stream launch", file=sys.stderr, flush=True)
 
@@ -400,6 +474,14 @@ def main():
 
     def mu_err(ipd):
         return float(np.abs(ipd - gipd).max()) if gipd is not None else None
+    if mixed is None:
+        # a subset of the legs (profiling): no headline figure
+        out = {"legs": sorted(legs)}
+        for key, val in (("multi_rhs", multi), ("serving", serving), ("large_60k", large)):
+            if val is not None:
+                out[key] = val
+        print(json.dumps(out))
+        return
     value = args.steps / mixed["elapsed"]
     calls, ms = mixed["stats"]["otf32"]
     avg_ms = ms / max(calls, 1)
@@ -438,6 +520,8 @@ def main():
     }
     if multi is not None:
         out["multi_rhs"] = multi
+    if large is not None:
+        out["large_60k"] = large
     if serving is not None:
         out["serving"] = serving
     if incore is not None:
@@ -465,7 +549,7 @@ def main():
                                  "achieved_tflops": pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12
                                  if c64 else None, "peak": PEAK_FP64_TFLOPS},
         })
-        if not args.no_cpu_baseline:
+        if "cpu" in legs:
             print("[bench] cpu baseline", file=sys.stderr, flush=True)
             out["cpu_baseline"] = cpu_baseline(spec, fp64["iters"])
     print(json.dumps(out))

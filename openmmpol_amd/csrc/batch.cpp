@@ -27,10 +27,10 @@ struct SegLayout {
     size_t vec, pf, part, red_hp, red_rz, ext, ered, eout, charges, total;
 };
 
-SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu) {
+SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int extra) {
     SegLayout L;
     const long n = 3L * npol;
-    const int ncols = nrhs * nsets;
+    const int ncols = nrhs * nsets + extra;
     L.tblocks = site_blocks(npol);
     L.vblocks = vec_blocks(n);
     if (m32 == kBatchM32) {
@@ -99,8 +99,8 @@ void batch_release(BatchEngine *&e) {
     e = nullptr;
 }
 
-size_t BatchEngine::segment_bytes(int npol, int nrhs, int nsets, bool wide, int ncu) {
-    return seg_layout(npol, nrhs, nsets, wide ? kBatchM32 : 2, ncu).total;
+size_t BatchEngine::segment_bytes(int npol, int nrhs, int nsets, bool wide, int ncu, int extra) {
+    return seg_layout(npol, nrhs, nsets, wide ? kBatchM32 : 2, ncu, extra).total;
 }
 
 void *BatchEngine::pool_get(size_t bytes) {
@@ -124,9 +124,9 @@ void *BatchEngine::pool_get(size_t bytes) {
 
 void BatchEngine::pool_put(void *p, size_t bytes) { free_.push_back({p, bytes}); }
 
-int BatchEngine::add(const DeviceView &v, int nrhs) {
+int BatchEngine::add(const DeviceView &v, int nrhs, int extra) {
     const int nsets = v.n_ipd;
-    const int ncols = nrhs * nsets;
+    const int ncols = nrhs * nsets + extra;
     int slot = -1;
     for (int i = 0; i < (int)segs.size(); i++)
         if (!segs[i].used) {
@@ -150,7 +150,7 @@ int BatchEngine::add(const DeviceView &v, int nrhs) {
         }
     }
     if (col0 < 0) return -1;
-    const SegLayout L = seg_layout(v.npol, nrhs, nsets, m32, num_cu);
+    const SegLayout L = seg_layout(v.npol, nrhs, nsets, m32, num_cu, extra);
     // keep the pool's block size so that it can be reused by a larger system
     char *mem = static_cast<char *>(pool_get(L.total));
     if (!mem) return -1;
@@ -331,17 +331,23 @@ const BCol *BatchEngine::wait_readback(long seq) {
 
 void BatchEngine::set_rhs(int seg) {
     const SegInfo &S = segs[seg];
-    launch_batch_rhs(stream, 3L * S.dev.npol, S.nsets, S.ncols, S.view.e_m2d, S.ext, S.dev.b);
+    launch_batch_rhs(stream, 3L * S.dev.npol, S.nsets, S.nrhs * S.nsets, S.view.e_m2d, S.ext, S.dev.b);
 }
 
 void BatchEngine::shift_rhs(int seg) {
     const SegInfo &S = segs[seg];
-    launch_batch_shift_rhs(stream, 3L * S.dev.npol, S.nsets, S.ncols, S.view.e_m2d, S.ext, S.dev.b, S.dev.r);
+    launch_batch_shift_rhs(stream, 3L * S.dev.npol, S.nsets, S.nrhs * S.nsets, S.view.e_m2d, S.ext, S.dev.b,
+                           S.dev.r);
 }
 
 void BatchEngine::start(int seg, double tol, int maxit, int rinit) {
     const SegInfo &S = segs[seg];
-    launch_batch_start(stream, d_cols, S.col0, S.ncols, tol, maxit, rinit);
+    launch_batch_start(stream, d_cols, S.col0, S.nrhs * S.nsets, tol, maxit, rinit);
+}
+
+void BatchEngine::start_cols(int seg, int c, int nc, double tol, int maxit, int rinit, int oneshot) {
+    const SegInfo &S = segs[seg];
+    launch_batch_start(stream, d_cols, S.col0 + c, nc, tol, maxit, rinit, oneshot);
 }
 
 void BatchEngine::energies(int seg) {

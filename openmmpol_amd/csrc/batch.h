@@ -53,8 +53,8 @@ struct BCol {
     double tol;
     int maxit;       // inner iterations per refinement step
     int rinit;       // first residual: kResMatvec, kResXZero or kResKnown
-    int seg;
-    int col;         // column within the segment
+    int oneshot;     // stop after the first inner solve, x = its correction (no FP64 check)
+    int pad;
 };
 
 // One system in a batch; device pointers only.
@@ -126,7 +126,15 @@ void launch_charges_field(hipStream_t st, int npol, const double4 *ps64, int nq,
 // Start (or restart) columns [c0, c0 + nc) of the batch column array: phase
 // OUTER, counters cleared, tolerance set; rinit says how the first residual
 // is formed (kResMatvec, kResXZero, kResKnown).
-void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit);
+void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit,
+                        int oneshot = 0);
+// Shared correction of a warm SCF step (only the external field changed, by
+// the same amount for every dipole set): ext_new in bs on entry; on exit
+// bs = ext_new - ext, ext = ext_new, b[set] = E_M2D[set] + ext_new.
+void launch_batch_shared_rhs(hipStream_t st, long n, int nsets, const double *e_m2d, double *ext, double *b,
+                             double *bs);
+// x[set] += xs for every set (the shared correction applied to each set).
+void launch_batch_feed(hipStream_t st, long n, int nsets, double *x, const double *xs);
 // New right-hand sides for columns whose x and FP64 residual r = b - T x are
 // those of their last solve (same geometry): b' = E_M2D + ext, r += b' - b,
 // b = b'.  The next solve then starts without an FP64 matvec (kResKnown).

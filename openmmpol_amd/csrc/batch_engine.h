@@ -16,7 +16,7 @@ constexpr int kMaxCharges = 4096;  // point charges held per segment (all steps 
 
 struct SegInfo {
     bool used = false;
-    int nrhs = 0, nsets = 1, ncols = 0, col0 = 0;
+    int nrhs = 0, nsets = 1, ncols = 0, col0 = 0;  // ncols = nrhs * nsets + extra
     size_t bytes = 0;
     void *mem = nullptr;
     BSeg dev{};              // the descriptor uploaded to the device
@@ -34,12 +34,13 @@ class BatchEngine {
     BatchEngine(const BatchEngine &) = delete;
     BatchEngine &operator=(const BatchEngine &) = delete;
 
-    // Segment for a system with nrhs right-hand sides (n_ipd columns each).
-    // Returns the slot, or -1 when the segment or column capacity is full.
-    int add(const DeviceView &v, int nrhs);
+    // Segment for a system with nrhs right-hand sides (n_ipd columns each)
+    // plus `extra` work columns after them.  Returns the slot, or -1 when the
+    // segment or column capacity is full.
+    int add(const DeviceView &v, int nrhs, int extra = 0);
     void remove(int seg);
     // Bytes a segment for this system would take (for admission decisions).
-    static size_t segment_bytes(int npol, int nrhs, int nsets, bool wide, int num_cu);
+    static size_t segment_bytes(int npol, int nrhs, int nsets, bool wide, int num_cu, int extra = 0);
 
     // Uploads descriptors and work lists when the composition changed.
     void commit();
@@ -56,8 +57,11 @@ class BatchEngine {
     void set_rhs(int seg);
     // Same, keeping r = b - T x of the last solve in step (see launch_batch_shift_rhs).
     void shift_rhs(int seg);
-    // Starts every column of the segment (phase OUTER); rinit as launch_batch_start.
+    // Starts every right-hand-side column of the segment (phase OUTER);
+    // rinit as launch_batch_start.
     void start(int seg, double tol, int maxit, int rinit);
+    // Starts columns [c, c + nc) of the segment.
+    void start_cols(int seg, int c, int nc, double tol, int maxit, int rinit, int oneshot);
     // Energies of the segment's right-hand sides into seg.eout (device).
     void energies(int seg);
 

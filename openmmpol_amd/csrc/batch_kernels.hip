@@ -39,30 +39,18 @@ __device__ __forceinline__ bool mv_active(const BCol &c, int use_p) {
     return use_p ? c.s[1].phase == kPhaseInner : (c.s[0].phase == kPhaseOuter && c.rinit == kResMatvec);
 }
 
-// Grouped on-the-fly pair kernel: one target per thread, sources of one split
-// staged through LDS, M columns per work item (as k_tmv_pairs).
-template <typename R, int M, bool AMOEBA>
-__global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
-                                                   const int4 *__restrict__ work, int use_p) {
+// Grouped on-the-fly pair kernel body: one target per thread, sources of one
+// split staged through LDS, MM columns cidx[] (as k_tmv_pairs).
+template <typename R, int MM, bool AMOEBA>
+__device__ __forceinline__ void bpairs_body(const BSeg &S, const int4 w, const double *__restrict__ xin,
+                                            const int (&cidx)[MM]) {
     using R4 = typename V4<R>::t;
-    const int4 w = work[blockIdx.x];
-    const BSeg &S = segs[w.x];
     const int npol = S.npol, ncols = S.ncols;
     const long n = 3L * npol;
-    const int c0 = w.w * M;
-    const double *xin = use_p ? S.p : S.x;
-    bool act[M];
-    bool any = false;
-#pragma unroll
-    for (int m = 0; m < M; m++) {
-        act[m] = c0 + m < ncols && mv_active(cols[S.col0 + c0 + m], use_p);
-        any = any || act[m];
-    }
-    if (!any) return;
     const R4 *__restrict__ site = SegSites<R>::site(S);
     const R *__restrict__ adamp = SegSites<R>::damp(S);
     __shared__ R4 s_pos[kBlock];
-    __shared__ R4 s_x[M][kBlock];
+    __shared__ R4 s_x[MM][kBlock];
     const int tid = threadIdx.x;
     const int j = w.y * kBlock + tid;
     const bool valid = j < npol;
@@ -76,9 +64,9 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
         pj.w = R(1);
         aj = R(0.39);
     }
-    R acc[M][3];
+    R acc[MM][3];
 #pragma unroll
-    for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = R(0);
+    for (int m = 0; m < MM; m++) acc[m][0] = acc[m][1] = acc[m][2] = R(0);
     const int i0 = w.z * S.chunk;
     const int i1 = min(npol, i0 + S.chunk);
     for (int base = i0; base < i1; base += kBlock) {
@@ -88,10 +76,10 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
             const int i = base + tid;
             s_pos[tid] = site[i];
 #pragma unroll
-            for (int m = 0; m < M; m++) {
+            for (int m = 0; m < MM; m++) {
                 R4 v;
-                if (act[m]) {
-                    const double *xc = xin + (long)(c0 + m) * n + 3L * i;
+                if (cidx[m] >= 0) {
+                    const double *xc = xin + (long)cidx[m] * n + 3L * i;
                     v.x = (R)xc[0];
                     v.y = (R)xc[1];
                     v.z = (R)xc[2];
@@ -107,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
             far.w = R(1);
             s_pos[tid] = far;
 #pragma unroll
-            for (int m = 0; m < M; m++) {
+            for (int m = 0; m < MM; m++) {
                 R4 z;
                 z.x = z.y = z.z = R(0);
                 z.w = R(0.39);
@@ -117,9 +105,9 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
         __syncthreads();
         const int cnt2 = (cnt + 1) & ~1;
         for (int k = 0; k < cnt2; k += 2) {
-            R4 xa[M], xb[M];
+            R4 xa[MM], xb[MM];
 #pragma unroll
-            for (int m = 0; m < M; m++) {
+            for (int m = 0; m < MM; m++) {
                 xa[m] = s_x[m][k];
                 xb[m] = s_x[m][k + 1];
             }
@@ -131,21 +119,50 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
                 pair_damp<R, AMOEBA>(arga, neara, k2a, k3a);
                 pair_damp<R, AMOEBA>(argb, nearb, k2b, k3b);
             }
-            pair_accum<R, M>(acc, xa, dxa, dya, dza, k2a, k3a);
-            pair_accum<R, M>(acc, xb, dxb, dyb, dzb, k2b, k3b);
+            pair_accum<R, MM>(acc, xa, dxa, dya, dza, k2a, k3a);
+            pair_accum<R, MM>(acc, xb, dxb, dyb, dzb, k2b, k3b);
         }
     }
     if (valid) {
         R *part = static_cast<R *>(S.part);
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            if (!act[m]) continue;
-            R *o = part + ((long)w.z * ncols + c0 + m) * n + 3L * j;
+        for (int m = 0; m < MM; m++) {
+            if (cidx[m] < 0) continue;
+            R *o = part + ((long)w.z * ncols + cidx[m]) * n + 3L * j;
             o[0] = acc[m][0];
             o[1] = acc[m][1];
             o[2] = acc[m][2];
         }
     }
+}
+
+// Grouped pair kernel over work items {segment, target block, split, column
+// group of M}.  A group of two with one active column runs the one-column body
+// (the same per-pair arithmetic, so results do not depend on the other column).
+template <typename R, int M, bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                   const int4 *__restrict__ work, int use_p) {
+    const int4 w = work[blockIdx.x];
+    const BSeg &S = segs[w.x];
+    const int c0 = w.w * M;
+    const double *xin = use_p ? S.p : S.x;
+    int cidx[M];
+    int nact = 0;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const bool a = c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
+        cidx[m] = a ? c0 + m : -1;
+        nact += a;
+    }
+    if (nact == 0) return;
+    if constexpr (M == 2) {
+        if (nact == 1) {
+            const int one[1] = {cidx[0] >= 0 ? cidx[0] : cidx[1]};
+            bpairs_body<R, 1, AMOEBA>(S, w, xin, one);
+            return;
+        }
+    }
+    bpairs_body<R, M, AMOEBA>(S, w, xin, cidx);
 }
 
 // ------------------------------------------------- matrix-core FP32 matvec
@@ -508,7 +525,8 @@ __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ 
         const long o = (long)w.y * n + k;
         if (ended) {
             if (k < n) S.x[o] += S.d[o];
-            nst.phase = kPhaseOuter;
+            nst.phase = C.oneshot ? kPhaseDone : kPhaseOuter;
+            if (C.oneshot) nst.status = 0;
             nst.inner_total = st.inner_total + st.it + 1;
         } else {
             const double beta = gnew / st.g;
@@ -603,7 +621,7 @@ void launch_charges_field(hipStream_t st, int npol, const double4 *ps64, int nq,
     HIP_CHECK(hipGetLastError());
 }
 
-__global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit, int rinit) {
+__global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit, int rinit, int oneshot) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= nc) return;
     BCol &C = cols[c0 + c];
@@ -622,12 +640,46 @@ __global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit,
     C.tol = tol;
     C.maxit = maxit;
     C.rinit = rinit;
+    C.oneshot = oneshot;
 }
 
-void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit) {
+void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit,
+                        int oneshot) {
     if (nc == 0) return;
     hipLaunchKernelGGL(k_batch_start, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cols, c0, nc, tol,
-                       maxit, rinit);
+                       maxit, rinit, oneshot);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_batch_shared_rhs(long n, int nsets, const double *__restrict__ e_m2d, double *__restrict__ ext,
+                                   double *__restrict__ b, double *__restrict__ bs) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const double e = bs[k];
+    bs[k] = e - ext[k];
+    ext[k] = e;
+    for (int set = 0; set < nsets; set++) b[(long)set * n + k] = e_m2d[(long)set * n + k] + e;
+}
+
+void launch_batch_shared_rhs(hipStream_t st, long n, int nsets, const double *e_m2d, double *ext, double *b,
+                             double *bs) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_batch_shared_rhs, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n,
+                       nsets, e_m2d, ext, b, bs);
+    HIP_CHECK(hipGetLastError());
+}
+
+__global__ void k_batch_feed(long n, int nsets, double *__restrict__ x, const double *__restrict__ xs) {
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    const double v = xs[k];
+    for (int set = 0; set < nsets; set++) x[(long)set * n + k] += v;
+}
+
+void launch_batch_feed(hipStream_t st, long n, int nsets, double *x, const double *xs) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_batch_feed, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, n, nsets,
+                       x, xs);
     HIP_CHECK(hipGetLastError());
 }
 

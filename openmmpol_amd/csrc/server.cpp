@@ -12,6 +12,12 @@
 // slot and memory to the next waiting request.  Column states are read back one
 // tick late, so the GPU always has the next tick queued.
 //
+// Warm steps of an SCF sequence change only the external field, and by the
+// same amount for the D and P sets; the correction T^-1 (ext_new - ext_old) is
+// therefore solved once, in an extra FP32-refined column, added to both sets,
+// and each set is then checked (and if needed refined) against its own FP64
+// residual.
+//
 // Each solve's arithmetic depends only on its own system (batch.h), so a
 // request gives the same dipoles and energies alone and in any mix.
 #include <hip/hip_runtime.h>
@@ -62,6 +68,8 @@ struct Request {
     // progress (worker thread only)
     int seg = -1;
     int step = 0;
+    bool shared = false;   // the shared-correction column of a warm step is running
+    int inner_shared = 0;  // its inner iterations (reported with the step)
     long valid_from = 0;
     double t_ready = 0.0;
     size_t footprint = 0;
@@ -76,6 +84,10 @@ struct Request {
 };
 
 class Server {
+    // tolerance of the shared correction relative to the solve's tolerance:
+    // its error adds to each set's residual before the FP64 check
+    static constexpr double kSharedTolFrac = 0.25;
+
   public:
     Server(double budget, int max_active) : budget_(budget), max_active_(std::max(1, max_active)) {
         SCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -169,7 +181,7 @@ class Server {
             Request *r = pending_.front();
             const int nsets = r->sys->n_ipd;
             const size_t fp = device_footprint_estimate(*r->sys) +
-                              BatchEngine::segment_bytes(r->sys->npol, 1, nsets, false, num_cu_);
+                              BatchEngine::segment_bytes(r->sys->npol, 1, nsets, false, num_cu_, nsets > 1);
             if (used_ + fp > budget_ && !active_.empty()) {
                 deferred_++;
                 return;  // wait for memory to be released
@@ -188,7 +200,7 @@ class Server {
 
     bool admit(Request *r, size_t fp_estimate) {
         System &s = *r->sys;
-        if (!engine_) engine_ = new BatchEngine(stream_, s.amoeba, false, max_active_, 2 * max_active_);
+        if (!engine_) engine_ = new BatchEngine(stream_, s.amoeba, false, max_active_, 3 * max_active_);
         if (s.amoeba != engine_->amoeba) fatal("All systems of a server must use the same force field");
         if (s.npol == 0) {
             // nothing to solve: every step has zero energy
@@ -197,7 +209,7 @@ class Server {
             return true;
         }
         DeviceView v = device_view(s);  // upload, static field (system stream, synchronized)
-        const int seg = engine_->add(v, 1);
+        const int seg = engine_->add(v, 1, s.n_ipd > 1 ? 1 : 0);
         if (seg < 0) return false;
         SegInfo &S = engine_->segs[seg];
         r->seg = seg;
@@ -222,17 +234,47 @@ class Server {
         return true;
     }
 
-    // First step: x = 0.  Later steps: x is the last solve's dipoles and the
-    // geometry is unchanged, so r = b - T x follows from the last FP64
+    // First step: x = 0, every set solved.  Later steps (x = the last
+    // solve's dipoles, same geometry): with two sets, the shared correction
+    // column first; with one set, r = b - T x follows from the last FP64
     // residual and the change of b without another FP64 matvec.
     void start_step(Request *r, bool first) {
         SegInfo &S = engine_->segs[r->seg];
         System &s = *r->sys;
-        launch_charges_field(stream_, s.npol, S.view.ps64, r->nq,
-                             S.charges + (long)r->step * r->nq, S.ext);
-        if (first) engine_->set_rhs(r->seg);
-        else engine_->shift_rhs(r->seg);
-        engine_->start(r->seg, s.tol, s.max_iter, first ? kResXZero : kResKnown);
+        const long n = 3L * s.npol;
+        const double4 *q = S.charges + (long)r->step * r->nq;
+        r->inner_shared = 0;
+        if (first) {
+            launch_charges_field(stream_, s.npol, S.view.ps64, r->nq, q, S.ext);
+            engine_->set_rhs(r->seg);
+            engine_->start(r->seg, s.tol, s.max_iter, kResXZero);
+            r->shared = false;
+        } else if (S.nsets > 1) {
+            double *bs = S.dev.b + (long)S.nsets * n;
+            launch_charges_field(stream_, s.npol, S.view.ps64, r->nq, q, bs);
+            launch_batch_shared_rhs(stream_, n, S.nsets, S.view.e_m2d, S.ext, S.dev.b, bs);
+            SCHECK(hipMemsetAsync(S.dev.x + (long)S.nsets * n, 0, sizeof(double) * n, stream_));
+            engine_->start_cols(r->seg, S.nsets, 1, kSharedTolFrac * s.tol, s.max_iter, kResXZero, 1);
+            r->shared = true;
+        } else {
+            launch_charges_field(stream_, s.npol, S.view.ps64, r->nq, q, S.ext);
+            engine_->shift_rhs(r->seg);
+            engine_->start(r->seg, s.tol, s.max_iter, kResKnown);
+            r->shared = false;
+        }
+        r->valid_from = engine_->next_readback();
+    }
+
+    // The shared correction is done: add it to every set and check each set
+    // against its FP64 residual.
+    void feed_shared(Request *r, const BCol *snap) {
+        SegInfo &S = engine_->segs[r->seg];
+        System &s = *r->sys;
+        const long n = 3L * s.npol;
+        r->inner_shared = snap[S.col0 + S.nsets].s[0].inner_total;
+        launch_batch_feed(stream_, n, S.nsets, S.dev.x, S.dev.x + (long)S.nsets * n);
+        engine_->start_cols(r->seg, 0, S.nsets, s.tol, s.max_iter, kResMatvec, 0);
+        r->shared = false;
         r->valid_from = engine_->next_readback();
     }
 
@@ -240,12 +282,13 @@ class Server {
     void complete_step(Request *r, const BCol *snap, double t) {
         SegInfo &S = engine_->segs[r->seg];
         int outer = 0, inner = 0;
-        for (int c = 0; c < S.ncols; c++) {
+        for (int c = 0; c < S.nsets; c++) {
             const BColSub &cs = snap[S.col0 + c].s[0];
             if (cs.status != 0) r->status = 2;
             outer = std::max(outer, cs.outer + 1);
             inner = std::max(inner, cs.inner_total);
         }
+        inner += r->inner_shared;
         const int k = r->step;
         r->outer[k] = outer;
         r->inner[k] = inner;
@@ -262,7 +305,7 @@ class Server {
         }
         // last solve: dipoles back into the system, slot and memory released
         const long n = 3L * r->sys->npol;
-        SCHECK(hipMemcpyAsync(S.view.ipd, S.dev.x, sizeof(double) * S.ncols * n, hipMemcpyDeviceToDevice,
+        SCHECK(hipMemcpyAsync(S.view.ipd, S.dev.x, sizeof(double) * S.nsets * n, hipMemcpyDeviceToDevice,
                               stream_));
         SCHECK(hipEventRecord(r->ev, stream_));
         engine_->remove(r->seg);
@@ -330,10 +373,16 @@ class Server {
                         continue;
                     }
                     const SegInfo &S = engine_->segs[r->seg];
+                    const int c0 = r->shared ? S.nsets : 0, c1 = r->shared ? S.nsets + 1 : S.nsets;
                     bool all = true;
-                    for (int c = 0; c < S.ncols && all; c++) all = snap[S.col0 + c].s[0].phase == kPhaseDone;
+                    for (int c = c0; c < c1 && all; c++) all = snap[S.col0 + c].s[0].phase == kPhaseDone;
                     if (!all) {
-                        live += S.ncols;
+                        live += c1 - c0;
+                        still.push_back(r);
+                        continue;
+                    }
+                    if (r->shared) {
+                        feed_shared(r, snap);
                         still.push_back(r);
                         continue;
                     }
