@@ -58,6 +58,18 @@ void ommp_set_external_field(OMMP_SYSTEM_PRT s, const double *ext, int32_t solve
 /* :41; mod_interface.F90:224-232: field = ext only, solves the D set only. */
 void ommp_set_external_field_nomm(OMMP_SYSTEM_PRT s, const double *ext, int32_t solver, int32_t matv);
 
+/* :43-48; mod_interface.F90:250-341.  Potential v[n] / field e[n][3] at n
+   external points cext[n][3] (bohr) of the static multipoles (mm), of the
+   induced dipoles (pol: D set for the potential, (D + P) / 2 for the field
+   with AMOEBA) or of both, undamped and unscreened; ADDED to v / e.  The
+   dipole terms need a completed solve (fatal otherwise). */
+void ommp_potential_mmpol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *v);
+void ommp_potential_mm2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *v);
+void ommp_potential_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *v);
+void ommp_field_mmpol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e);
+void ommp_field_mm2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e);
+void ommp_field_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e);
+
 /* :65; mod_c_interface.F90 C_ommp_ff_is_amoeba */
 bool ommp_ff_is_amoeba(OMMP_SYSTEM_PRT s);
 /* :68-71 */

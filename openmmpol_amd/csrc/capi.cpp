@@ -209,6 +209,27 @@ void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out) {
     for (int k = 0; k < 3; k++) out[k] = sys(s)->last_ms[k];
 }
 
+// mod_interface.F90:250-341 (potential_M2E / D2E, field_M2E / D2E), results
+// added to the caller's arrays.
+void ommp_potential_mmpol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *v) {
+    ommp::x2e_host(*sys(s), n, cext, true, true, v, nullptr);
+}
+void ommp_potential_mm2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *v) {
+    ommp::x2e_host(*sys(s), n, cext, true, false, v, nullptr);
+}
+void ommp_potential_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *v) {
+    ommp::x2e_host(*sys(s), n, cext, false, true, v, nullptr);
+}
+void ommp_field_mmpol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e) {
+    ommp::x2e_host(*sys(s), n, cext, true, true, nullptr, e);
+}
+void ommp_field_mm2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e) {
+    ommp::x2e_host(*sys(s), n, cext, true, false, nullptr, e);
+}
+void ommp_field_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e) {
+    ommp::x2e_host(*sys(s), n, cext, false, true, nullptr, e);
+}
+
 void ommp_amd_set_external_fields_multi(OMMP_SYSTEM_PRT s, int32_t nrhs, const double *ext, double *ipd,
                                         double *epol) {
     ommp::solve_multi(*sys(s), nrhs, ext, false, ipd, false, epol);

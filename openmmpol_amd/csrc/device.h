@@ -82,6 +82,14 @@ void launch_tpack_mv(hipStream_t st, int prec, int npol, int M, const void *tpac
                      hipEvent_t eve = nullptr);
 size_t tpack_part_bytes(int npol, int M);
 
+// Potential and field at n external points cext[n][3] of the multipoles (mm)
+// and/or of the induced dipoles (pol), FP64: out[n] = (V, Ex, Ey, Ez); part
+// holds (smm + spol) * n double4; *bad is set if a point sits on a source.
+int x2e_splits(int n, int nsrc);
+void launch_x2e(hipStream_t st, int n, const double *cext, bool mm, bool pol, bool amoeba, int ld, int nmm,
+                const double4 *ms, const double *mq, int npol, const double4 *ps, const double *ipd, double4 *part,
+                int smm, int spol, double4 *out, int *bad);
+
 // CG pieces (columns m < M, vectors [m * n + k]).
 void launch_norm2_guess(hipStream_t st, long n, int M, double *x, const double *alpha, double *red);
 void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double *h, double *x,

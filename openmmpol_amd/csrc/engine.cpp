@@ -75,6 +75,11 @@ struct DeviceState {
     std::vector<size_t> alloc_bytes;
     // batched multi-RHS solver state (cached between calls)
     BatchEngine *batch = nullptr;
+    // external-point potentials and fields (grown on demand)
+    double *x2e_c = nullptr;
+    double4 *x2e_part = nullptr, *x2e_out = nullptr;
+    int *x2e_bad = nullptr;
+    long x2e_cap = 0, x2e_cap_part = 0;
     // live kernel timing: event pairs in flight and accumulated totals per class
     bool prof = false;
     std::vector<hipEvent_t> prof_pool;
@@ -904,6 +909,51 @@ void charges_field_host(System &s, int nq, const double *charges, double *out) {
     HCHECK(hipMemcpyAsync(out, dout, sizeof(double) * d.n, hipMemcpyDeviceToHost, d.stream));
     HCHECK(hipStreamSynchronize(d.stream));
     if (dq) HCHECK(hipFree(dq));
+}
+
+// potential_M2E / potential_D2E / field_M2E / field_D2E at n external points
+// (mod_electrostatics.F90:3067-3336), added to V[n] and / or E[n][3] on the host
+// as the reference adds to the caller's arrays.
+void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V, double *E) {
+    if (n <= 0) return;
+    if (n > (1 << 28)) fatal("Too many external points");
+    pol = pol && s.npol > 0;
+    if (pol && !s.ipd_done)
+        fatal(V ? "IPD should be computed before D2E potential." : "IPD should be computed before computing D2E field.");
+    if (!mm && !pol) return;
+    DeviceState &d = ensure_device(s);
+    const int smm = mm ? x2e_splits(n, s.nmm) : 0, spol = pol ? x2e_splits(n, s.npol) : 0;
+    const long nparts = (long)(smm + spol) * n;
+    if (n > d.x2e_cap) {
+        dfree(d, d.x2e_c);
+        dfree(d, d.x2e_out);
+        d.x2e_c = dalloc<double>(d, 3L * n);
+        d.x2e_out = dalloc<double4>(d, n);
+        d.x2e_cap = n;
+    }
+    if (nparts > d.x2e_cap_part) {
+        dfree(d, d.x2e_part);
+        d.x2e_part = dalloc<double4>(d, nparts);
+        d.x2e_cap_part = nparts;
+    }
+    if (!d.x2e_bad) d.x2e_bad = dalloc<int>(d, 1);
+    std::vector<double> out(4L * n);
+    int bad = 0;
+    HCHECK(hipMemsetAsync(d.x2e_bad, 0, sizeof(int), d.stream));
+    HCHECK(hipMemcpyAsync(d.x2e_c, cext, sizeof(double) * 3L * n, hipMemcpyHostToDevice, d.stream));
+    launch_x2e(d.stream, n, d.x2e_c, mm, pol, s.amoeba, s.ld_cart, s.nmm, d.ms64, d.mq, s.npol, d.ps64, d.ipd,
+               d.x2e_part, smm, spol, d.x2e_out, d.x2e_bad);
+    HCHECK(hipMemcpyAsync(out.data(), d.x2e_out, sizeof(double) * 4L * n, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipMemcpyAsync(&bad, d.x2e_bad, sizeof(int), hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    if (bad)
+        fatal("Requesting Coulomb kernel for two atoms placed in the same point, this could be an internal bug "
+              "or a problem in your input file, please check.");
+    for (long j = 0; j < n; j++) {
+        if (V) V[j] += out[4 * j];
+        if (E)
+            for (int c = 0; c < 3; c++) E[3 * j + c] += out[4 * j + 1 + c];
+    }
 }
 
 const double *device_ipd(System &s) {

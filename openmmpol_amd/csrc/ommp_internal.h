@@ -120,6 +120,7 @@ void profile_stats(System &s, double *out, bool reset);
 void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *ipd_out, bool out_dev,
                  double *epol);
 void charges_field_host(System &s, int nq, const double *charges, double *out);
+void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V, double *E);
 
 // server.cpp
 void *server_create(double budget, int max_active);

@@ -1,0 +1,201 @@
+// Potential and field of the static multipoles and of the induced dipoles at
+// arbitrary external points (potential_M2E / potential_D2E / field_M2E /
+// field_D2E, mod_electrostatics.F90:3067-3336): undamped, unscreened, FP64.
+// One thread per external point, sources staged through LDS in tiles of 256,
+// the source range split over gridDim.y; the per-split partial results are
+// added in a fixed order by the finalize kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+
+#include "device.h"
+
+namespace ommp {
+
+#define HIP_CHECK(x)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                      \
+            std::abort();                                                                \
+        }                                                                                \
+    } while (0)
+
+constexpr double kEps2 = kEpsRpDev * kEpsRpDev;
+
+// Multipole sources: ms[i] = (x, y, z, .), mq[i][LD] = q, mu, quadrupole / 3
+// (xx, xy, yy, xz, yz, zz).  V = q k1 + (mu.d) k2 + 3 (d.Q d) k3;
+// E = q k2 d + 3 (mu.d) k3 d - k2 mu + 15 (d.Q d) k4 d - 6 k3 Q d, d = r_ext - r_i.
+template <int LD>
+__global__ __launch_bounds__(kBlock) void k_m2e(int n, const double *__restrict__ cext, int nsrc, int chunk,
+                                                const double4 *__restrict__ ms, const double *__restrict__ mq,
+                                                double4 *__restrict__ part, int *__restrict__ bad) {
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_q[LD][kBlock];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * kBlock + tid;
+    const bool valid = j < n;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (valid) {
+        px = cext[3L * j];
+        py = cext[3L * j + 1];
+        pz = cext[3L * j + 2];
+    }
+    double v = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+    int hit = 0;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(nsrc, i0 + chunk);
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            s_pos[tid] = ms[base + tid];
+#pragma unroll
+            for (int c = 0; c < LD; c++) s_q[c][tid] = mq[(long)LD * (base + tid) + c];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int k = 0; k < cnt; k++) {
+            const double dx = px - s_pos[k].x, dy = py - s_pos[k].y, dz = pz - s_pos[k].z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < kEps2) {
+                hit = 1;
+                continue;
+            }
+            const double k1 = rsqrt(r2), ir2 = k1 * k1, k2 = k1 * ir2;
+            const double q = s_q[0][k];
+            if (LD == 1) {
+                v += q * k1;
+                ex += q * k2 * dx;
+                ey += q * k2 * dy;
+                ez += q * k2 * dz;
+            } else {
+                const double k3 = k2 * ir2, k4 = k3 * ir2;
+                const double mx = s_q[1][k], my = s_q[2][k], mz = s_q[3][k];
+                const double md = mx * dx + my * dy + mz * dz;
+                const double qx = s_q[4][k] * dx + s_q[5][k] * dy + s_q[7][k] * dz;
+                const double qy = s_q[5][k] * dx + s_q[6][k] * dy + s_q[8][k] * dz;
+                const double qz = s_q[7][k] * dx + s_q[8][k] * dy + s_q[9][k] * dz;
+                const double qrr = qx * dx + qy * dy + qz * dz;
+                v += q * k1 + md * k2 + 3.0 * qrr * k3;
+                const double cd = q * k2 + 3.0 * md * k3 + 15.0 * qrr * k4;
+                ex += cd * dx - mx * k2 - 6.0 * qx * k3;
+                ey += cd * dy - my * k2 - 6.0 * qy * k3;
+                ez += cd * dz - mz * k2 - 6.0 * qz * k3;
+            }
+        }
+    }
+    if (valid) part[(long)blockIdx.y * n + j] = make_double4(v, ex, ey, ez);
+    if (hit) *bad = 1;
+}
+
+// Induced-dipole sources: ps[i] = position, potential from muV (the D set for
+// AMOEBA), field from muE (1/2 (mu_D + mu_P) for AMOEBA, mod_electrostatics.F90:3250).
+__global__ __launch_bounds__(kBlock) void k_d2e(int n, const double *__restrict__ cext, int nsrc, int chunk,
+                                                const double4 *__restrict__ ps, const double *__restrict__ ipd,
+                                                int amoeba, double4 *__restrict__ part, int *__restrict__ bad) {
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_mv[3][kBlock], s_me[3][kBlock];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * kBlock + tid;
+    const bool valid = j < n;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (valid) {
+        px = cext[3L * j];
+        py = cext[3L * j + 1];
+        pz = cext[3L * j + 2];
+    }
+    double v = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+    int hit = 0;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(nsrc, i0 + chunk);
+    const long np3 = 3L * nsrc;
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            const long i = base + tid;
+            s_pos[tid] = ps[i];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const double d = ipd[3 * i + c];
+                s_mv[c][tid] = d;
+                s_me[c][tid] = amoeba ? 0.5 * (ipd[np3 + 3 * i + c] + d) : d;
+            }
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int k = 0; k < cnt; k++) {
+            const double dx = px - s_pos[k].x, dy = py - s_pos[k].y, dz = pz - s_pos[k].z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < kEps2) {
+                hit = 1;
+                continue;
+            }
+            const double k1 = rsqrt(r2), ir2 = k1 * k1, k2 = k1 * ir2, k3 = k2 * ir2;
+            v += (s_mv[0][k] * dx + s_mv[1][k] * dy + s_mv[2][k] * dz) * k2;
+            const double mx = s_me[0][k], my = s_me[1][k], mz = s_me[2][k];
+            const double c3 = 3.0 * (mx * dx + my * dy + mz * dz) * k3;
+            ex += c3 * dx - mx * k2;
+            ey += c3 * dy - my * k2;
+            ez += c3 * dz - mz * k2;
+        }
+    }
+    if (valid) part[(long)blockIdx.y * n + j] = make_double4(v, ex, ey, ez);
+    if (hit) *bad = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_x2e_finalize(int n, int nparts, const double4 *__restrict__ part,
+                                                         double4 *__restrict__ out) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    double4 a = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int s = 0; s < nparts; s++) {
+        const double4 p = part[(long)s * n + j];
+        a.x += p.x;
+        a.y += p.y;
+        a.z += p.z;
+        a.w += p.w;
+    }
+    out[j] = a;
+}
+
+int x2e_splits(int n, int nsrc) {
+    const int pb = (n + kBlock - 1) / kBlock;
+    const int max_s = std::max(1, (nsrc + kBlock - 1) / kBlock);
+    const int want = (2048 + pb - 1) / pb;
+    return std::max(1, std::min(want, max_s));
+}
+
+void launch_x2e(hipStream_t st, int n, const double *cext, bool mm, bool pol, bool amoeba, int ld, int nmm,
+                const double4 *ms, const double *mq, int npol, const double4 *ps, const double *ipd, double4 *part,
+                int smm, int spol, double4 *out, int *bad) {
+    if (n == 0) return;
+    const int pb = (n + kBlock - 1) / kBlock;
+    int off = 0;
+    if (mm && nmm > 0) {
+        const int chunk = (nmm + smm - 1) / smm;
+        if (ld == 10)
+            hipLaunchKernelGGL(k_m2e<10>, dim3(pb, smm), dim3(kBlock), 0, st, n, cext, nmm, chunk, ms, mq, part,
+                               bad);
+        else
+            hipLaunchKernelGGL(k_m2e<1>, dim3(pb, smm), dim3(kBlock), 0, st, n, cext, nmm, chunk, ms, mq, part,
+                               bad);
+        HIP_CHECK(hipGetLastError());
+        off = smm;
+    }
+    if (pol && npol > 0) {
+        const int chunk = (npol + spol - 1) / spol;
+        hipLaunchKernelGGL(k_d2e, dim3(pb, spol), dim3(kBlock), 0, st, n, cext, npol, chunk, ps, ipd,
+                           amoeba ? 1 : 0, part + (long)off * n, bad);
+        HIP_CHECK(hipGetLastError());
+        off += spol;
+    }
+    hipLaunchKernelGGL(k_x2e_finalize, dim3(pb), dim3(kBlock), 0, st, n, off, part, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ommp

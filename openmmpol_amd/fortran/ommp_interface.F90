@@ -102,8 +102,52 @@ module ommp_interface
             type(c_ptr), value :: s
             real(c_double), intent(in) :: c(*)
         end subroutine
+        subroutine c_ommp_potential_mmpol2ext(s, n, cext, v) bind(c, name='ommp_potential_mmpol2ext')
+            import :: c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: cext(*)
+            real(c_double), intent(inout) :: v(*)
+        end subroutine
+        subroutine c_ommp_potential_mm2ext(s, n, cext, v) bind(c, name='ommp_potential_mm2ext')
+            import :: c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: cext(*)
+            real(c_double), intent(inout) :: v(*)
+        end subroutine
+        subroutine c_ommp_potential_pol2ext(s, n, cext, v) bind(c, name='ommp_potential_pol2ext')
+            import :: c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: cext(*)
+            real(c_double), intent(inout) :: v(*)
+        end subroutine
+        subroutine c_ommp_field_mmpol2ext(s, n, cext, e) bind(c, name='ommp_field_mmpol2ext')
+            import :: c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: cext(*)
+            real(c_double), intent(inout) :: e(*)
+        end subroutine
+        subroutine c_ommp_field_mm2ext(s, n, cext, e) bind(c, name='ommp_field_mm2ext')
+            import :: c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: cext(*)
+            real(c_double), intent(inout) :: e(*)
+        end subroutine
+        subroutine c_ommp_field_pol2ext(s, n, cext, e) bind(c, name='ommp_field_pol2ext')
+            import :: c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            real(c_double), intent(in) :: cext(*)
+            real(c_double), intent(inout) :: e(*)
+        end subroutine
     end interface
 
+    public :: ommp_potential_mmpol2ext, ommp_potential_mm2ext, ommp_potential_pol2ext
+    public :: ommp_field_mmpol2ext, ommp_field_mm2ext, ommp_field_pol2ext
     public :: ommp_init_mmp, ommp_terminate, ommp_set_verbose
     public :: ommp_set_default_solver, ommp_set_default_matv
     public :: ommp_set_external_field, ommp_set_external_field_nomm
@@ -213,6 +257,60 @@ contains
         type(ommp_system), intent(in) :: s
         real(ommp_real), intent(in) :: new_c(:,:)
         call c_ommp_update_coordinates(s%h, new_c)
+    end subroutine
+
+    ! mod_interface.F90:250-295: results are added to v
+    subroutine ommp_potential_mmpol2ext(s, n, cext, v)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in) :: n
+        real(ommp_real), intent(in) :: cext(3, n)
+        real(ommp_real), intent(inout) :: v(n)
+        call c_ommp_potential_mmpol2ext(s%h, n, cext, v)
+    end subroutine
+
+    ! mod_interface.F90:250-295: results are added to v
+    subroutine ommp_potential_mm2ext(s, n, cext, v)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in) :: n
+        real(ommp_real), intent(in) :: cext(3, n)
+        real(ommp_real), intent(inout) :: v(n)
+        call c_ommp_potential_mm2ext(s%h, n, cext, v)
+    end subroutine
+
+    ! mod_interface.F90:250-295: results are added to v
+    subroutine ommp_potential_pol2ext(s, n, cext, v)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in) :: n
+        real(ommp_real), intent(in) :: cext(3, n)
+        real(ommp_real), intent(inout) :: v(n)
+        call c_ommp_potential_pol2ext(s%h, n, cext, v)
+    end subroutine
+
+    ! mod_interface.F90:297-341: results are added to e
+    subroutine ommp_field_mmpol2ext(s, n, cext, e)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in), value :: n
+        real(ommp_real), intent(in) :: cext(3, n)
+        real(ommp_real), intent(inout) :: e(3, n)
+        call c_ommp_field_mmpol2ext(s%h, n, cext, e)
+    end subroutine
+
+    ! mod_interface.F90:297-341: results are added to e
+    subroutine ommp_field_mm2ext(s, n, cext, e)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in), value :: n
+        real(ommp_real), intent(in) :: cext(3, n)
+        real(ommp_real), intent(inout) :: e(3, n)
+        call c_ommp_field_mm2ext(s%h, n, cext, e)
+    end subroutine
+
+    ! mod_interface.F90:297-341: results are added to e
+    subroutine ommp_field_pol2ext(s, n, cext, e)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in), value :: n
+        real(ommp_real), intent(in) :: cext(3, n)
+        real(ommp_real), intent(inout) :: e(3, n)
+        call c_ommp_field_pol2ext(s%h, n, cext, e)
     end subroutine
 
 end module ommp_interface

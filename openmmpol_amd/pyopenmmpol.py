@@ -159,6 +159,34 @@ class OMMPSystem:
     def get_polelec_energy(self):
         return lib().ommp_get_polelec_energy(self.handler)
 
+    # potentials and fields at external points (pyommp_interface.cpp:362-480):
+    # cext [n, 3] in bohr; new zeroed arrays are returned
+    def _x2e(self, fn, cext, field):
+        c = np.ascontiguousarray(cext, dtype=np.float64)
+        if c.ndim != 2 or c.shape[1] != 3:
+            raise ValueError("cext should be shaped [:, 3]")
+        out = np.zeros((c.shape[0], 3) if field else c.shape[0])
+        fn(self.handler, c.shape[0], c.ctypes.data_as(c_dp), out.ctypes.data_as(c_dp))
+        return out
+
+    def potential_mm2ext(self, cext):
+        return self._x2e(lib().ommp_potential_mm2ext, cext, False)
+
+    def potential_pol2ext(self, cext):
+        return self._x2e(lib().ommp_potential_pol2ext, cext, False)
+
+    def potential_mmpol2ext(self, cext):
+        return self._x2e(lib().ommp_potential_mmpol2ext, cext, False)
+
+    def field_mm2ext(self, cext):
+        return self._x2e(lib().ommp_field_mm2ext, cext, True)
+
+    def field_pol2ext(self, cext):
+        return self._x2e(lib().ommp_field_pol2ext, cext, True)
+
+    def field_mmpol2ext(self, cext):
+        return self._x2e(lib().ommp_field_mmpol2ext, cext, True)
+
     def update_coordinates(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)
         if c.ndim != 2 or c.shape[0] != self.mm_atoms or c.shape[1] != 3:

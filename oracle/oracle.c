@@ -927,6 +927,55 @@ void oracle_charges_field(void *h, int nq, const double *cq, const double *qq, d
     }
 }
 
+/* potential_M2E / potential_D2E / field_M2E / field_D2E
+   (mod_electrostatics.F90:3067-3336): potential V[n] and field E[n][3] at n
+   external points cext[n][3], undamped and unscreened, ADDED to V and E
+   (either may be NULL).  which: 1 = static multipoles (M2E), 2 = induced
+   dipoles (D2E), 3 = both.  D2E uses the D set for the potential (:3094-3110)
+   and 1/2 (mu_P + mu_D) for the field (:3250) with AMOEBA; ipd is
+   [n_ipd][npol][3].  A point on top of a source is fatal (coulomb_kernel,
+   :1003-1008). */
+void oracle_x2e(void *h, const double *ipd, int n, const double *cext, int which, double *V, double *E)
+{
+    osys *s = (osys *)h;
+    int bad = 0;
+    #pragma omp parallel for schedule(dynamic, 16) reduction(|:bad)
+    for (int j = 0; j < n; j++) {
+        const double *p = cext + 3 * j;
+        double v = 0.0, e[3] = {0.0, 0.0, 0.0};
+        if (which & 1) {
+            for (int i = 0; i < s->nmm; i++) {
+                const double *c = s->cmm + 3 * i;
+                double dr[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+                double k[4];
+                if (coulomb_kernel(dr, s->amoeba ? 3 : 1, k)) { bad = 1; continue; }
+                double et[3];
+                v += oracle_multipole_potential_field(s->q + (long)s->ld_cart * i, s->ld_cart, dr, k, et);
+                for (int c3 = 0; c3 < 3; c3++) e[c3] += et[c3];
+            }
+        }
+        if ((which & 2) && s->npol > 0) {
+            for (int i = 0; i < s->npol; i++) {
+                const double *c = s->cmm + 3 * s->polar_mm[i];
+                double dr[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+                double k[3];
+                if (coulomb_kernel(dr, 2, k)) { bad = 1; continue; }
+                const double *mD = ipd + 3L * i;
+                double mu[3];
+                for (int c3 = 0; c3 < 3; c3++)
+                    mu[c3] = s->amoeba ? 0.5 * (mD[c3] + ipd[3L * s->npol + 3L * i + c3]) : mD[c3];
+                v += (mD[0] * dr[0] + mD[1] * dr[1] + mD[2] * dr[2]) * k[1];
+                mu_field(mu, dr, k, e);
+            }
+        }
+        if (V) V[j] += v;
+        if (E)
+            for (int c3 = 0; c3 < 3; c3++) E[3 * j + c3] += e[c3];
+    }
+    if (bad)
+        oracle_fatal(s, "Requesting Coulomb kernel for two atoms placed in the same point");
+}
+
 int oracle_num_threads(void)
 {
 #ifdef _OPENMP

@@ -1,6 +1,8 @@
 ! Drives the library through the Fortran module: reads a .mmp file and an
 ! external-field file (one row x y z per MM atom, or 'none'), solves with the
-! default solver, prints the polarization energy and the dipoles.
+! default solver, prints the polarization energy and the dipoles, then the
+! potential and field of multipoles plus dipoles at two fixed external points
+! (each evaluated twice into the same array, which must add up).
 program test_fortran_api
     use ommp_interface
     use iso_c_binding
@@ -11,7 +13,7 @@ program test_fortran_api
     real(ommp_real), pointer :: ipd(:,:,:)
     integer(ommp_integer), pointer :: pmm(:)
     integer :: i, k, npol, nmm, u
-    real(ommp_real) :: ep
+    real(ommp_real) :: ep, cext(3, 2), v(2), e(3, 2)
 
     call get_command_argument(1, mmp)
     call get_command_argument(2, eff)
@@ -41,6 +43,17 @@ program test_fortran_api
         do i = 1, npol
             write(*, '(A, 3ES24.15)') 'IPD ', ipd(:, i, k)
         end do
+    end do
+    cext(:, 1) = [25.0_ommp_real, -31.0_ommp_real, 17.0_ommp_real]
+    cext(:, 2) = [-40.0_ommp_real, 12.5_ommp_real, -22.0_ommp_real]
+    v = 0.0_ommp_real
+    e = 0.0_ommp_real
+    call ommp_potential_mmpol2ext(s, 2, cext, v)
+    call ommp_potential_mm2ext(s, 2, cext, v)
+    call ommp_field_mmpol2ext(s, 2, cext, e)
+    call ommp_field_pol2ext(s, 2, cext, e)
+    do i = 1, 2
+        write(*, '(A, 4ES24.15)') 'X2E ', v(i), e(:, i)
     end do
     call ommp_terminate(s)
 end program

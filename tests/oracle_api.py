@@ -60,6 +60,7 @@ def lib():
         L.oracle_charges_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_set_small_step_eps.argtypes = [ctypes.c_double]
+        L.oracle_x2e.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -211,6 +212,18 @@ class OracleSystem:
         ipd = np.ascontiguousarray(ipd, dtype=np.float64)
         em = np.ascontiguousarray(self.prepare_polelec())
         return lib().oracle_energy_pol(self.h, _ptr(ipd, _dp), _ptr(em, _dp))
+
+    def x2e(self, cext, which, ipd=None):
+        """Potential [n] and field [n, 3] at external points (potential_M2E /
+        D2E, field_M2E / D2E): which 1 = multipoles, 2 = induced dipoles, 3 = both."""
+        cext = np.ascontiguousarray(cext, dtype=np.float64).reshape(-1, 3)
+        n = cext.shape[0]
+        V = np.zeros(n)
+        E = np.zeros((n, 3))
+        ip = np.ascontiguousarray(self.ipd if ipd is None else ipd, dtype=np.float64)
+        lib().oracle_x2e(self.h, _ptr(ip, _dp), n, _ptr(cext, _dp), int(which), _ptr(V, _dp), _ptr(E, _dp))
+        self.check()
+        return V, E
 
     def charges_field(self, cq, qq):
         cq = np.ascontiguousarray(cq, dtype=np.float64)

@@ -97,6 +97,15 @@ def test_fatal_errors_exit_1_like_the_reference():
              "s = o.OMMPSystem.from_arrays(True, np.zeros((1,3)), np.zeros((1,10)), [1.0], [0,0], [], pgroup=[0])\n"
              "o.lib().ommp_set_default_matv(s.handler, 9)")
     assert r.returncode == 1 and "Unrecognized setting for default matrix-vector method" in r.stdout
+    # potential_D2E / field_D2E need a solve first (mod_electrostatics.F90:3090, 3238)
+    r = _run("import openmmpol_amd as o, numpy as np\n"
+             "s = o.OMMPSystem.from_arrays(True, np.zeros((1,3)), np.zeros((1,10)), [1.0], [0,0], [], pgroup=[0])\n"
+             "s.potential_pol2ext(np.ones((2, 3)))")
+    assert r.returncode == 1 and "IPD should be computed before D2E potential" in r.stdout
+    r = _run("import openmmpol_amd as o, numpy as np\n"
+             "s = o.OMMPSystem.from_arrays(True, np.zeros((1,3)), np.zeros((1,10)), [1.0], [0,0], [], pgroup=[0])\n"
+             "s.field_mmpol2ext(np.ones((2, 3)))")
+    assert r.returncode == 1 and "IPD should be computed before computing D2E field" in r.stdout
 
 
 def test_python_argument_checks(fixture_file):

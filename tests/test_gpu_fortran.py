@@ -40,3 +40,11 @@ def test_fortran_module_solve(name, ff, k, fixture_file, tmp_path):
     eo = o.get_polelec_energy()
     assert abs(ep - eo) <= 1e-10 * abs(eo)
     assert np.abs(ipd - o.ipd.reshape(-1, 3)).max() < 1e-9
+    # v = V_mmpol + V_mm, e = E_mmpol + E_pol at two fixed points
+    x2e = np.array([[float(v) for v in l.split()[1:]] for l in lines if l.startswith("X2E")])
+    cext = np.array([[25.0, -31.0, 17.0], [-40.0, 12.5, -22.0]])
+    vmp, emp = o.x2e(cext, 3)
+    vm, _ = o.x2e(cext, 1)
+    _, ep_ = o.x2e(cext, 2)
+    assert np.allclose(x2e[:, 0], vmp + vm, rtol=1e-12, atol=1e-15)
+    assert np.allclose(x2e[:, 1:], emp + ep_, rtol=1e-10, atol=1e-14)

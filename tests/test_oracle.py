@@ -317,3 +317,81 @@ def test_solvers_agree_with_dense_solve(ff):
         o.ipd[:] = 0
         ipd = o.polarization(E, solver=solver, matv=matv, use_guess=False)
         assert np.abs(ipd - exact).max() < tol, (solver, matv)
+
+
+# --------------------------------------------------- external-point properties
+def _x2e_points(o, n, rng, margin=1.5):
+    """Random points at least `margin` bohr from every MM site."""
+    c = np.asarray(o.prep["cmm"]).reshape(-1, 3)
+    lo, hi = c.min(0) - 4.0, c.max(0) + 4.0
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(lo, hi)
+        if np.min(np.linalg.norm(c - p, axis=1)) > margin:
+            pts.append(p)
+    return np.array(pts)
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_x2e_field_is_minus_gradient_of_potential(ff):
+    """field_M2E / field_D2E against central differences of potential_M2E /
+    potential_D2E (mod_electrostatics.F90:3067-3336).  With AMOEBA the D2E field
+    uses (mu_D + mu_P) / 2 and the potential mu_D, so the dipole check uses
+    mu_P = mu_D there."""
+    o = oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path("N-methylacetamide", f"input_{ff}.mmp")))
+    ext = mmp_io.read_ef(mmp_io.fixture_path("N-methylacetamide", "EF_1.txt"))[o.polar_mm]
+    o.set_external_field(ext)
+    ipd = o.ipd.copy()
+    if o.amoeba:
+        ipd[1] = ipd[0]
+    rng = np.random.default_rng(4)
+    pts = _x2e_points(o, 12, rng)
+    h = 1e-4
+    for which in (1, 2):
+        _, E = o.x2e(pts, which, ipd)
+        grad = np.zeros_like(E)
+        for c in range(3):
+            dp = np.zeros(3)
+            dp[c] = h
+            vp, _ = o.x2e(pts + dp, which, ipd)
+            vm, _ = o.x2e(pts - dp, which, ipd)
+            grad[:, c] = (vp - vm) / (2 * h)
+        assert np.allclose(E, -grad, rtol=1e-6, atol=1e-9 * np.abs(E).max()), which
+
+
+def test_x2e_dipole_potential_reciprocity():
+    """sum_c q_c V_D2E(r_c) = -sum_i mu_D,i . E_charges(r_i): the induced
+    dipoles' potential at a set of charges against the charges' field at the
+    dipoles (E_n2p), two independently written kernels."""
+    o = oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path("1crn", "input_AMOEBA.mmp")))
+    ext = mmp_io.read_ef(mmp_io.fixture_path("1crn", "EF_1.txt"))[o.polar_mm]
+    o.set_external_field(ext)
+    rng = np.random.default_rng(9)
+    pts = _x2e_points(o, 20, rng)
+    qq = rng.normal(0.0, 0.5, size=len(pts))
+    V, _ = o.x2e(pts, 2)
+    Ec = o.charges_field(pts, qq)
+    lhs = float(np.dot(qq, V))
+    rhs = -float(np.sum(o.ipd[0] * Ec))
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+
+
+def test_x2e_point_charge_closed_form():
+    """A system whose only multipole is one charge: V = q / r, E = q d / r^3."""
+    from openmmpol_amd import synth
+    spec = synth.water_box(nside=1)
+    spec["q"] = np.zeros_like(spec["q"])
+    spec["q"][1, 0] = 0.7
+    o = oa.OracleSystem(synth.oracle_prep(spec))
+    pts = np.array([[5.0, -3.0, 2.0], [-7.5, 1.0, 0.5]])
+    V, E = o.x2e(pts, 1)
+    d = pts - np.asarray(spec["cmm"])[1]
+    r = np.linalg.norm(d, axis=1)
+    assert np.allclose(V, 0.7 / r, rtol=1e-14)
+    assert np.allclose(E, 0.7 * d / r[:, None] ** 3, rtol=1e-14)
+
+
+def test_x2e_coincident_point_is_fatal():
+    o = oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path("N-methylacetamide", "input_AMOEBA.mmp")))
+    with pytest.raises(RuntimeError, match="same point"):
+        o.x2e(np.asarray(o.prep["cmm"]).reshape(-1, 3)[:1], 1)
