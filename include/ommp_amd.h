@@ -106,8 +106,9 @@ double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const do
    launch of the T matvec (on-the-fly, both precisions) and every packed-T
    stream launch is bracketed by HIP events.  stats receives, per class
    k = 0 on-the-fly FP64, 1 on-the-fly FP32, 2 packed FP32, 3 packed FP64,
-   4 static field: stats[2k] = launches, stats[2k+1] = total milliseconds.
-   stats must hold 10 entries; reset clears the counters. */
+   4 static field, 5 geometric-gradient pair sweeps: stats[2k] = launches,
+   stats[2k+1] = total milliseconds.  stats must hold 12 entries; reset clears
+   the counters. */
 void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable);
 void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset);
 

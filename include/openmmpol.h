@@ -70,6 +70,21 @@ void ommp_field_mmpol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, doub
 void ommp_field_mm2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e);
 void ommp_field_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e);
 
+/* :91; mod_c_interface.F90:833-846 -> mod_interface.F90:671-683 ->
+   polelec_geomgrad (mod_geomgrad.F90:113-273).  grd[mm_atoms][3] is set to the
+   gradient of the polarization energy with respect to the MM coordinates
+   (Hartree/bohr), AMOEBA multipole rotation included; 0 without polarizable
+   atoms.  Solves with the default solver (no external field) first if no
+   dipoles are available.  The external field's own position dependence is the
+   caller's (as in the reference). */
+void ommp_polelec_geomgrad(OMMP_SYSTEM_PRT s, double *grd);
+/* :92; mod_c_interface.F90:606-624 -> mod_interface.F90:699-710 ->
+   rotation_geomgrad (rotate_multipoles.F90:3-230).  grd[mm_atoms][3] is set to
+   the gradient, through the frame atoms, of the energy of the rotated
+   multipoles in a field E[mm_atoms][3] and field gradient Egrd[mm_atoms][6]
+   (xx, xy, yy, xz, yz, zz; -dE_b/dr_a).  0 for non-AMOEBA systems. */
+void ommp_rotation_geomgrad(OMMP_SYSTEM_PRT s, const double *E, const double *Egrd, double *grd);
+
 /* :65; mod_c_interface.F90 C_ommp_ff_is_amoeba */
 bool ommp_ff_is_amoeba(OMMP_SYSTEM_PRT s);
 /* :68-71 */

@@ -71,6 +71,8 @@ SIGNATURES = [
     ("ommp_field_mmpol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_field_mm2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_field_pol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
+    ("ommp_polelec_geomgrad", None, [c_vp, c_dp]),
+    ("ommp_rotation_geomgrad", None, [c_vp, c_dp, c_dp, c_dp]),
     ("ommp_amd_set_external_fields_multi", None, [c_vp, ctypes.c_int32, c_dp, c_dp, c_dp]),
     ("ommp_amd_set_external_fields_multi_device", None, [c_vp, ctypes.c_int32, c_vp, c_vp, c_dp]),
     ("ommp_amd_charges_field", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),

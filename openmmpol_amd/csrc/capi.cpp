@@ -230,6 +230,12 @@ void ommp_field_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double
     ommp::x2e_host(*sys(s), n, cext, false, true, nullptr, e);
 }
 
+void ommp_polelec_geomgrad(OMMP_SYSTEM_PRT s, double *grd) { ommp::polelec_geomgrad_host(*sys(s), grd); }
+
+void ommp_rotation_geomgrad(OMMP_SYSTEM_PRT s, const double *E, const double *Egrd, double *grd) {
+    ommp::rotation_geomgrad_host(*sys(s), E, Egrd, grd);
+}
+
 void ommp_amd_set_external_fields_multi(OMMP_SYSTEM_PRT s, int32_t nrhs, const double *ext, double *ipd,
                                         double *epol) {
     ommp::solve_multi(*sys(s), nrhs, ext, false, ipd, false, epol);

@@ -90,6 +90,30 @@ void launch_x2e(hipStream_t st, int n, const double *cext, bool mm, bool pol, bo
                 const double4 *ms, const double *mq, int npol, const double4 *ps, const double *ipd, double4 *part,
                 int smm, int spol, double4 *out, int *bad);
 
+// Geometric gradient of the polarization energy.  part_pol holds
+// (split_s + split_p) * 3 * npol doubles, part_mm split_m * 12 * nmm (AMOEBA) or
+// split_m * 3 * nmm; mp: correction list per MM target (polarizable sources,
+// f = 1 - s for the D and P sets, self pairs excluded); frames[nmm] = (convention,
+// z, x, y atom; 0-based, -1 = none); rev_ri/rev_ci list, per atom, the entries
+// 4 * site + role (0 self, 1 x, 2 y, 3 z) whose frame uses that atom.
+struct GradArgs {
+    int nmm = 0, npol = 0;
+    bool amoeba = false;
+    const double4 *ms = nullptr, *ps = nullptr;
+    const double *ma = nullptr, *mq = nullptr, *pa = nullptr, *ipd = nullptr, *q0 = nullptr;
+    const int *polar_mm = nullptr, *mm_polar = nullptr, *rev_ri = nullptr, *rev_ci = nullptr;
+    const int4 *frames = nullptr;
+    CsrDev sp, pp, mp;
+    double *part_pol = nullptr, *part_mm = nullptr;
+    int split_s = 1, split_p = 1, split_m = 1;
+    double *gpol = nullptr, *gtr = nullptr, *emm = nullptr, *egrd = nullptr, *rotc = nullptr, *out = nullptr;
+};
+// out[nmm][3] = d E_pol / d r (pair sweeps + rotation term); emm/egrd receive
+// E_D2M and Egrd_D2M.  evb/eve bracket the three pair sweeps.
+void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
+// out[nmm][3] = rotation term only, for the field a.emm and gradient a.egrd.
+void launch_rotation_grad(hipStream_t st, const GradArgs &a);
+
 // CG pieces (columns m < M, vectors [m * n + k]).
 void launch_norm2_guess(hipStream_t st, long n, int M, double *x, const double *alpha, double *red);
 void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double *h, double *x,

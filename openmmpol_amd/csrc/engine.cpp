@@ -85,8 +85,19 @@ struct DeviceState {
     std::vector<hipEvent_t> prof_pool;
     std::vector<int> prof_class;
     int prof_used = 0;
-    double prof_ms[5] = {0, 0, 0, 0, 0};
-    long prof_count[5] = {0, 0, 0, 0, 0};
+    double prof_ms[6] = {0, 0, 0, 0, 0, 0};
+    long prof_count[6] = {0, 0, 0, 0, 0, 0};
+    // geometric gradient (built on first use): MM-target correction list, frames,
+    // per-atom rotation index, partial sums and results
+    CsrDev mp;
+    int *mm_polar = nullptr;
+    double *q0 = nullptr;
+    int4 *frames = nullptr;
+    int *rev_ri = nullptr, *rev_ci = nullptr;
+    double *g_part_pol = nullptr, *g_part_mm = nullptr, *g_pol = nullptr, *g_tr = nullptr;
+    double *g_emm = nullptr, *g_egrd = nullptr, *g_rotc = nullptr, *g_out = nullptr;
+    int g_split_s = 1, g_split_p = 1, g_split_m = 1;
+    bool grad_static_ready = false;
 };
 
 namespace {
@@ -193,16 +204,13 @@ void free_csr(DeviceState &d, CsrDev &c) {
     c = CsrDev();
 }
 
-void upload_lists(System &s, DeviceState &d) {
-    batch_release(d.batch);  // its segment points at the old lists
-    free_csr(d, d.sp);
-    free_csr(d, d.pp);
-    free_csr(d, d.pd);
+// S->P corrections per polarizable target: MM source, f = 1 - s for the D and
+// P sets (AMBER: one set, stored in both slots); the self pair is never summed.
+std::vector<std::vector<std::pair<int, double2>>> sp_rows(const System &s) {
     const int npol = s.npol;
-    // S->P corrections per polarizable target: f = 1 - s for the D and P sets.
     std::vector<std::vector<std::pair<int, double2>>> rows(npol);
     auto add = [&](int i, int jp, double sd, double spv, bool setd, bool setp) {
-        if (s.polar_mm[jp] == i) return;  // the self pair is never summed
+        if (s.polar_mm[jp] == i) return;
         for (auto &e : rows[jp])
             if (e.first == i) {
                 if (setd) e.second.x = 1.0 - sd;
@@ -217,7 +225,6 @@ void upload_lists(System &s, DeviceState &d) {
     for (int i = 0; i < s.nmm; i++)
         for (int k = s.list_S_P_P.ri[i]; k < s.list_S_P_P.ri[i + 1]; k++) {
             const double sc = s.list_S_P_P.val[k];
-            // AMBER: one set, stored in both slots
             add(i, s.list_S_P_P.ci[k], sc, sc, !s.amoeba, true);
         }
     if (s.amoeba)
@@ -225,6 +232,17 @@ void upload_lists(System &s, DeviceState &d) {
             for (int k = s.list_S_P_D.ri[i]; k < s.list_S_P_D.ri[i + 1]; k++)
                 add(i, s.list_S_P_D.ci[k], s.list_S_P_D.val[k], 1.0, true, false);
     for (auto &r : rows) std::sort(r.begin(), r.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    return rows;
+}
+
+void upload_lists(System &s, DeviceState &d) {
+    batch_release(d.batch);  // its segment points at the old lists
+    free_csr(d, d.sp);
+    free_csr(d, d.pp);
+    free_csr(d, d.pd);
+    free_csr(d, d.mp);
+    const int npol = s.npol;
+    auto rows = sp_rows(s);
     upload_csr(d, npol, rows, d.sp, true);
     // P->P corrections of the on-the-fly matvec (list_P_P, rows = source; transpose)
     rows.assign(npol, {});
@@ -956,6 +974,141 @@ void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V
     }
 }
 
+namespace {
+
+// Index lists and buffers of the gradient, built on first use.  The MM-target
+// correction list is the transpose of the S->P one (dropped with it when the
+// lists change); frames, unrotated multipoles and the per-atom rotation index
+// do not depend on the coordinates.
+void ensure_grad(System &s, DeviceState &d) {
+    if (!d.mp.ri) {
+        const auto rows = sp_rows(s);
+        std::vector<std::vector<std::pair<int, double2>>> mrows(s.nmm);
+        for (int jp = 0; jp < s.npol; jp++)
+            for (const auto &e : rows[jp]) mrows[e.first].push_back({jp, e.second});
+        upload_csr(d, s.nmm, mrows, d.mp, true);
+    }
+    if (d.grad_static_ready) return;
+    const int nmm = s.nmm, npol = s.npol;
+    d.mm_polar = dalloc<int>(d, nmm);
+    upload(s.mm_polar.data(), d.mm_polar, sizeof(int) * nmm);
+    std::vector<int> rri(nmm + 1, 0), rci;
+    if (s.amoeba) {
+        d.q0 = dalloc<double>(d, 10L * nmm);
+        upload(s.q0.data(), d.q0, sizeof(double) * 10 * nmm);
+        std::vector<int4> fr(nmm);
+        std::vector<std::vector<int>> users(nmm);
+        for (int j = 0; j < nmm; j++) {
+            fr[j] = make_int4(s.mol_frame[j], s.iz[j], s.ix[j], s.iy[j]);
+            if (s.mol_frame[j] == 0) continue;
+            if (s.iz[j] >= nmm || s.ix[j] >= nmm || s.iy[j] >= nmm || s.mol_frame[j] < 0 || s.mol_frame[j] > 5)
+                fatal("Invalid multipole frame definition");
+            const int at[4] = {j, s.ix[j] >= 0 ? s.ix[j] : j, s.iy[j] >= 0 ? s.iy[j] : j,
+                               s.iz[j] >= 0 ? s.iz[j] : j};
+            for (int r = 0; r < 4; r++) users[at[r]].push_back(4 * j + r);
+        }
+        for (int a = 0; a < nmm; a++) {
+            rri[a + 1] = rri[a] + (int)users[a].size();
+            rci.insert(rci.end(), users[a].begin(), users[a].end());
+        }
+        d.frames = dalloc<int4>(d, nmm);
+        upload(fr.data(), d.frames, sizeof(int4) * nmm);
+        d.g_rotc = dalloc<double>(d, 12L * nmm);
+        d.g_egrd = dalloc<double>(d, 6L * nmm);
+    }
+    d.rev_ri = dalloc<int>(d, nmm + 1);
+    d.rev_ci = dalloc<int>(d, std::max<size_t>(rci.size(), 1));
+    upload(rri.data(), d.rev_ri, sizeof(int) * (nmm + 1));
+    upload(rci.data(), d.rev_ci, sizeof(int) * rci.size());
+    int cu = 0;
+    HCHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, d.device));
+    cu = std::max(cu, 1);
+    d.g_split_s = balanced_split(site_blocks(npol), nmm, cu);
+    d.g_split_p = balanced_split(site_blocks(npol), npol, cu);
+    d.g_split_m = balanced_split(site_blocks(nmm), npol, cu);
+    d.g_part_pol = dalloc<double>(d, (size_t)(d.g_split_s + d.g_split_p) * 3 * npol);
+    d.g_part_mm = dalloc<double>(d, (size_t)d.g_split_m * (s.amoeba ? 12 : 3) * nmm);
+    d.g_pol = dalloc<double>(d, 3L * npol);
+    d.g_tr = dalloc<double>(d, 3L * nmm);
+    d.g_emm = dalloc<double>(d, 3L * nmm);
+    d.g_out = dalloc<double>(d, 3L * nmm);
+    d.grad_static_ready = true;
+}
+
+GradArgs grad_args(const System &s, DeviceState &d) {
+    GradArgs a;
+    a.nmm = s.nmm;
+    a.npol = s.npol;
+    a.amoeba = s.amoeba;
+    a.ms = d.ms64;
+    a.ps = d.ps64;
+    a.ma = d.ma64;
+    a.mq = d.mq;
+    a.pa = d.pa64;
+    a.ipd = d.ipd;
+    a.q0 = d.q0;
+    a.polar_mm = d.polar_mm;
+    a.mm_polar = d.mm_polar;
+    a.rev_ri = d.rev_ri;
+    a.rev_ci = d.rev_ci;
+    a.frames = d.frames;
+    a.sp = d.sp;
+    a.pp = d.pp;
+    a.mp = d.mp;
+    a.part_pol = d.g_part_pol;
+    a.part_mm = d.g_part_mm;
+    a.split_s = d.g_split_s;
+    a.split_p = d.g_split_p;
+    a.split_m = d.g_split_m;
+    a.gpol = d.g_pol;
+    a.gtr = d.g_tr;
+    a.emm = d.g_emm;
+    a.egrd = d.g_egrd;
+    a.rotc = d.g_rotc;
+    a.out = d.g_out;
+    return a;
+}
+
+}  // namespace
+
+// ommp_polelec_geomgrad (mod_interface.F90:671-683) -> polelec_geomgrad
+// (mod_geomgrad.F90:113-273): grd[nmm][3] = 0, then d E_pol / d r when the
+// system has polarizable sites; dipoles are solved first (default solver, no
+// external field) if they are not current.
+void polelec_geomgrad_host(System &s, double *grd) {
+    std::fill(grd, grd + 3L * s.nmm, 0.0);
+    if (s.npol == 0) return;
+    if (!s.ipd_done) {
+        prepare_polelec(s);
+        polarization(s, s.dev->e_m2d, kSolverNone, kMatvNone, 3);
+    }
+    DeviceState &d = ensure_device(s);
+    ensure_grad(s, d);
+    const GradArgs a = grad_args(s, d);
+    const int pi = prof_begin(d, 5);
+    launch_geomgrad(d.stream, a, pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
+                    pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
+    HCHECK(hipMemcpyAsync(grd, d.g_out, sizeof(double) * 3L * s.nmm, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+}
+
+// ommp_rotation_geomgrad (mod_interface.F90:699-710) -> rotation_geomgrad
+// (rotate_multipoles.F90:3-230): grd = 0, then the gradient of the rotated
+// multipoles' energy in the field E[nmm][3] and field gradient Egrd[nmm][6]
+// through their frame atoms.  Without AMOEBA multipoles there is nothing to
+// rotate and grd stays 0.
+void rotation_geomgrad_host(System &s, const double *E, const double *Egrd, double *grd) {
+    std::fill(grd, grd + 3L * s.nmm, 0.0);
+    if (!s.amoeba || s.nmm == 0) return;
+    DeviceState &d = ensure_device(s);
+    ensure_grad(s, d);
+    HCHECK(hipMemcpyAsync(d.g_emm, E, sizeof(double) * 3L * s.nmm, hipMemcpyHostToDevice, d.stream));
+    HCHECK(hipMemcpyAsync(d.g_egrd, Egrd, sizeof(double) * 6L * s.nmm, hipMemcpyHostToDevice, d.stream));
+    launch_rotation_grad(d.stream, grad_args(s, d));
+    HCHECK(hipMemcpyAsync(grd, d.g_out, sizeof(double) * 3L * s.nmm, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+}
+
 const double *device_ipd(System &s) {
     DeviceState &d = ensure_device(s);
     return d.ipd;
@@ -989,7 +1142,7 @@ void profile_enable(System &s, bool on) {
 void profile_stats(System &s, double *out, bool reset) {
     DeviceState &d = ensure_device(s);
     prof_flush(d);
-    for (int k = 0; k < 5; k++) {
+    for (int k = 0; k < 6; k++) {
         out[2 * k] = (double)d.prof_count[k];
         out[2 * k + 1] = d.prof_ms[k];
         if (reset) {

@@ -1,0 +1,722 @@
+// Geometric gradient of the polarization energy (polelec_geomgrad,
+// mod_geomgrad.F90:113-273, and rotation_geomgrad, rotate_multipoles.F90:3-230),
+// FP64 throughout.
+//
+// Two pair sweeps, one thread per target, sources staged through LDS in tiles
+// of 256 and split over gridDim.y:
+//   polarizable target j   sum_b w_b Egrd_ab of the static multipoles, with
+//                          w = 1/2 (mu_D + mu_P) (AMOEBA) or mu (AMBER), and
+//                          1/2 [mu_D,j . Egrd(mu_P) + mu_P,j . Egrd(mu_D)] of
+//                          the other dipoles (mu_j . Egrd(mu) for AMBER).  The
+//                          contraction is done per pair, so a thread keeps three
+//                          accumulators instead of two 6-component gradients.
+//   MM target m            field E, field gradient Egrd (kept for the rotation
+//                          term) and Q_m : EHes of the dipoles
+//                          nu = 1/2 (mu_P + mu_D) (AMOEBA) or mu (AMBER).
+// Sign conventions are the reference's: Egrd_ab = -dE_b/dr_a, EHes_abc =
+// d2E_c/dr_a dr_b, with d = r_target - r_source.  Damped kernels k_n =
+// lambda_{2n-1} / r^{2n-1} obey dk_n/dd_a = -(2n-1) k_{n+1} d_a for both damping
+// schemes, which is what makes these the derivatives of the damped fields.
+// As in the static-field kernel, the sweeps use scale 1 for every pair and the
+// finalize kernels add the per-split partial sums in a fixed order and subtract
+// (1 - s) times the listed pairs.
+// The rotation term differentiates the frame construction with forward-mode
+// dual numbers, one thread per (site, frame atom); an assemble kernel gathers
+// the per-atom sum through a fixed index list.  No atomics: results are
+// bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+
+#include "device.h"
+
+namespace ommp {
+
+#define HIP_CHECK(x)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) {                                                          \
+            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                      \
+            std::abort();                                                                \
+        }                                                                                \
+    } while (0)
+
+// Damped k2..k5 of one pair.  ir = 1/r, or 0 for the self pair (every k is then
+// 0).  ti, tj = 1/thole (+inf when not polarizable: undamped), ai, aj = Thole
+// a (AMOEBA, +inf when not polarizable so that min picks the polarizable end).
+// Wang damping defines k2..k4 only (k5 is left undamped and never used).
+template <bool AMOEBA>
+__device__ __forceinline__ void damp_k5(double r2, double ir, double ti, double tj, double ai, double aj,
+                                        double &k2, double &k3, double &k4, double &k5) {
+    const double ir2 = ir * ir;
+    k2 = ir * ir2;
+    k3 = k2 * ir2;
+    k4 = k3 * ir2;
+    k5 = k4 * ir2;
+    const double u = (r2 * ir) * (ti * tj);
+    if (AMOEBA) {
+        const double a = fmin(ai, aj);
+        const double au3 = a * (u * u * u);
+        const bool near = au3 < 50.0;
+        if (__ballot(near)) {
+            const double e = exp(-au3);
+            const double x2 = au3 * au3;
+            k2 = near ? k2 * (1.0 - e) : k2;
+            k3 = near ? k3 * (1.0 - (1.0 + au3) * e) : k3;
+            k4 = near ? k4 * (1.0 - (1.0 + au3 + 0.6 * x2) * e) : k4;
+            k5 = near ? k5 * (1.0 - (1.0 + au3 + (18.0 / 35.0) * x2 + (9.0 / 35.0) * x2 * au3) * e) : k5;
+        }
+    } else {
+        const bool near = u < 1.0;
+        if (__ballot(near)) {
+            const double u3 = u * u * u, u4 = u3 * u;
+            k2 = near ? k2 * (4.0 * u3 - 3.0 * u4) : k2;
+            k3 = near ? k3 * u4 : k3;
+            k4 = near ? k4 * (0.2 * u4) : k4;
+        }
+    }
+}
+
+// g += sum_b w_b Egrd_ab of a static multipole q (charge; AMOEBA: + dipole q[1..3],
+// quadrupole / 3 in q[4..9] as xx, xy, yy, xz, yz, zz).
+template <bool AMOEBA>
+__device__ __forceinline__ void static_grad_pair(double dx, double dy, double dz, double k2, double k3, double k4,
+                                                 double k5, const double *q, double wx, double wy, double wz,
+                                                 double &gx, double &gy, double &gz) {
+    const double wd = wx * dx + wy * dy + wz * dz;
+    if (AMOEBA) {
+        const double mx = q[1], my = q[2], mz = q[3];
+        const double md = mx * dx + my * dy + mz * dz;
+        const double mw = mx * wx + my * wy + mz * wz;
+        const double qx = q[4] * dx + q[5] * dy + q[7] * dz;
+        const double qy = q[5] * dx + q[6] * dy + q[8] * dz;
+        const double qz = q[7] * dx + q[8] * dy + q[9] * dz;
+        const double qrr = qx * dx + qy * dy + qz * dz;
+        const double wqd = wx * qx + wy * qy + wz * qz;
+        const double Qwx = q[4] * wx + q[5] * wy + q[7] * wz;
+        const double Qwy = q[5] * wx + q[6] * wy + q[8] * wz;
+        const double Qwz = q[7] * wx + q[8] * wy + q[9] * wz;
+        const double cd = 3.0 * q[0] * k3 * wd + 15.0 * k4 * md * wd - 3.0 * k3 * mw + 105.0 * k5 * qrr * wd -
+                          30.0 * k4 * wqd;
+        const double cw = -q[0] * k2 - 3.0 * k3 * md - 15.0 * k4 * qrr;
+        const double cm = -3.0 * k3 * wd;
+        const double cq = -30.0 * k4 * wd;
+        const double c6 = 6.0 * k3;
+        gx += cd * dx + cw * wx + cm * mx + cq * qx + c6 * Qwx;
+        gy += cd * dy + cw * wy + cm * my + cq * qy + c6 * Qwy;
+        gz += cd * dz + cw * wz + cm * mz + cq * qz + c6 * Qwz;
+    } else {
+        const double c = 3.0 * q[0] * k3 * wd, cw = -q[0] * k2;
+        gx += c * dx + cw * wx;
+        gy += c * dy + cw * wy;
+        gz += c * dz + cw * wz;
+    }
+}
+
+// g += f * sum_b w_b Egrd_ab(mu) = f * [15 k4 (mu.d)(w.d) d - 3 k3 (mu (w.d) + w (mu.d) + (mu.w) d)]
+__device__ __forceinline__ void dip_grad_pair(double dx, double dy, double dz, double k3, double k4, double wx,
+                                              double wy, double wz, double mx, double my, double mz, double f,
+                                              double &gx, double &gy, double &gz) {
+    const double wd = wx * dx + wy * dy + wz * dz;
+    const double md = mx * dx + my * dy + mz * dz;
+    const double mw = mx * wx + my * wy + mz * wz;
+    const double cd = f * (15.0 * k4 * md * wd - 3.0 * k3 * mw);
+    const double cw = -3.0 * f * k3 * md, cm = -3.0 * f * k3 * wd;
+    gx += cd * dx + cw * wx + cm * mx;
+    gy += cd * dy + cw * wy + cm * my;
+    gz += cd * dz + cw * wz + cm * mz;
+}
+
+// ------------------------------------------------- polarizable targets
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_gpol_static(int nmm, int npol, int chunk, const double4 *__restrict__ ms,
+                                                        const double *__restrict__ ma, const double *__restrict__ mq,
+                                                        const double4 *__restrict__ ps, const double *__restrict__ pa,
+                                                        const int *__restrict__ polar_mm,
+                                                        const double *__restrict__ ipd, double *__restrict__ part) {
+    constexpr int LD = AMOEBA ? 10 : 1;
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_a[kBlock];
+    __shared__ double s_q[LD][kBlock];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * kBlock + tid;
+    const bool valid = j < npol;
+    const long n = 3L * npol;
+    const double4 pj = valid ? ps[j] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double aj = valid ? pa[j] : 0.39;
+    const int jm = valid ? polar_mm[j] : -1;
+    double wx = 0.0, wy = 0.0, wz = 0.0;
+    if (valid) {
+        wx = ipd[3L * j];
+        wy = ipd[3L * j + 1];
+        wz = ipd[3L * j + 2];
+        if (AMOEBA) {
+            wx = 0.5 * (wx + ipd[n + 3L * j]);
+            wy = 0.5 * (wy + ipd[n + 3L * j + 1]);
+            wz = 0.5 * (wz + ipd[n + 3L * j + 2]);
+        }
+    }
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(nmm, i0 + chunk);
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            s_pos[tid] = ms[base + tid];
+            s_a[tid] = ma[base + tid];
+#pragma unroll
+            for (int c = 0; c < LD; c++) s_q[c][tid] = mq[(long)LD * (base + tid) + c];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; k++) {
+            const double4 pi = s_pos[k];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double ir = (base + k) == jm ? 0.0 : rsqrt(r2);
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, ir, pi.w, pj.w, s_a[k], aj, k2, k3, k4, k5);
+            double q[LD];
+#pragma unroll
+            for (int c = 0; c < LD; c++) q[c] = s_q[c][k];
+            static_grad_pair<AMOEBA>(dx, dy, dz, k2, k3, k4, k5, q, wx, wy, wz, gx, gy, gz);
+        }
+    }
+    if (valid) {
+        double *o = part + (long)blockIdx.y * 3 * npol;
+        o[j] = gx;
+        o[npol + j] = gy;
+        o[2L * npol + j] = gz;
+    }
+}
+
+// Dipole-dipole part: AMOEBA 1/2 [mu_D,j . Egrd(mu_P,i) + mu_P,j . Egrd(mu_D,i)],
+// AMBER mu_j . Egrd(mu_i).
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_gpol_dd(int npol, int chunk, const double4 *__restrict__ ps,
+                                                    const double *__restrict__ pa, const double *__restrict__ ipd,
+                                                    double *__restrict__ part) {
+    constexpr int NS = AMOEBA ? 6 : 3;
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_a[kBlock];
+    __shared__ double s_m[NS][kBlock];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * kBlock + tid;
+    const bool valid = j < npol;
+    const long n = 3L * npol;
+    const double4 pj = valid ? ps[j] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double aj = valid ? pa[j] : 0.39;
+    // a = mu_D,j (contracted with the field of mu_P), b = mu_P,j (with that of mu_D)
+    double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+    if (valid) {
+        ax = ipd[3L * j];
+        ay = ipd[3L * j + 1];
+        az = ipd[3L * j + 2];
+        if (AMOEBA) {
+            bx = ipd[n + 3L * j];
+            by = ipd[n + 3L * j + 1];
+            bz = ipd[n + 3L * j + 2];
+        }
+    }
+    const double f = AMOEBA ? 0.5 : 1.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(npol, i0 + chunk);
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            const long i = base + tid;
+            s_pos[tid] = ps[i];
+            s_a[tid] = pa[i];
+            // slots 0-2: mu_P (AMOEBA) / mu (AMBER); 3-5: mu_D
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                s_m[c][tid] = AMOEBA ? ipd[n + 3 * i + c] : ipd[3 * i + c];
+                if (AMOEBA) s_m[3 + c][tid] = ipd[3 * i + c];
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; k++) {
+            const double4 pi = s_pos[k];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double ir = (base + k) == j ? 0.0 : rsqrt(r2);
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, ir, pi.w, pj.w, s_a[k], aj, k2, k3, k4, k5);
+            dip_grad_pair(dx, dy, dz, k3, k4, ax, ay, az, s_m[0][k], s_m[1][k], s_m[2][k], f, gx, gy, gz);
+            if (AMOEBA)
+                dip_grad_pair(dx, dy, dz, k3, k4, bx, by, bz, s_m[3][k], s_m[4][k], s_m[5][k], f, gx, gy, gz);
+        }
+    }
+    if (valid) {
+        double *o = part + (long)blockIdx.y * 3 * npol;
+        o[j] = gx;
+        o[npol + j] = gy;
+        o[2L * npol + j] = gz;
+    }
+}
+
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_gpol_finalize(int npol, int nparts, const double *__restrict__ part,
+                                                          const double4 *__restrict__ ms,
+                                                          const double *__restrict__ ma,
+                                                          const double *__restrict__ mq,
+                                                          const double4 *__restrict__ ps,
+                                                          const double *__restrict__ pa,
+                                                          const double *__restrict__ ipd, const int *__restrict__ sp_ri,
+                                                          const int *__restrict__ sp_ci,
+                                                          const double2 *__restrict__ sp_f,
+                                                          const int *__restrict__ pp_ri,
+                                                          const int *__restrict__ pp_ci,
+                                                          const double *__restrict__ pp_f,
+                                                          double *__restrict__ gpol) {
+    constexpr int LD = AMOEBA ? 10 : 1;
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= npol) return;
+    const long n = 3L * npol;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int s = 0; s < nparts; s++) {
+        const double *p = part + (long)s * 3 * npol;
+        gx += p[j];
+        gy += p[npol + j];
+        gz += p[2L * npol + j];
+    }
+    const double4 pj = ps[j];
+    const double aj = pa[j];
+    const double dX = ipd[3L * j], dY = ipd[3L * j + 1], dZ = ipd[3L * j + 2];
+    const double pX = AMOEBA ? ipd[n + 3L * j] : dX, pY = AMOEBA ? ipd[n + 3L * j + 1] : dY,
+                 pZ = AMOEBA ? ipd[n + 3L * j + 2] : dZ;
+    // listed static sources: remove (1 - s_p) of the mu_D contraction and
+    // (1 - s_d) of the mu_P one (AMBER: both slots hold 1 - s)
+    for (int k = sp_ri[j]; k < sp_ri[j + 1]; k++) {
+        const int i = sp_ci[k];
+        const double2 f = sp_f[k];
+        const double wx = AMOEBA ? 0.5 * (f.y * dX + f.x * pX) : f.y * dX;
+        const double wy = AMOEBA ? 0.5 * (f.y * dY + f.x * pY) : f.y * dY;
+        const double wz = AMOEBA ? 0.5 * (f.y * dZ + f.x * pZ) : f.y * dZ;
+        const double4 pi = ms[i];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double k2, k3, k4, k5;
+        damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, ma[i], aj, k2, k3, k4, k5);
+        double q[LD];
+        for (int c = 0; c < LD; c++) q[c] = mq[(long)LD * i + c];
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        static_grad_pair<AMOEBA>(dx, dy, dz, k2, k3, k4, k5, q, wx, wy, wz, cx, cy, cz);
+        gx -= cx;
+        gy -= cy;
+        gz -= cz;
+    }
+    // listed dipole sources: remove (1 - s_u) of the pair
+    for (int k = pp_ri[j]; k < pp_ri[j + 1]; k++) {
+        const int i = pp_ci[k];
+        const double4 pi = ps[i];
+        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double k2, k3, k4, k5;
+        damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, pa[i], aj, k2, k3, k4, k5);
+        const double f = pp_f[k] * (AMOEBA ? 0.5 : 1.0);
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (AMOEBA) {
+            dip_grad_pair(dx, dy, dz, k3, k4, dX, dY, dZ, ipd[n + 3L * i], ipd[n + 3L * i + 1], ipd[n + 3L * i + 2],
+                          f, cx, cy, cz);
+            dip_grad_pair(dx, dy, dz, k3, k4, pX, pY, pZ, ipd[3L * i], ipd[3L * i + 1], ipd[3L * i + 2], f, cx, cy,
+                          cz);
+        } else {
+            dip_grad_pair(dx, dy, dz, k3, k4, dX, dY, dZ, ipd[3L * i], ipd[3L * i + 1], ipd[3L * i + 2], f, cx, cy,
+                          cz);
+        }
+        gx -= cx;
+        gy -= cy;
+        gz -= cz;
+    }
+    gpol[3L * j] = gx;
+    gpol[3L * j + 1] = gy;
+    gpol[3L * j + 2] = gz;
+}
+
+// ------------------------------------------------------------ MM targets
+// Field, field gradient and Q : EHes at the target of a dipole nu, e = r_target - r_source:
+//   E    = 3 k3 (nu.e) e - k2 nu
+//   Egrd = 15 k4 (nu.e) e e - 3 k3 (nu e + e nu + (nu.e) I)
+//   Q:EHes = 105 k5 (nu.e)(eQe) e - 15 k4 [nu (eQe) + 2 e (nu.Qe) + 2 (nu.e) Qe + (nu.e) tr(Q) e]
+//            + 3 k3 [nu tr(Q) + 2 Q nu]
+// acc: E (0-2), Egrd xx xy yy xz yz zz (3-8), Q:EHes (9-11).
+template <bool AMOEBA>
+__device__ __forceinline__ void dip_at_mm(double ex, double ey, double ez, double k2, double k3, double k4, double k5,
+                                          double nx, double ny, double nz, const double *Q, double trQ,
+                                          double *acc) {
+    const double ne = nx * ex + ny * ey + nz * ez;
+    const double c3 = 3.0 * k3 * ne;
+    acc[0] += c3 * ex - k2 * nx;
+    acc[1] += c3 * ey - k2 * ny;
+    acc[2] += c3 * ez - k2 * nz;
+    if (AMOEBA) {
+        const double c4 = 15.0 * k4 * ne, t3 = 3.0 * k3;
+        acc[3] += c4 * ex * ex - t3 * (2.0 * nx * ex + ne);
+        acc[4] += c4 * ex * ey - t3 * (nx * ey + ny * ex);
+        acc[5] += c4 * ey * ey - t3 * (2.0 * ny * ey + ne);
+        acc[6] += c4 * ex * ez - t3 * (nx * ez + nz * ex);
+        acc[7] += c4 * ey * ez - t3 * (ny * ez + nz * ey);
+        acc[8] += c4 * ez * ez - t3 * (2.0 * nz * ez + ne);
+        // Q = xx, xy, yy, xz, yz, zz
+        const double qx = Q[0] * ex + Q[1] * ey + Q[3] * ez;
+        const double qy = Q[1] * ex + Q[2] * ey + Q[4] * ez;
+        const double qz = Q[3] * ex + Q[4] * ey + Q[5] * ez;
+        const double eqe = qx * ex + qy * ey + qz * ez;
+        const double nqe = nx * qx + ny * qy + nz * qz;
+        const double qnx = Q[0] * nx + Q[1] * ny + Q[3] * nz;
+        const double qny = Q[1] * nx + Q[2] * ny + Q[4] * nz;
+        const double qnz = Q[3] * nx + Q[4] * ny + Q[5] * nz;
+        const double ce = 105.0 * k5 * ne * eqe - 15.0 * k4 * (2.0 * nqe + ne * trQ);
+        const double cn = -15.0 * k4 * eqe + 3.0 * k3 * trQ;
+        const double cq = -30.0 * k4 * ne;
+        const double cqn = 6.0 * k3;
+        acc[9] += ce * ex + cn * nx + cq * qx + cqn * qnx;
+        acc[10] += ce * ey + cn * ny + cq * qy + cqn * qny;
+        acc[11] += ce * ez + cn * nz + cq * qz + cqn * qnz;
+    }
+}
+
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_gmm_pairs(int nmm, int npol, int chunk, const double4 *__restrict__ ms,
+                                                      const double *__restrict__ ma, const double *__restrict__ mq,
+                                                      const double4 *__restrict__ ps,
+                                                      const double *__restrict__ pa,
+                                                      const int *__restrict__ mm_polar,
+                                                      const double *__restrict__ ipd, double *__restrict__ part) {
+    constexpr int NA = AMOEBA ? 12 : 3;
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_a[kBlock];
+    __shared__ double s_n[3][kBlock];
+    const int tid = threadIdx.x;
+    const int m = blockIdx.x * kBlock + tid;
+    const bool valid = m < nmm;
+    const long n = 3L * npol;
+    const double4 pm = valid ? ms[m] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double am = valid ? ma[m] : 0.39;
+    const int mp = valid ? mm_polar[m] : -1;
+    double Q[6] = {0, 0, 0, 0, 0, 0};
+    if (AMOEBA && valid)
+        for (int c = 0; c < 6; c++) Q[c] = mq[10L * m + 4 + c];
+    const double trQ = Q[0] + Q[2] + Q[5];
+    double acc[NA];
+#pragma unroll
+    for (int c = 0; c < NA; c++) acc[c] = 0.0;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(npol, i0 + chunk);
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            const long i = base + tid;
+            s_pos[tid] = ps[i];
+            s_a[tid] = pa[i];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                s_n[c][tid] = AMOEBA ? 0.5 * (ipd[3 * i + c] + ipd[n + 3 * i + c]) : ipd[3 * i + c];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; k++) {
+            const double4 pi = s_pos[k];
+            const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            const double ir = (base + k) == mp ? 0.0 : rsqrt(r2);
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, ir, pi.w, pm.w, s_a[k], am, k2, k3, k4, k5);
+            dip_at_mm<AMOEBA>(ex, ey, ez, k2, k3, k4, k5, s_n[0][k], s_n[1][k], s_n[2][k], Q, trQ, acc);
+        }
+    }
+    if (valid) {
+        double *o = part + (long)blockIdx.y * NA * nmm;
+#pragma unroll
+        for (int c = 0; c < NA; c++) o[(long)c * nmm + m] = acc[c];
+    }
+}
+
+// E_D2M -> emm[m][3], Egrd_D2M -> egrd[m][6] (AMOEBA), and the MM-site gradient
+// -q E + mu . Egrd - Q : EHes -> gtr[m][3].
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_gmm_finalize(int nmm, int npol, int nparts,
+                                                         const double *__restrict__ part,
+                                                         const double4 *__restrict__ ms,
+                                                         const double *__restrict__ ma, const double *__restrict__ mq,
+                                                         const double4 *__restrict__ ps,
+                                                         const double *__restrict__ pa,
+                                                         const double *__restrict__ ipd,
+                                                         const int *__restrict__ mp_ri, const int *__restrict__ mp_ci,
+                                                         const double2 *__restrict__ mp_f, double *__restrict__ emm,
+                                                         double *__restrict__ egrd, double *__restrict__ gtr) {
+    constexpr int NA = AMOEBA ? 12 : 3;
+    constexpr int LD = AMOEBA ? 10 : 1;
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= nmm) return;
+    const long n = 3L * npol;
+    double acc[NA];
+    for (int c = 0; c < NA; c++) acc[c] = 0.0;
+    for (int s = 0; s < nparts; s++) {
+        const double *p = part + (long)s * NA * nmm;
+        for (int c = 0; c < NA; c++) acc[c] += p[(long)c * nmm + m];
+    }
+    double q[LD];
+    for (int c = 0; c < LD; c++) q[c] = mq[(long)LD * m + c];
+    const double *Q = q + (AMOEBA ? 4 : 0);
+    const double trQ = AMOEBA ? Q[0] + Q[2] + Q[5] : 0.0;
+    const double4 pm = ms[m];
+    const double am = ma[m];
+    // listed sources: remove 1/2 ((1 - s_d) mu_P + (1 - s_p) mu_D) (AMBER: (1 - s) mu)
+    for (int k = mp_ri[m]; k < mp_ri[m + 1]; k++) {
+        const int i = mp_ci[k];
+        const double2 f = mp_f[k];
+        double nx, ny, nz;
+        if (AMOEBA) {
+            nx = -0.5 * (f.x * ipd[n + 3L * i] + f.y * ipd[3L * i]);
+            ny = -0.5 * (f.x * ipd[n + 3L * i + 1] + f.y * ipd[3L * i + 1]);
+            nz = -0.5 * (f.x * ipd[n + 3L * i + 2] + f.y * ipd[3L * i + 2]);
+        } else {
+            nx = -f.x * ipd[3L * i];
+            ny = -f.x * ipd[3L * i + 1];
+            nz = -f.x * ipd[3L * i + 2];
+        }
+        const double4 pi = ps[i];
+        const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
+        const double r2 = ex * ex + ey * ey + ez * ez;
+        double k2, k3, k4, k5;
+        damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pm.w, pa[i], am, k2, k3, k4, k5);
+        dip_at_mm<AMOEBA>(ex, ey, ez, k2, k3, k4, k5, nx, ny, nz, Q, trQ, acc);
+    }
+    double g[3];
+    for (int c = 0; c < 3; c++) {
+        g[c] = -q[0] * acc[c];
+        emm[3L * m + c] = acc[c];
+    }
+    if (AMOEBA) {
+        const double *G = acc + 3;  // xx xy yy xz yz zz
+        g[0] += q[1] * G[0] + q[2] * G[1] + q[3] * G[3] - acc[9];
+        g[1] += q[1] * G[1] + q[2] * G[2] + q[3] * G[4] - acc[10];
+        g[2] += q[1] * G[3] + q[2] * G[4] + q[3] * G[5] - acc[11];
+        for (int c = 0; c < 6; c++) egrd[6L * m + c] = G[c];
+    }
+    for (int c = 0; c < 3; c++) gtr[3L * m + c] = g[c];
+}
+
+// ------------------------------------------------------------- rotation
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+    const double iv = 1.0 / b.v;
+    return {a.v * iv, (a.d - a.v * iv * b.d) * iv};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a) {
+    const double s = sqrt(a.v);
+    return {s, 0.5 * a.d / s};
+}
+__device__ __forceinline__ Dual dot3(const Dual *a, const Dual *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// rotation_matrix (rotate_multipoles.F90:298-463): columns of r are the frame
+// axes (ex, ey, ez) of a site at p[0] with x, y, z reference atoms at p[1..3].
+__device__ void frame_axes(int conv, const Dual (&p)[4][3], Dual (&r)[3][3]) {
+    Dual xi[3], eta[3], zeta[3], u[3], v[3];
+    for (int k = 0; k < 3; k++) {
+        xi[k] = p[3][k] - p[0][k];
+        eta[k] = p[1][k] - p[0][k];
+        zeta[k] = p[2][k] - p[0][k];
+    }
+    const Dual xn = dsqrt(dot3(xi, xi)), en = dsqrt(dot3(eta, eta)), zn = dsqrt(dot3(zeta, zeta));
+    for (int k = 0; k < 3; k++) {
+        switch (conv) {
+            case 1:  // z-then-x
+                u[k] = xi[k];
+                v[k] = eta[k];
+                break;
+            case 2:  // bisector
+                u[k] = en * xi[k] + xn * eta[k];
+                v[k] = eta[k];
+                break;
+            case 3:  // z-only: v is a fixed lab axis
+                u[k] = xi[k];
+                v[k] = {0.0, 0.0};
+                break;
+            case 4:  // z-bisector
+                u[k] = xi[k];
+                v[k] = zn * eta[k] + en * zeta[k];
+                break;
+            default:  // 5: 3-fold
+                u[k] = en * zn * xi[k] + xn * zn * eta[k] + xn * en * zeta[k];
+                v[k] = eta[k];
+                break;
+        }
+    }
+    if (conv == 3) {
+        if (u[2].v / xn.v <= 0.866 && fabs(u[1].v) > kEpsRpDev) v[0].v = 1.0;
+        else v[1].v = 1.0;
+    }
+    const Dual un = dsqrt(dot3(u, u));
+    Dual ez[3], ex[3], ey[3];
+    for (int k = 0; k < 3; k++) ez[k] = u[k] / un;
+    const Dual vez = dot3(v, ez);
+    for (int k = 0; k < 3; k++) ex[k] = v[k] - vez * ez[k];
+    const Dual exn = dsqrt(dot3(ex, ex));
+    for (int k = 0; k < 3; k++) ex[k] = ex[k] / exn;
+    ey[0] = ez[1] * ex[2] - ez[2] * ex[1];
+    ey[1] = ez[2] * ex[0] - ez[0] * ex[2];
+    ey[2] = ez[0] * ex[1] - ez[1] * ex[0];
+    for (int k = 0; k < 3; k++) {
+        r[k][0] = ex[k];
+        r[k][1] = ey[k];
+        r[k][2] = ez[k];
+    }
+}
+
+// One thread per (site j, frame atom role: 0 self, 1 x, 2 y, 3 z); rotc[(4 j + role)][t]
+// = d/dr_t of  -mu_lab . E_j + sum_ab Q_lab,ab Egrd_j,ab  through that atom.
+__global__ __launch_bounds__(kBlock) void k_grot(int nmm, const double4 *__restrict__ ms,
+                                                 const double *__restrict__ q0, const int4 *__restrict__ frames,
+                                                 const double *__restrict__ emm, const double *__restrict__ egrd,
+                                                 double *__restrict__ rotc) {
+    const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= 4L * nmm) return;
+    const int j = (int)(t >> 2), role = (int)(t & 3);
+    const int4 f = frames[j];
+    double *o = rotc + 3 * t;
+    if (f.x == 0) {
+        o[0] = o[1] = o[2] = 0.0;
+        return;
+    }
+    // frames = (convention, z atom, x atom, y atom), -1 = the site itself
+    const int at[4] = {j, f.z >= 0 ? f.z : j, f.w >= 0 ? f.w : j, f.y >= 0 ? f.y : j};
+    double pos[4][3];
+    for (int a = 0; a < 4; a++) {
+        const double4 c = ms[at[a]];
+        pos[a][0] = c.x;
+        pos[a][1] = c.y;
+        pos[a][2] = c.z;
+    }
+    const double *mu0 = q0 + 10L * j + 1, *Qs = q0 + 10L * j + 4;
+    const double Q0[3][3] = {{Qs[0], Qs[1], Qs[3]}, {Qs[1], Qs[2], Qs[4]}, {Qs[3], Qs[4], Qs[5]}};
+    const double *g6 = egrd + 6L * j;
+    const double G[3][3] = {{g6[0], g6[1], g6[3]}, {g6[1], g6[2], g6[4]}, {g6[3], g6[4], g6[5]}};
+    const double E[3] = {emm[3L * j], emm[3L * j + 1], emm[3L * j + 2]};
+    for (int tc = 0; tc < 3; tc++) {
+        Dual p[4][3], R[3][3];
+        for (int a = 0; a < 4; a++)
+            for (int k = 0; k < 3; k++) p[a][k] = {pos[a][k], (a == role && k == tc) ? 1.0 : 0.0};
+        frame_axes(f.x, p, R);
+        // -(dR mu0) . E + 2 sum_ab (dR Q0 R^T)_ab G_ab  (G symmetric)
+        double acc = 0.0;
+        for (int a = 0; a < 3; a++) {
+            const double dmu = R[a][0].d * mu0[0] + R[a][1].d * mu0[1] + R[a][2].d * mu0[2];
+            acc -= dmu * E[a];
+            double M[3];
+            for (int c = 0; c < 3; c++) M[c] = R[a][0].d * Q0[0][c] + R[a][1].d * Q0[1][c] + R[a][2].d * Q0[2][c];
+            for (int b = 0; b < 3; b++) {
+                const double mr = M[0] * R[b][0].v + M[1] * R[b][1].v + M[2] * R[b][2].v;
+                acc += 2.0 * mr * G[a][b];
+            }
+        }
+        o[tc] = acc;
+    }
+}
+
+// out[a] = gtr[a] + gpol[mm_polar[a]] + sum of the rotation terms listed for a.
+__global__ __launch_bounds__(kBlock) void k_gassemble(int nmm, const double *__restrict__ gtr,
+                                                      const double *__restrict__ gpol,
+                                                      const int *__restrict__ mm_polar,
+                                                      const int *__restrict__ rev_ri, const int *__restrict__ rev_ci,
+                                                      const double *__restrict__ rotc, double *__restrict__ out) {
+    const int a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= nmm) return;
+    double g[3] = {0.0, 0.0, 0.0};
+    if (gtr)
+        for (int c = 0; c < 3; c++) g[c] = gtr[3L * a + c];
+    if (gpol) {
+        const int p = mm_polar[a];
+        if (p >= 0)
+            for (int c = 0; c < 3; c++) g[c] += gpol[3L * p + c];
+    }
+    if (rotc)
+        for (int k = rev_ri[a]; k < rev_ri[a + 1]; k++) {
+            const long e = rev_ci[k];
+            for (int c = 0; c < 3; c++) g[c] += rotc[3 * e + c];
+        }
+    for (int c = 0; c < 3; c++) out[3L * a + c] = g[c];
+}
+
+// ------------------------------------------------------------- launchers
+void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb, hipEvent_t eve) {
+    const int nmm = a.nmm, npol = a.npol;
+    const int pb = (npol + kBlock - 1) / kBlock, mb = (nmm + kBlock - 1) / kBlock;
+    if (evb) HIP_CHECK(hipEventRecord(evb, st));
+    if (npol > 0) {
+        const int cs = (nmm + a.split_s - 1) / a.split_s, cp = (npol + a.split_p - 1) / a.split_p;
+        double *part_p = a.part_pol + (long)a.split_s * 3 * npol;
+        const int cm = (npol + a.split_m - 1) / a.split_m;
+        if (a.amoeba) {
+            hipLaunchKernelGGL(k_gpol_static<true>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms,
+                               a.ma, a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
+            hipLaunchKernelGGL(k_gpol_dd<true>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa,
+                               a.ipd, part_p);
+            hipLaunchKernelGGL(k_gmm_pairs<true>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms,
+                               a.ma, a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
+        } else {
+            hipLaunchKernelGGL(k_gpol_static<false>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms,
+                               a.ma, a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
+            hipLaunchKernelGGL(k_gpol_dd<false>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa,
+                               a.ipd, part_p);
+            hipLaunchKernelGGL(k_gmm_pairs<false>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms,
+                               a.ma, a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+    if (eve) HIP_CHECK(hipEventRecord(eve, st));
+    if (npol > 0) {
+        const int np = a.split_s + a.split_p;
+        if (a.amoeba) {
+            hipLaunchKernelGGL(k_gpol_finalize<true>, dim3(pb), dim3(kBlock), 0, st, npol, np, a.part_pol, a.ms,
+                               a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci, a.pp.f,
+                               a.gpol);
+            hipLaunchKernelGGL(k_gmm_finalize<true>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, a.split_m, a.part_mm,
+                               a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci, a.mp.f2, a.emm, a.egrd,
+                               a.gtr);
+        } else {
+            hipLaunchKernelGGL(k_gpol_finalize<false>, dim3(pb), dim3(kBlock), 0, st, npol, np, a.part_pol, a.ms,
+                               a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci, a.pp.f,
+                               a.gpol);
+            hipLaunchKernelGGL(k_gmm_finalize<false>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, a.split_m,
+                               a.part_mm, a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci, a.mp.f2, a.emm,
+                               a.egrd, a.gtr);
+        }
+        HIP_CHECK(hipGetLastError());
+    }
+    if (a.amoeba) {
+        const long nt = 4L * nmm;
+        hipLaunchKernelGGL(k_grot, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, nmm, a.ms,
+                           a.q0, a.frames, a.emm, a.egrd, a.rotc);
+        HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_gassemble, dim3(mb), dim3(kBlock), 0, st, nmm, npol > 0 ? a.gtr : nullptr,
+                       npol > 0 ? a.gpol : nullptr, a.mm_polar, a.rev_ri, a.rev_ci, a.amoeba ? a.rotc : nullptr,
+                       a.out);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_rotation_grad(hipStream_t st, const GradArgs &a) {
+    const int nmm = a.nmm;
+    const long nt = 4L * nmm;
+    hipLaunchKernelGGL(k_grot, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, nmm, a.ms, a.q0,
+                       a.frames, a.emm, a.egrd, a.rotc);
+    HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_gassemble, dim3((nmm + kBlock - 1) / kBlock), dim3(kBlock), 0, st, nmm, nullptr, nullptr,
+                       a.mm_polar, a.rev_ri, a.rev_ci, a.rotc, a.out);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace ommp

@@ -121,6 +121,8 @@ void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *i
                  double *epol);
 void charges_field_host(System &s, int nq, const double *charges, double *out);
 void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V, double *E);
+void polelec_geomgrad_host(System &s, double *grd);
+void rotation_geomgrad_host(System &s, const double *E, const double *Egrd, double *grd);
 
 // server.cpp
 void *server_create(double budget, int max_active);

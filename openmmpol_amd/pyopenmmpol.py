@@ -187,6 +187,46 @@ class OMMPSystem:
     def field_mmpol2ext(self, cext):
         return self._x2e(lib().ommp_field_mmpol2ext, cext, True)
 
+    # geometric gradients (pyommp_interface.cpp:518-606): [mm_atoms, 3], Hartree/bohr
+    def _numerical_grad(self, energy, dd=1e-5):
+        """Central differences of energy() over every MM coordinate, as the
+        reference's numerical_grad; the coordinates are restored exactly."""
+        c0 = np.array(self.cmm, dtype=np.float64)
+        g = np.zeros_like(c0)
+        try:
+            for i in range(c0.shape[0]):
+                for k in range(3):
+                    c = c0.copy()
+                    c[i, k] += dd
+                    self.update_coordinates(c)
+                    ep = energy()
+                    c[i, k] = c0[i, k] - dd
+                    self.update_coordinates(c)
+                    g[i, k] = (ep - energy()) / (2.0 * dd)
+        finally:
+            self.update_coordinates(c0)
+        return g
+
+    def polelec_geomgrad(self, numerical=False):
+        """d E_pol / d r of every MM atom (ommp_polelec_geomgrad)."""
+        if numerical:
+            return self._numerical_grad(self.get_polelec_energy)
+        g = np.zeros((self.mm_atoms, 3))
+        lib().ommp_polelec_geomgrad(self.handler, g.ctypes.data_as(c_dp))
+        return g
+
+    def rotation_geomgrad(self, E, Egrd):
+        """Gradient through the multipole frames of the energy of the rotated
+        multipoles in a field E [mm_atoms, 3] and field gradient Egrd [mm_atoms, 6]."""
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        Egrd = np.ascontiguousarray(Egrd, dtype=np.float64)
+        if E.shape != (self.mm_atoms, 3) or Egrd.shape != (self.mm_atoms, 6):
+            raise ValueError("E should be shaped [mm_atoms, 3] and Egrd [mm_atoms, 6]")
+        g = np.zeros((self.mm_atoms, 3))
+        lib().ommp_rotation_geomgrad(self.handler, E.ctypes.data_as(c_dp), Egrd.ctypes.data_as(c_dp),
+                                     g.ctypes.data_as(c_dp))
+        return g
+
     def update_coordinates(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)
         if c.ndim != 2 or c.shape[0] != self.mm_atoms or c.shape[1] != 3:
@@ -267,10 +307,11 @@ class OMMPSystem:
         lib().ommp_amd_profile_kernels(self.handler, bool(enable))
 
     def kernel_stats(self, reset=False):
-        """{class: (launches, total ms)} for otf64, otf32, incore32, incore64, static_field."""
-        out = (ctypes.c_double * 10)()
+        """{class: (launches, total ms)} for otf64, otf32, incore32, incore64, static_field,
+        geomgrad."""
+        out = (ctypes.c_double * 12)()
         lib().ommp_amd_get_kernel_stats(self.handler, out, bool(reset))
-        names = ["otf64", "otf32", "incore32", "incore64", "static_field"]
+        names = ["otf64", "otf32", "incore32", "incore64", "static_field", "geomgrad"]
         return {nm: (int(out[2 * k]), out[2 * k + 1]) for k, nm in enumerate(names)}
 
     def set_external_field_device(self, d_ext_ptr, solver="none"):

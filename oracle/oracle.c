@@ -21,6 +21,7 @@
  *   - screening lists: the reference's summary_*.ref fixtures for the
  *     topology levels they print, plus brute-force rules on tiny systems.
  */
+#include <complex.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -974,6 +975,331 @@ void oracle_x2e(void *h, const double *ipd, int n, const double *cext, int which
     }
     if (bad)
         oracle_fatal(s, "Requesting Coulomb kernel for two atoms placed in the same point");
+}
+
+/* ------------------------------------------------- geometric gradients */
+/* Field gradient G[3][3] (G_ab = -dE_b/dr_a at the target, the quantity the
+   reference stores as Egrd) and field Hessian H[3][3][3] (H_abc = d2E_c /
+   dr_a dr_b, stored as EHes) of a charge, a dipole and a quadrupole (stored
+   /3), ADDED to G and H (either may be NULL): grdE and HE branches of
+   q_elec_prop / mu_elec_prop / quad_elec_prop (mod_electrostatics.F90:1128-
+   1174, 1207-1283, 1320-1345), written as full tensors.  k[1..4] = damped
+   lambda_3/r^3 .. lambda_9/r^9, d = r_target - r_source. */
+static void q_grad(double q, const double d[3], const double *k, double G[3][3])
+{
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) G[a][b] += q * (3.0 * k[2] * d[a] * d[b] - (a == b ? k[1] : 0.0));
+}
+static void mu_grad_hes(const double *mu, const double d[3], const double *k, double G[3][3], double H[3][3][3])
+{
+    double md = mu[0] * d[0] + mu[1] * d[1] + mu[2] * d[2];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double dab = a == b ? 1.0 : 0.0;
+            if (G) G[a][b] += 15.0 * k[3] * md * d[a] * d[b] - 3.0 * k[2] * (mu[a] * d[b] + mu[b] * d[a] + md * dab);
+            if (H)
+                for (int c = 0; c < 3; c++) {
+                    double dac = a == c ? 1.0 : 0.0, dbc = b == c ? 1.0 : 0.0;
+                    H[a][b][c] += 105.0 * k[4] * md * d[a] * d[b] * d[c]
+                                  - 15.0 * k[3] * (mu[a] * d[b] * d[c] + mu[b] * d[a] * d[c] + mu[c] * d[a] * d[b]
+                                                   + md * (dab * d[c] + dac * d[b] + dbc * d[a]))
+                                  + 3.0 * k[2] * (mu[a] * dbc + mu[b] * dac + mu[c] * dab);
+                }
+        }
+}
+static void quad_grad(const double *Qs, const double d[3], const double *k, double G[3][3])
+{
+    const double Q[3][3] = {{Qs[0], Qs[1], Qs[3]}, {Qs[1], Qs[2], Qs[4]}, {Qs[3], Qs[4], Qs[5]}};
+    double qd[3];
+    for (int a = 0; a < 3; a++) qd[a] = Q[a][0] * d[0] + Q[a][1] * d[1] + Q[a][2] * d[2];
+    double qrr = qd[0] * d[0] + qd[1] * d[1] + qd[2] * d[2];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            G[a][b] += 105.0 * k[4] * qrr * d[a] * d[b] - 30.0 * k[3] * (qd[a] * d[b] + qd[b] * d[a])
+                       - (a == b ? 15.0 * k[3] * qrr : 0.0) + 6.0 * k[2] * Q[a][b];
+}
+static void add_scaled_gh(double f, double G[3][3], double H[3][3][3], double Gt[3][3], double Ht[3][3][3])
+{
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            Gt[a][b] += f * G[a][b];
+            if (H)
+                for (int c = 0; c < 3; c++) Ht[a][b][c] += f * H[a][b][c];
+        }
+}
+
+/* polelec_geomgrad (mod_geomgrad.F90:113-273) without its last call
+   (rotation_geomgrad, see oracle_rotation_geomgrad), on the fields that
+   prepare_polelec(do_gg) builds (mod_electrostatics.F90:1563-1590):
+     Egrd_M2D  field gradient of the static multipoles at the polarizable
+               sites, D and P screening as E_M2D (elec_prop_M2D, :2313-2732);
+     Egrd_D2D  field gradient of mu_P and of mu_D at the polarizable sites,
+               u-screening (elec_prop_D2D, :2113-2311);
+     E/Egrd/EHes_D2M  field, gradient and Hessian at every MM site of
+               1/2 (mu_P with D screening + mu_D with P screening) for AMOEBA,
+               of mu with P screening otherwise (elec_prop_D2M, :2736-3065).
+   grad[nmm][3] is OVERWRITTEN with
+     - q E_D2M + mu . Egrd_D2M - Q : EHes_D2M       at every MM site,
+     + 1/2 mu_D . (Egrd_M2D + Egrd_D2D)(P) + 1/2 mu_P . (...)(D)  (AMOEBA) or
+       mu . (Egrd_M2D + Egrd_D2D) (AMBER)            at every polarizable site.
+   Emm[nmm][3] and Egrd[nmm][6] (xx, xy, yy, xz, yz, zz) receive E_D2M and
+   Egrd_D2M, the inputs of the rotation term.  ipd is [n_ipd][npol][3]. */
+void oracle_geomgrad(void *h, const double *ipd, double *grad, double *Emm, double *Egrd)
+{
+    osys *s = (osys *)h;
+    const int npol = s->npol, nmm = s->nmm, amoeba = s->amoeba;
+    const long n = 3L * npol;
+    const double *muD = ipd, *muP = amoeba ? ipd + n : ipd;
+    memset(grad, 0, sizeof(double) * 3 * nmm);
+    memset(Emm, 0, sizeof(double) * 3 * nmm);
+    memset(Egrd, 0, sizeof(double) * 6 * nmm);
+    /* polarizable targets */
+    #pragma omp parallel
+    {
+        double *sp = (double *)malloc(sizeof(double) * (nmm ? nmm : 1));
+        double *sd = (double *)malloc(sizeof(double) * (nmm ? nmm : 1));
+        double *su = (double *)malloc(sizeof(double) * (npol ? npol : 1));
+        char *hp = (char *)calloc(nmm ? nmm : 1, 1), *hd = (char *)calloc(nmm ? nmm : 1, 1);
+        char *hu = (char *)calloc(npol ? npol : 1, 1);
+        #pragma omp for schedule(dynamic, 4)
+        for (int j = 0; j < npol; j++) {
+            for (int k = s->tS_P_P.ri[j]; k < s->tS_P_P.ri[j + 1]; k++) {
+                sp[s->tS_P_P.ci[k]] = s->tS_P_P.val[k]; hp[s->tS_P_P.ci[k]] = 1;
+            }
+            if (amoeba)
+                for (int k = s->tS_P_D.ri[j]; k < s->tS_P_D.ri[j + 1]; k++) {
+                    sd[s->tS_P_D.ci[k]] = s->tS_P_D.val[k]; hd[s->tS_P_D.ci[k]] = 1;
+                }
+            for (int k = s->tP_P.ri[j]; k < s->tP_P.ri[j + 1]; k++) {
+                su[s->tP_P.ci[k]] = s->tP_P.val[k]; hu[s->tP_P.ci[k]] = 1;
+            }
+            const int jm = s->polar_mm[j];
+            double GP[3][3] = {{0}}, GD[3][3] = {{0}};
+            /* Egrd_M2D */
+            for (int i = 0; i < nmm; i++) {
+                if (i == jm) continue;
+                double scp = hp[i] ? sp[i] : 1.0, scd = 1.0;
+                int do_p = !hp[i] || fabs(scp) > EPS_RP, do_d = 0;
+                if (amoeba) {
+                    scd = hd[i] ? sd[i] : 1.0;
+                    do_d = !hd[i] || fabs(scd) > EPS_RP;
+                }
+                if (!(do_p || do_d)) continue;
+                double k[5], dr[3], G[3][3] = {{0}};
+                damped_kernel(s, i, jm, amoeba ? 4 : 2, k, dr);
+                q_grad(s->q[s->ld_cart * i], dr, k, G);
+                if (amoeba) {
+                    mu_grad_hes(s->q + 10 * i + 1, dr, k, G, NULL);
+                    quad_grad(s->q + 10 * i + 4, dr, k, G);
+                }
+                if (do_p) add_scaled_gh(scp, G, NULL, GP, NULL);
+                if (do_d) add_scaled_gh(scd, G, NULL, GD, NULL);
+            }
+            /* Egrd_D2D: field of mu_P adds to the P set, of mu_D to the D set */
+            for (int i = 0; i < npol; i++) {
+                if (i == j) continue;
+                double scu = hu[i] ? su[i] : 1.0;
+                if (hu[i] && !(fabs(scu) > EPS_RP)) continue;
+                double k[5], dr[3], G[3][3] = {{0}};
+                damped_kernel(s, s->polar_mm[i], jm, 3, k, dr);
+                mu_grad_hes(muP + 3 * i, dr, k, G, NULL);
+                add_scaled_gh(scu, G, NULL, GP, NULL);
+                if (amoeba) {
+                    double G2[3][3] = {{0}};
+                    mu_grad_hes(muD + 3 * i, dr, k, G2, NULL);
+                    add_scaled_gh(scu, G2, NULL, GD, NULL);
+                }
+            }
+            double g[3] = {0, 0, 0};
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    if (amoeba)
+                        g[a] += 0.5 * muD[3 * j + b] * GP[a][b] + 0.5 * muP[3 * j + b] * GD[a][b];
+                    else
+                        g[a] += muP[3 * j + b] * GP[a][b];
+                }
+            /* each polarizable site belongs to one MM atom: no write conflict */
+            for (int a = 0; a < 3; a++) grad[3 * jm + a] += g[a];
+            for (int k = s->tS_P_P.ri[j]; k < s->tS_P_P.ri[j + 1]; k++) hp[s->tS_P_P.ci[k]] = 0;
+            if (amoeba)
+                for (int k = s->tS_P_D.ri[j]; k < s->tS_P_D.ri[j + 1]; k++) hd[s->tS_P_D.ci[k]] = 0;
+            for (int k = s->tP_P.ri[j]; k < s->tP_P.ri[j + 1]; k++) hu[s->tP_P.ci[k]] = 0;
+        }
+        free(sp); free(sd); free(su); free(hp); free(hd); free(hu);
+    }
+    /* MM targets: fields of the induced dipoles */
+    #pragma omp parallel
+    {
+        double *sp = (double *)malloc(sizeof(double) * (npol ? npol : 1));
+        double *sd = (double *)malloc(sizeof(double) * (npol ? npol : 1));
+        char *hp = (char *)calloc(npol ? npol : 1, 1), *hd = (char *)calloc(npol ? npol : 1, 1);
+        #pragma omp for schedule(dynamic, 4)
+        for (int m = 0; m < nmm; m++) {
+            for (int k = s->list_S_P_P.ri[m]; k < s->list_S_P_P.ri[m + 1]; k++) {
+                sp[s->list_S_P_P.ci[k]] = s->list_S_P_P.val[k]; hp[s->list_S_P_P.ci[k]] = 1;
+            }
+            if (amoeba)
+                for (int k = s->list_S_P_D.ri[m]; k < s->list_S_P_D.ri[m + 1]; k++) {
+                    sd[s->list_S_P_D.ci[k]] = s->list_S_P_D.val[k]; hd[s->list_S_P_D.ci[k]] = 1;
+                }
+            double E[3] = {0, 0, 0}, G[3][3] = {{0}}, H[3][3][3] = {{{0}}};
+            for (int i = 0; i < npol; i++) {
+                if (s->polar_mm[i] == m) continue;
+                double k[5], dr[3];
+                damped_kernel(s, s->polar_mm[i], m, amoeba ? 4 : 2, k, dr);
+                if (amoeba) {
+                    /* 'P' kind: mu_P screened as the D field; 'D' kind: mu_D screened as P */
+                    double scd = hd[i] ? sd[i] : 1.0, scp = hp[i] ? sp[i] : 1.0;
+                    if (!hd[i] || fabs(scd) > EPS_RP) {
+                        double Et[3] = {0, 0, 0}, Gt[3][3] = {{0}}, Ht[3][3][3] = {{{0}}};
+                        mu_field(muP + 3 * i, dr, k, Et);
+                        mu_grad_hes(muP + 3 * i, dr, k, Gt, Ht);
+                        for (int c = 0; c < 3; c++) E[c] += 0.5 * scd * Et[c];
+                        add_scaled_gh(0.5 * scd, Gt, Ht, G, H);
+                    }
+                    if (!hp[i] || fabs(scp) > EPS_RP) {
+                        double Et[3] = {0, 0, 0}, Gt[3][3] = {{0}}, Ht[3][3][3] = {{{0}}};
+                        mu_field(muD + 3 * i, dr, k, Et);
+                        mu_grad_hes(muD + 3 * i, dr, k, Gt, Ht);
+                        for (int c = 0; c < 3; c++) E[c] += 0.5 * scp * Et[c];
+                        add_scaled_gh(0.5 * scp, Gt, Ht, G, H);
+                    }
+                } else {
+                    double scp = hp[i] ? sp[i] : 1.0;
+                    if (hp[i] && !(fabs(scp) > EPS_RP)) continue;
+                    double Et[3] = {0, 0, 0};
+                    mu_field(muP + 3 * i, dr, k, Et);
+                    for (int c = 0; c < 3; c++) E[c] += scp * Et[c];
+                }
+            }
+            double g[3];
+            const double *qm = s->q + (long)s->ld_cart * m;
+            for (int a = 0; a < 3; a++) g[a] = -qm[0] * E[a];
+            if (amoeba) {
+                const double *Qs = qm + 4;
+                const double Q[3][3] = {{Qs[0], Qs[1], Qs[3]}, {Qs[1], Qs[2], Qs[4]}, {Qs[3], Qs[4], Qs[5]}};
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) {
+                        g[a] += qm[1 + b] * G[a][b];
+                        for (int c = 0; c < 3; c++) g[a] -= Q[b][c] * H[b][c][a];
+                    }
+                Egrd[6 * m + 0] = G[0][0]; Egrd[6 * m + 1] = G[0][1]; Egrd[6 * m + 2] = G[1][1];
+                Egrd[6 * m + 3] = G[0][2]; Egrd[6 * m + 4] = G[1][2]; Egrd[6 * m + 5] = G[2][2];
+            }
+            for (int a = 0; a < 3; a++) Emm[3 * m + a] = E[a];
+            #pragma omp critical
+            for (int a = 0; a < 3; a++) grad[3 * m + a] += g[a];
+            for (int k = s->list_S_P_P.ri[m]; k < s->list_S_P_P.ri[m + 1]; k++) hp[s->list_S_P_P.ci[k]] = 0;
+            if (amoeba)
+                for (int k = s->list_S_P_D.ri[m]; k < s->list_S_P_D.ri[m + 1]; k++) hd[s->list_S_P_D.ci[k]] = 0;
+        }
+        free(sp); free(sd); free(hp); free(hd);
+    }
+}
+
+/* rotation_matrix (rotate_multipoles.F90:298-463) in complex arithmetic: the
+   frame vectors of site c from the z, x and y reference atoms.  Evaluated
+   with an imaginary perturbation of one coordinate it returns that
+   coordinate's derivative in the imaginary part (complex-step
+   differentiation, Squire & Trapp, SIAM Rev. 40 (1998) 110), which is exact
+   to rounding -- the reference instead chains hand-written derivatives
+   (:464-642).  Returns 1 for an ill-defined frame. */
+static int rot_cplx(int frame, const double complex c[3], const double complex cx[3],
+                    const double complex cy[3], const double complex cz[3], double complex r[3][3])
+{
+    double complex xi[3], eta[3], zeta[3], u[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        xi[k] = cz[k] - c[k]; eta[k] = cx[k] - c[k]; zeta[k] = cy[k] - c[k];
+        for (int l = 0; l < 3; l++) r[k][l] = k == l ? 1.0 : 0.0;
+    }
+    if (frame == 0) return 0;
+    double complex xn = csqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+    double complex en = csqrt(eta[0] * eta[0] + eta[1] * eta[1] + eta[2] * eta[2]);
+    double complex zn = csqrt(zeta[0] * zeta[0] + zeta[1] * zeta[1] + zeta[2] * zeta[2]);
+    for (int k = 0; k < 3; k++) {
+        switch (frame) {
+        case 1: u[k] = xi[k]; v[k] = eta[k]; break;
+        case 2: u[k] = en * xi[k] + xn * eta[k]; v[k] = eta[k]; break;
+        case 3: u[k] = xi[k]; break;
+        case 4: u[k] = xi[k]; v[k] = zn * eta[k] + en * zeta[k]; break;
+        case 5: u[k] = en * zn * xi[k] + xn * zn * eta[k] + xn * en * zeta[k]; v[k] = eta[k]; break;
+        default: return 1;
+        }
+    }
+    if (frame == 3) {
+        /* constant v: picked by the real geometry only */
+        if (creal(u[2]) / creal(xn) <= 0.866 && fabs(creal(u[1])) > EPS_RP) v[0] = 1.0;
+        else v[1] = 1.0;
+    }
+    double un = sqrt(creal(u[0]) * creal(u[0]) + creal(u[1]) * creal(u[1]) + creal(u[2]) * creal(u[2]));
+    double vn = sqrt(creal(v[0]) * creal(v[0]) + creal(v[1]) * creal(v[1]) + creal(v[2]) * creal(v[2]));
+    if (un < EPS_RP || vn < EPS_RP) return 1;
+    double complex uu = csqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]), ez[3], ex[3], ey[3];
+    for (int k = 0; k < 3; k++) ez[k] = u[k] / uu;
+    double complex vez = v[0] * ez[0] + v[1] * ez[1] + v[2] * ez[2];
+    for (int k = 0; k < 3; k++) ex[k] = v[k] - vez * ez[k];
+    double complex exn = csqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
+    for (int k = 0; k < 3; k++) ex[k] /= exn;
+    ey[0] = ez[1] * ex[2] - ez[2] * ex[1];
+    ey[1] = ez[2] * ex[0] - ez[0] * ex[2];
+    ey[2] = ez[0] * ex[1] - ez[1] * ex[0];
+    for (int k = 0; k < 3; k++) { r[k][0] = ex[k]; r[k][1] = ey[k]; r[k][2] = ez[k]; }
+    return 0;
+}
+
+/* rotation_geomgrad (rotate_multipoles.F90:3-230): gradient of the energy of
+   the rotated multipoles in a field E[nmm][3] and field gradient
+   Egrd[nmm][6] through the frame atoms of each site, ADDED to grad[nmm][3]:
+   for every site j and every coordinate t of j and of its z, x, y frame
+   atoms,  dE/dt = -(dR/dt mu0) . E_j + sum_ab dQ_ab Egrd_j,ab  with
+   dQ = dR Q0 R^T + R Q0 dR^T.  q0 [nmm][10] unrotated (quadrupoles /3),
+   frames [nmm][4] = (convention, z, x, y atom; 0-based, -1 = none).
+   The reference's z-bisector branch (:575-587) stores the diagonal of
+   d v / d r_y into the u derivative; this restatement differentiates the
+   frame exactly, so the two differ for that convention only.  Returns 1 on
+   an ill-defined frame. */
+int oracle_rotation_geomgrad(int nmm, const double *cmm, const double *q0, const int *frames,
+                             const double *E, const double *Egrd, double *grad)
+{
+    const double hstep = 1e-30;
+    for (int j = 0; j < nmm; j++) {
+        const int fr = frames[4 * j];
+        if (fr == 0) continue;
+        int at[4] = {j, frames[4 * j + 2], frames[4 * j + 3], frames[4 * j + 1]}; /* c, x, y, z */
+        for (int r = 1; r < 4; r++) if (at[r] < 0) at[r] = j;
+        const double *mu0 = q0 + 10 * j + 1, *Qs = q0 + 10 * j + 4;
+        const double Q0[3][3] = {{Qs[0], Qs[1], Qs[3]}, {Qs[1], Qs[2], Qs[4]}, {Qs[3], Qs[4], Qs[5]}};
+        const double *g6 = Egrd + 6 * j;
+        const double G[3][3] = {{g6[0], g6[1], g6[3]}, {g6[1], g6[2], g6[4]}, {g6[3], g6[4], g6[5]}};
+        for (int role = 0; role < 4; role++)
+            for (int t = 0; t < 3; t++) {
+                double complex p[4][3], R[3][3];
+                for (int a = 0; a < 4; a++)
+                    for (int k = 0; k < 3; k++) p[a][k] = cmm[3 * at[a] + k];
+                p[role][t] += I * hstep;
+                if (rot_cplx(fr, p[0], p[1], p[2], p[3], R)) return 1;
+                double R0[3][3], dR[3][3];
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) { R0[a][b] = creal(R[a][b]); dR[a][b] = cimag(R[a][b]) / hstep; }
+                double dmu[3], RQ[3][3], dRQ[3][3], acc = 0.0;
+                for (int a = 0; a < 3; a++) {
+                    dmu[a] = dR[a][0] * mu0[0] + dR[a][1] * mu0[1] + dR[a][2] * mu0[2];
+                    for (int b = 0; b < 3; b++) {
+                        RQ[a][b] = R0[a][0] * Q0[0][b] + R0[a][1] * Q0[1][b] + R0[a][2] * Q0[2][b];
+                        dRQ[a][b] = dR[a][0] * Q0[0][b] + dR[a][1] * Q0[1][b] + dR[a][2] * Q0[2][b];
+                    }
+                    acc -= dmu[a] * E[3 * j + a];
+                }
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) {
+                        double dq = 0.0;
+                        for (int c = 0; c < 3; c++) dq += dRQ[a][c] * R0[b][c] + RQ[a][c] * dR[b][c];
+                        acc += dq * G[a][b];
+                    }
+                grad[3 * at[role] + t] += acc;
+            }
+    }
+    return 0;
 }
 
 int oracle_num_threads(void)

@@ -1,7 +1,8 @@
 """Copy the reference's own hot-path test fixtures into tests/golden/reference/.
 
 These are data files the reference's ctest suite holds (inputs, external
-fields, golden induced dipoles and energies, prepared-state summaries), stored
+fields, golden induced dipoles and energies, prepared-state summaries, and the
+FULL_POTENTIAL.ref energy/gradient listings of the AMOEBA inputs), stored
 gzipped.  Run once in a container that has the reference tree mounted:
 
     python tests/golden/collect_reference_fixtures.py /root/reference
@@ -21,12 +22,14 @@ FILES = {
         "IPD_0_WANG_AL.ref", "IPD_1_WANG_AL.ref", "IPD_0_WANG_AL_CUT.ref", "IPD_1_WANG_AL_CUT.ref",
         "ENE_0_AMOEBA.ref", "ENE_1_AMOEBA.ref", "ENE_0_WANG_AL.ref", "ENE_1_WANG_AL.ref",
         "summary_AMOEBA.ref", "summary_WANG_AL.ref",
+        "FULL_POTENTIAL.ref",
     ],
     "1crn": [
         "input_AMOEBA.mmp", "input_WANG_AL.mmp", "EF_1.txt",
         "IPD_0_AMOEBA.ref", "IPD_1_AMOEBA.ref", "IPD_0_WANG_AL.ref", "IPD_1_WANG_AL.ref",
         "ENE_0_AMOEBA.ref", "ENE_1_AMOEBA.ref", "ENE_0_WANG_AL.ref", "ENE_1_WANG_AL.ref",
         "summary_AMOEBA.ref", "summary_WANG_AL.ref",
+        "FULL_POTENTIAL.ref",
     ],
     "1ubq": [
         "input_AMOEBA.mmp", "input_WANG_AL.mmp", "EF_1.txt",

@@ -11,6 +11,7 @@ import os
 import numpy as np
 
 ANG2AU = 1.8897261245650  # config/openmmpol_const.h.in:34
+AU2KCALMOL = 627.5096080306  # config/openmmpol_const.h.in:30
 EPS_RP = np.finfo(np.float64).eps * 100.0  # src/mod_constants.F90:61
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -218,6 +219,23 @@ def read_ene(path):
                 key = line.split()[0]
                 out[key] = float(line.split(":")[-1])
     return out
+
+
+def read_grad(path, term="EP"):
+    """FULL_POTENTIAL.ref: the 'Grad <term>' block as an N x 3 array, in
+    kcal/mol/angstrom (the energies above it are in Hartree)."""
+    rows = None
+    with open_text(path) as f:
+        for line in f:
+            if rows is None:
+                t = line.split()
+                if len(t) >= 2 and t[0] == "Grad" and t[1] == term:
+                    rows = []
+            elif line.strip():
+                rows.append([float(v) for v in line.split()])
+            else:
+                break
+    return None if rows is None else np.array(rows)
 
 
 def read_ef(path):

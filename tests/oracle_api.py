@@ -61,6 +61,9 @@ def lib():
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_set_small_step_eps.argtypes = [ctypes.c_double]
         L.oracle_x2e.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.oracle_geomgrad.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.oracle_rotation_geomgrad.restype = ctypes.c_int
+        L.oracle_rotation_geomgrad.argtypes = [ctypes.c_int, _dp, _dp, _ip, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -225,9 +228,53 @@ class OracleSystem:
         self.check()
         return V, E
 
+    def geomgrad_parts(self, ipd=None):
+        """Translational part of polelec_geomgrad [nmm, 3] plus the field E_D2M
+        [nmm, 3] and field gradient Egrd_D2M [nmm, 6] the rotation term needs."""
+        ip = np.ascontiguousarray(self.ipd if ipd is None else ipd, dtype=np.float64)
+        g = np.zeros((self.nmm, 3))
+        E = np.zeros((self.nmm, 3))
+        G = np.zeros((self.nmm, 6))
+        lib().oracle_geomgrad(self.h, _ptr(ip, _dp), _ptr(g, _dp), _ptr(E, _dp), _ptr(G, _dp))
+        self.check()
+        return g, E, G
+
+    def polelec_geomgrad(self, ipd=None):
+        """polelec_geomgrad (mod_geomgrad.F90:113-273): dE_pol/dr [nmm, 3] in
+        Hartree/bohr, rotation term included when the prep carries frames."""
+        g, E, G = self.geomgrad_parts(ipd)
+        if self.amoeba and "frames" in self.prep:
+            g += rotation_geomgrad(self.prep["cmm"], self.prep["q0"], self.prep["frames"], E, G)
+        return g
+
     def charges_field(self, cq, qq):
         cq = np.ascontiguousarray(cq, dtype=np.float64)
         qq = np.ascontiguousarray(qq, dtype=np.float64)
         E = np.zeros((self.npol, 3))
         lib().oracle_charges_field(self.h, len(qq), _ptr(cq, _dp), _ptr(qq, _dp), _ptr(E, _dp))
         return E
+
+
+def frames_0based(frames):
+    """.mmp frame rows (convention, z, x, y; 1-based, 0 = none) -> 0-based, -1 = none."""
+    f = np.asarray(frames, dtype=np.int64).reshape(-1, 4)
+    out = np.empty_like(f, dtype=np.int32)
+    out[:, 0] = f[:, 0]
+    out[:, 1:] = f[:, 1:] - 1
+    return np.ascontiguousarray(out)
+
+
+def rotation_geomgrad(cmm, q0, frames, E, Egrd):
+    """rotation_geomgrad (rotate_multipoles.F90:3-230): [nmm, 3] gradient of the
+    rotated multipoles' energy in the field E [nmm, 3] and gradient Egrd [nmm, 6]."""
+    cmm = np.ascontiguousarray(cmm, dtype=np.float64)
+    q0 = np.ascontiguousarray(q0, dtype=np.float64)
+    fr = frames_0based(frames)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    Egrd = np.ascontiguousarray(Egrd, dtype=np.float64)
+    g = np.zeros((cmm.shape[0], 3))
+    rc = lib().oracle_rotation_geomgrad(cmm.shape[0], _ptr(cmm, _dp), _ptr(q0, _dp), _ptr(fr, _ip),
+                                        _ptr(E, _dp), _ptr(Egrd, _dp), _ptr(g, _dp))
+    if rc:
+        raise RuntimeError("Ill defined rotation axis (either u or v vectors have 0 norm.")
+    return g

@@ -1,0 +1,153 @@
+"""Pins of the oracle's polarization-energy gradient (polelec_geomgrad,
+mod_geomgrad.F90:113-273, with rotation_geomgrad, rotate_multipoles.F90:3-230).
+
+- central finite differences of the oracle's own converged polarization energy
+  (multipoles re-rotated at every displaced geometry), N-methylacetamide with
+  AMOEBA and Wang-AL inputs and a synthetic cluster that uses every frame
+  convention 0-5;
+- the 'Grad EP' listings of the reference's FULL_POTENTIAL.ref fixtures
+  (N-methylacetamide and 1CRN, AMOEBA; kcal/mol/angstrom, converted with the
+  reference's constants); those listings come from a separately converged
+  dipole set, so they agree to ~1e-6 relative, which still fails any dropped
+  term, wrong sign or wrong factor by orders of magnitude;
+- the sum rule: with traceless quadrupoles the gradient sums to zero over the
+  atoms (the energy does not change under a rigid translation).
+"""
+import numpy as np
+import pytest
+
+import mmp_io
+import oracle_api as oa
+
+CONV = mmp_io.AU2KCALMOL * mmp_io.ANG2AU  # Hartree/bohr -> kcal/mol/angstrom
+
+
+def _solve(prep, cmm, tol=1e-13):
+    p = dict(prep)
+    p["cmm"] = np.asarray(cmm, dtype=float)
+    if p["amoeba"] and "frames" in p:
+        p["q"] = mmp_io.rotate_multipoles(p["cmm"], p["q0"], p["frames"])
+    o = oa.OracleSystem(p)
+    oa.lib().oracle_set_small_step_eps(0.0)
+    try:
+        ipd = o.polarization(o.m2d(), use_guess=False, tol=tol, maxit=2000)
+    finally:
+        oa.lib().oracle_set_small_step_eps(mmp_io.EPS_RP)
+    return o, ipd, o.energy(ipd)
+
+
+def _fd(prep, atoms, h=1e-4):
+    fd = np.zeros((len(atoms), 3))
+    for n, a in enumerate(atoms):
+        for c in range(3):
+            cp = prep["cmm"].copy()
+            cm = prep["cmm"].copy()
+            cp[a, c] += h
+            cm[a, c] -= h
+            fd[n, c] = (_solve(prep, cp)[2] - _solve(prep, cm)[2]) / (2 * h)
+    return fd
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_nma_gradient_matches_finite_differences(ff):
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path("N-methylacetamide", f"input_{ff}.mmp"))
+    o, ipd, _ = _solve(prep, prep["cmm"])
+    g = o.polelec_geomgrad(ipd)
+    fd = _fd(prep, range(prep["nmm"]))
+    assert np.abs(g).max() > 1e-4
+    assert np.abs(g - fd).max() < 1e-9 * max(1.0, np.abs(g).max() / 1e-3)
+
+
+@pytest.mark.parametrize("name,tol", [("N-methylacetamide", 2e-6), ("1crn", 5e-6)])
+def test_gradient_matches_reference_listing(name, tol):
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path(name, "input_AMOEBA.mmp"))
+    o, ipd, ep = _solve(prep, prep["cmm"], tol=1e-12)
+    full = mmp_io.fixture_path(name, "FULL_POTENTIAL.ref")
+    assert abs(ep - mmp_io.read_ene(full)["EP"]) <= 1e-6 * abs(ep)
+    want = mmp_io.read_grad(full, "EP")
+    got = o.polelec_geomgrad(ipd) * CONV
+    assert want.shape == got.shape
+    assert np.abs(got - want).max() <= tol * np.abs(want).max()
+
+
+def _frames_cluster(seed=7):
+    """14 sites in three groups, bonded as a chain, every frame convention."""
+    rng = np.random.default_rng(seed)
+    n = 14
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(-4.5, 4.5, 3)
+        if all(np.linalg.norm(p - x) > 2.0 for x in pts):
+            pts.append(p)
+    cmm = np.array(pts)
+    frames = np.zeros((n, 4), dtype=np.int64)  # convention, z, x, y (1-based, 0 = none)
+    for j in range(n):
+        conv = j % 6
+        z, x, y = (j + 1) % n + 1, (j + 2) % n + 1, (j + 3) % n + 1
+        if conv == 2:
+            x = (j - 1) % n + 1
+        frames[j] = [conv, z if conv else 0, x if conv in (1, 2, 4, 5) else 0, y if conv in (4, 5) else 0]
+    q0 = np.zeros((n, 10))
+    q0[:, 0] = rng.normal(0.0, 0.3, n)
+    q0[:, 1:4] = rng.normal(0.0, 0.3, (n, 3))
+    Q = rng.normal(0.0, 0.3, (n, 3, 3))
+    Q = 0.5 * (Q + Q.transpose(0, 2, 1))
+    Q -= np.einsum("nii->n", Q)[:, None, None] * np.eye(3) / 3.0
+    q0[:, 4], q0[:, 5], q0[:, 6] = Q[:, 0, 0], Q[:, 0, 1], Q[:, 1, 1]
+    q0[:, 7], q0[:, 8], q0[:, 9] = Q[:, 0, 2], Q[:, 1, 2], Q[:, 2, 2]
+    pol = rng.uniform(5.0, 12.0, n)
+    pol[[3, 10]] = 0.0
+    bonds = [(i, i + 1) for i in range(n - 1)]
+    ri = np.zeros(n + 1, dtype=np.int32)
+    adj = [[] for _ in range(n)]
+    for a, b in bonds:
+        adj[a].append(b)
+        adj[b].append(a)
+    for i in range(n):
+        ri[i + 1] = ri[i] + len(adj[i])
+    ci = np.array([v for a in adj for v in sorted(a)], dtype=np.int32)
+    grp = np.array([0] * 4 + [1] * 5 + [2] * 5, dtype=np.int32)
+    prep = dict(nmm=n, amoeba=True, cmm=cmm, q0=q0, frames=frames, pol=pol, adj_ri=ri, adj_ci=ci,
+                grp=grp, td=np.full(n, 0.39), scales=mmp_io.SCALES_AMOEBA.copy(), thole_scale=0.0)
+    prep["q"] = mmp_io.rotate_multipoles(cmm, q0, frames)
+    return prep
+
+
+def test_every_frame_convention_matches_finite_differences():
+    """Conventions 3-5 appear in no reference fixture; this pins the rotation
+    term for them (and 0-2) against the energy itself.  The z-bisector (4)
+    derivative is exact here; the reference's hand-written chain for it
+    (rotate_multipoles.F90:575-587) stores the |eta| I part of d v / d r_y as
+    d u / d r_y, so its analytic gradient would not pass this check."""
+    prep = _frames_cluster()
+    o, ipd, _ = _solve(prep, prep["cmm"])
+    gt, E, G = o.geomgrad_parts(ipd)
+    grot = oa.rotation_geomgrad(prep["cmm"], prep["q0"], prep["frames"], E, G)
+    assert np.abs(grot).max() > 1e-3 * np.abs(gt).max()
+    g = gt + grot
+    fd = _fd(prep, range(prep["nmm"]))
+    assert np.abs(g - fd).max() < 2e-9 * max(1.0, np.abs(g).max() / 1e-3)
+    # traceless quadrupoles: rigid translation leaves the energy unchanged
+    assert np.abs(g.sum(axis=0)).max() < 1e-13 * np.abs(g).max() * len(g)
+
+
+def test_rigid_translation_sum_rule_1crn():
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path("1crn", "input_AMOEBA.mmp"))
+    q0 = prep["q0"].copy()
+    tr = (q0[:, 4] + q0[:, 6] + q0[:, 9]) / 3.0
+    for c in (4, 6, 9):
+        q0[:, c] -= tr
+    prep["q0"] = q0
+    prep["q"] = mmp_io.rotate_multipoles(prep["cmm"], q0, prep["frames"])
+    o, ipd, _ = _solve(prep, prep["cmm"], tol=1e-10)
+    g = o.polelec_geomgrad(ipd)
+    assert np.abs(g.sum(axis=0)).max() < 1e-12 * np.abs(g).max() * len(g)
+
+
+def test_rotation_term_of_identity_frames_is_zero():
+    prep = _frames_cluster()
+    fr = prep["frames"].copy()
+    fr[:, 0] = 0
+    rng = np.random.default_rng(1)
+    g = oa.rotation_geomgrad(prep["cmm"], prep["q0"], fr, rng.normal(size=(14, 3)), rng.normal(size=(14, 6)))
+    assert not g.any()
