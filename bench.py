@@ -36,6 +36,14 @@ def pair_flops(n, m):
     return n * (n - 1) * (29 + 18 * m)
 
 
+def grad_flops(npol, nmm):
+    """FP64 FLOP per gradient (FMA = 2, exponentials of damped pairs not
+    counted): the static-multipole sweep 135 per (polarizable target, MM source)
+    pair, the dipole-dipole sweep 113 per ordered polarizable pair, the MM-target
+    sweep 166 per (MM target, polarizable source) pair."""
+    return 135 * npol * nmm + 113 * npol * npol + 166 * nmm * npol
+
+
 def incore_bytes(n, m, s):
     """Packed symmetric T streamed once per matvec plus the vectors in and out:
     [N(N-1)/2 * 6 + N] * s + 2 * 3N * m * s bytes."""
@@ -369,6 +377,50 @@ def run_large(tol, bufs):
     return res
 
 
+def run_grad(spec, tol, repeat, cpu):
+    """d E_pol / d r of every site (ommp_polelec_geomgrad) after a converged
+    FP64 solve: wall time per call (copy-back of the 3N gradient included) and
+    live event timing of the three FP64 pair sweeps.  With cpu, the oracle's
+    gradient of the same dipoles is timed once on the host cores (the full
+    15,000 sites, about 10 s of CPU work) and compared with the GPU's."""
+    from openmmpol_amd import synth
+    s = synth.make_system(spec)
+    s.set_precision("fp64")
+    s.set_solver_tolerance(tol, 500)
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    ipd = s.ipd.copy()
+    g = s.polelec_geomgrad()  # first call builds the gradient's lists and buffers
+    s.profile_kernels(True)
+    s.kernel_stats(reset=True)
+    t = time.perf_counter()
+    for _ in range(repeat):
+        g = s.polelec_geomgrad()
+    wall = (time.perf_counter() - t) / repeat
+    calls, ms = s.kernel_stats(reset=True)["geomgrad"]
+    s.profile_kernels(False)
+    npol, nmm = s.pol_atoms, s.mm_atoms
+    s.terminate()
+    sweep_ms = ms / max(calls, 1)
+    flops = grad_flops(npol, nmm)
+    res = {"workload": f"{nmm}-site AMOEBA sphere, gradient of E_pol after a converged FP64 solve",
+           "ms_per_gradient": wall * 1e3, "gradients_per_s": 1.0 / wall,
+           "pair_sweeps_avg_ms": sweep_ms, "launches": calls, "flop_per_gradient": flops,
+           "achieved_tflops": flops / (sweep_ms * 1e-3) / 1e12 if calls else None,
+           "peak_tflops": PEAK_FP64_TFLOPS, "bound": "valu (FP64)",
+           "sum_rule_rel": float(np.abs(g.sum(axis=0)).max() / np.abs(g).max())}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_api as oa
+        o = oa.OracleSystem(synth.oracle_prep(spec))
+        t = time.perf_counter()
+        go = o.polelec_geomgrad(ipd)
+        t_cpu = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "gradients/s", "cores": oa.lib().oracle_num_threads(),
+                               "kind": "port", "sample": f"one oracle gradient of the same {nmm} sites and dipoles"}
+        res["max_abs_diff_vs_oracle_rel"] = float(np.abs(g - go).max() / np.abs(go).max())
+    return res
+
+
 def cpu_baseline(spec, fp64_iters):
     """Oracle (FP64 C/OpenMP restatement of the reference) on the host cores:
     static field once + 8 T matvecs of the same system, extrapolated to one
@@ -411,7 +463,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=16)
     ap.add_argument("--serving-gap-ms", type=float, default=10.0)
     ap.add_argument("--serving-slots", type=int, default=16)
-    ap.add_argument("--legs", default="headline,fp64,multi,serving,incore,large,cpu",
+    ap.add_argument("--legs", default="headline,fp64,multi,serving,incore,large,grad,cpu",
                     help="comma-separated subset to run (profiling one leg); the default runs all")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -461,6 +513,10 @@ def main():
         bufs.free()
         print(f"[bench] 60k sites: mixed {large['mixed_otf_solves_per_s']:.2f} solves/s", file=sys.stderr,
               flush=True)
+    grad = None
+    if "grad" in legs:
+        grad = run_grad(spec, args.tol, 10, "cpu" in legs)
+        print(f"[bench] gradient: {grad['ms_per_gradient']:.2f} ms per call", file=sys.stderr, flush=True)
     incore = None
     if "incore" in legs:
         incore = run_incore(spec, 20)
@@ -477,7 +533,7 @@ def main():
     if mixed is None:
         # a subset of the legs (profiling): no headline figure
         out = {"legs": sorted(legs)}
-        for key, val in (("multi_rhs", multi), ("serving", serving), ("large_60k", large)):
+        for key, val in (("multi_rhs", multi), ("serving", serving), ("large_60k", large), ("geomgrad", grad)):
             if val is not None:
                 out[key] = val
         print(json.dumps(out))
@@ -524,6 +580,8 @@ def main():
         out["large_60k"] = large
     if serving is not None:
         out["serving"] = serving
+    if grad is not None:
+        out["geomgrad"] = grad
     if incore is not None:
         b = incore_bytes(n, 2, 4)
         out["incore_fp32_matvec"] = {

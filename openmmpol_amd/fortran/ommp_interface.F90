@@ -144,10 +144,22 @@ module ommp_interface
             real(c_double), intent(in) :: cext(*)
             real(c_double), intent(inout) :: e(*)
         end subroutine
+        subroutine c_ommp_polelec_geomgrad(s, grd) bind(c, name='ommp_polelec_geomgrad')
+            import :: c_ptr, c_double
+            type(c_ptr), value :: s
+            real(c_double), intent(out) :: grd(*)
+        end subroutine
+        subroutine c_ommp_rotation_geomgrad(s, e, egrd, grd) bind(c, name='ommp_rotation_geomgrad')
+            import :: c_ptr, c_double
+            type(c_ptr), value :: s
+            real(c_double), intent(in) :: e(*), egrd(*)
+            real(c_double), intent(out) :: grd(*)
+        end subroutine
     end interface
 
     public :: ommp_potential_mmpol2ext, ommp_potential_mm2ext, ommp_potential_pol2ext
     public :: ommp_field_mmpol2ext, ommp_field_mm2ext, ommp_field_pol2ext
+    public :: ommp_polelec_geomgrad, ommp_rotation_geomgrad
     public :: ommp_init_mmp, ommp_terminate, ommp_set_verbose
     public :: ommp_set_default_solver, ommp_set_default_matv
     public :: ommp_set_external_field, ommp_set_external_field_nomm
@@ -311,6 +323,23 @@ contains
         real(ommp_real), intent(in) :: cext(3, n)
         real(ommp_real), intent(inout) :: e(3, n)
         call c_ommp_field_pol2ext(s%h, n, cext, e)
+    end subroutine
+
+    ! mod_interface.F90:671-683: grd(3, mm_atoms) = 0, then polelec_geomgrad
+    ! (Hartree/bohr)
+    subroutine ommp_polelec_geomgrad(s, grd)
+        type(ommp_system), intent(in) :: s
+        real(ommp_real), intent(out), contiguous :: grd(:, :)
+        call c_ommp_polelec_geomgrad(s%h, grd)
+    end subroutine
+
+    ! mod_interface.F90:699-710: grd = 0, then rotation_geomgrad for the field
+    ! e(3, mm_atoms) and field gradient egrd(6, mm_atoms)
+    subroutine ommp_rotation_geomgrad(s, e, egrd, grd)
+        type(ommp_system), intent(in) :: s
+        real(ommp_real), intent(in), contiguous :: e(:, :), egrd(:, :)
+        real(ommp_real), intent(out), contiguous :: grd(:, :)
+        call c_ommp_rotation_geomgrad(s%h, e, egrd, grd)
     end subroutine
 
 end module ommp_interface

@@ -2,14 +2,15 @@
 ! external-field file (one row x y z per MM atom, or 'none'), solves with the
 ! default solver, prints the polarization energy and the dipoles, then the
 ! potential and field of multipoles plus dipoles at two fixed external points
-! (each evaluated twice into the same array, which must add up).
+! (each evaluated twice into the same array, which must add up), then the
+! gradient of the polarization energy (one row per MM atom).
 program test_fortran_api
     use ommp_interface
     use iso_c_binding
     implicit none
     type(ommp_system), pointer :: s
     character(len=4096) :: mmp, eff
-    real(ommp_real), allocatable :: ef(:,:), efmm(:,:)
+    real(ommp_real), allocatable :: ef(:,:), efmm(:,:), grd(:,:)
     real(ommp_real), pointer :: ipd(:,:,:)
     integer(ommp_integer), pointer :: pmm(:)
     integer :: i, k, npol, nmm, u
@@ -54,6 +55,12 @@ program test_fortran_api
     call ommp_field_pol2ext(s, 2, cext, e)
     do i = 1, 2
         write(*, '(A, 4ES24.15)') 'X2E ', v(i), e(:, i)
+    end do
+    allocate(grd(3, nmm))
+    grd = 1.0_ommp_real
+    call ommp_polelec_geomgrad(s, grd)
+    do i = 1, nmm
+        write(*, '(A, 3ES24.15)') 'GRD ', grd(:, i)
     end do
     call ommp_terminate(s)
 end program

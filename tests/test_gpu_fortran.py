@@ -1,7 +1,8 @@
 """The Fortran ISO_C_BINDING module drives the GPU solve: a Fortran program
 (tests/fortran/test_fortran_api.F90, built by __graft_entry__.build) calls
-ommp_init_mmp / ommp_set_external_field / ommp_get_polelec_energy and reads the
-dipoles through the module; results must equal the oracle's."""
+ommp_init_mmp / ommp_set_external_field / ommp_get_polelec_energy, reads the
+dipoles through the module, evaluates external-point potentials and fields and
+the polarization-energy gradient; results must equal the oracle's."""
 import gzip
 import os
 import shutil
@@ -48,3 +49,8 @@ def test_fortran_module_solve(name, ff, k, fixture_file, tmp_path):
     _, ep_ = o.x2e(cext, 2)
     assert np.allclose(x2e[:, 0], vmp + vm, rtol=1e-12, atol=1e-15)
     assert np.allclose(x2e[:, 1:], emp + ep_, rtol=1e-10, atol=1e-14)
+    # gradient after the solve; the oracle's own dipoles differ by < 1e-9 a.u.
+    grd = np.array([[float(v) for v in l.split()[1:]] for l in lines if l.startswith("GRD")])
+    go = o.polelec_geomgrad()
+    assert grd.shape == go.shape
+    assert np.abs(grd - go).max() <= 1e-6 * np.abs(go).max()
