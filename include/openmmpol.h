@@ -107,11 +107,22 @@ int32_t *ommp_get_polar_mm(OMMP_SYSTEM_PRT s);
    static field, the dipoles and the warm-start guess, re-rotates multipoles. */
 void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c);
 
-/* :153-155.  This library computes every interaction directly (no fast
-   multipoles): enable only records the request with a warning, use returns false. */
+/* :153-160; mod_c_interface.F90:1781-1880.  Fast multipoles for the static
+   field and the T matvec of the solvers (CG, DIIS; inversion stays dense): a
+   uniform octree whose leaves are at least max(min cell size, near-field
+   distance) wide (defaults 7 and 5 Angstrom, given in bohr), pairs in the 27
+   leaves around a site computed directly with damping and screening, farther
+   pairs through complex solid-harmonic expansions of order lmax (static) and
+   lmax_pol (dipoles), default 8, at most 20.  Any change drops the static field
+   and the dipoles.  With fast multipoles the solve runs in FP64 whatever
+   ommp_amd_set_precision says; the gradient and the batched solvers stay direct. */
 void ommp_enable_fmm(OMMP_SYSTEM_PRT s);
 void ommp_disable_fmm(OMMP_SYSTEM_PRT s);
 bool ommp_use_fmm(OMMP_SYSTEM_PRT s);
+void ommp_set_fmm_lmax_pol(OMMP_SYSTEM_PRT s, int32_t l);
+void ommp_set_fmm_lmax(OMMP_SYSTEM_PRT s, int32_t l);
+void ommp_set_fmm_distance(OMMP_SYSTEM_PRT s, double d);
+void ommp_set_fmm_min_cell_size(OMMP_SYSTEM_PRT s, double d);
 
 #ifdef __cplusplus
 }

@@ -45,6 +45,10 @@ SIGNATURES = [
     ("ommp_enable_fmm", None, [c_vp]),
     ("ommp_disable_fmm", None, [c_vp]),
     ("ommp_use_fmm", ctypes.c_bool, [c_vp]),
+    ("ommp_set_fmm_lmax_pol", None, [c_vp, ctypes.c_int32]),
+    ("ommp_set_fmm_lmax", None, [c_vp, ctypes.c_int32]),
+    ("ommp_set_fmm_distance", None, [c_vp, ctypes.c_double]),
+    ("ommp_set_fmm_min_cell_size", None, [c_vp, ctypes.c_double]),
     ("ommp_amd_init_arrays", c_vp, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_dp, c_dp, c_dp, c_dp,
                                     c_ip, c_ip, c_ip, c_ip]),
     ("ommp_amd_set_screening_parameters", None, [c_vp, c_dp, c_dp, c_dp, c_dp, c_dp]),

@@ -88,16 +88,44 @@ void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c) {
     ommp::invalidate_geometry(p);
 }
 
+// mod_c_interface.F90:1781-1880
 void ommp_enable_fmm(OMMP_SYSTEM_PRT s) {
     sys(s)->fmm_requested = true;
-    ommp::message("Fast multipoles are not available in this library: every interaction is computed directly",
-                  1);
+    ommp::fmm_settings_changed(*sys(s));
 }
 
-void ommp_disable_fmm(OMMP_SYSTEM_PRT s) { sys(s)->fmm_requested = false; }
-bool ommp_use_fmm(OMMP_SYSTEM_PRT s) {
-    (void)sys(s);
-    return false;
+void ommp_disable_fmm(OMMP_SYSTEM_PRT s) {
+    sys(s)->fmm_requested = false;
+    ommp::fmm_settings_changed(*sys(s));
+}
+
+bool ommp_use_fmm(OMMP_SYSTEM_PRT s) { return sys(s)->fmm_requested; }
+
+static int fmm_order(int32_t l) {
+    if (l < 0 || l > 20) ommp::fatal("FMM expansion order must be between 0 and 20");
+    return l;
+}
+
+void ommp_set_fmm_lmax_pol(OMMP_SYSTEM_PRT s, int32_t l) {
+    sys(s)->fmm_lmax_pol = fmm_order(l);
+    ommp::fmm_settings_changed(*sys(s));
+}
+
+void ommp_set_fmm_lmax(OMMP_SYSTEM_PRT s, int32_t l) {
+    sys(s)->fmm_lmax_static = fmm_order(l);
+    ommp::fmm_settings_changed(*sys(s));
+}
+
+void ommp_set_fmm_distance(OMMP_SYSTEM_PRT s, double d) {
+    if (!(d >= 0.0)) ommp::fatal("FMM near-field distance must be non-negative");
+    sys(s)->fmm_distance = d;
+    ommp::fmm_settings_changed(*sys(s));
+}
+
+void ommp_set_fmm_min_cell_size(OMMP_SYSTEM_PRT s, double d) {
+    if (!(d > 0.0)) ommp::fatal("FMM minimum cell size must be positive");
+    sys(s)->fmm_min_cell = d;
+    ommp::fmm_settings_changed(*sys(s));
 }
 
 // ------------------------------------------------------------ extensions

@@ -65,6 +65,14 @@ void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, cons
                     const CsrDev &pp, const double *x, long xstride, double *y, long ystride,
                     bool dodiag, double *dotpart, void *part, int nsplit, const int *skip_flag,
                     hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
+// Finalize alone, for FP64 partial fields computed elsewhere (fast multipoles).
+void launch_tmv_finalize64(hipStream_t st, bool amoeba, int npol, int M, const double *part, int nsplit,
+                           const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
+                           const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
+                           const int *skip_flag);
+void launch_rhs_finalize(hipStream_t st, bool amoeba, int npol, int nsplit, const double *part, const double4 *ms64,
+                         const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
+                         const CsrDev &sp, double *e_m2d);
 int tmv_split(int npol);
 int balanced_split(int tb, int nsrc, int num_cu);
 size_t tmv_part_bytes(int npol, int M, int prec, int nsplit);

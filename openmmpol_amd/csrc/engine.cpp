@@ -15,6 +15,8 @@
 #include "batch.h"
 #include "batch_engine.h"
 #include "device.h"
+#include "fmm.h"
+#include "fmm_host.h"
 #include "ommp_internal.h"
 
 namespace ommp {
@@ -26,7 +28,7 @@ namespace ommp {
             fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
     } while (0)
 
-enum MvKind { kOtf64 = 0, kOtf32 = 1, kPack32 = 2, kPack64 = 3 };
+enum MvKind { kOtf64 = 0, kOtf32 = 1, kPack32 = 2, kPack64 = 3, kFmm = 4 };
 
 struct DeviceState {
     int device = 0;
@@ -98,6 +100,11 @@ struct DeviceState {
     double *g_emm = nullptr, *g_egrd = nullptr, *g_rotc = nullptr, *g_out = nullptr;
     int g_split_s = 1, g_split_p = 1, g_split_m = 1;
     bool grad_static_ready = false;
+    // fast multipoles (built on first use, rebuilt after a geometry change)
+    FmmDev fmm;
+    bool fmm_valid = false;
+    std::vector<void *> fmm_ptrs;
+    double *fmm_part = nullptr, *fmm_rhs_part = nullptr;
 };
 
 namespace {
@@ -363,9 +370,109 @@ DeviceState &ensure_device(System &s) {
     return d;
 }
 
+template <typename T> T *fmm_alloc(DeviceState &d, size_t count) {
+    T *p = dalloc<T>(d, count);
+    d.fmm_ptrs.push_back((void *)p);
+    return p;
+}
+
+void fmm_release(DeviceState &d) {
+    for (void *p : d.fmm_ptrs) dfree(d, p);
+    d.fmm_ptrs.clear();
+    d.fmm = FmmDev();
+    d.fmm_part = d.fmm_rhs_part = nullptr;
+    d.fmm_valid = false;
+}
+
+// Octree, sorted copies of the sites and atoms, tables and expansion storage
+// for the current coordinates.
+void ensure_fmm(System &s, DeviceState &d) {
+    if (d.fmm_valid) return;
+    fmm_release(d);
+    const FmmHost h = fmm_build(s, s.fmm_lmax_pol, s.fmm_lmax_static, s.fmm_min_cell, s.fmm_distance);
+    const int npol = s.npol, nmm = s.nmm, ld = s.ld_cart;
+    FmmDev &f = d.fmm;
+    f.npol = npol;
+    f.nmm = nmm;
+    f.levels = h.levels;
+    f.p_pol = h.p_pol;
+    f.p_static = h.p_static;
+    f.p_tab = h.p_tab;
+    f.ncoef_tab = h.ncoef_tab;
+    f.ncoef2_tab = h.ncoef2_tab;
+    f.leaf_h = h.leaf_h;
+    f.origin = make_double3(h.origin[0], h.origin[1], h.origin[2]);
+    for (int l = 0; l < (int)h.cell_off.size(); l++) f.cell_off[l] = h.cell_off[l];
+    const double inf = std::numeric_limits<double>::infinity();
+    auto site = [&](int i) {
+        const double t = s.thole[i];
+        return make_double4(s.cmm[3 * i], s.cmm[3 * i + 1], s.cmm[3 * i + 2], t > kEpsRp ? 1.0 / t : inf);
+    };
+    std::vector<double4> ppos(npol), mpos(nmm);
+    std::vector<double> pa(npol), ma(nmm), mq((size_t)nmm * ld);
+    std::vector<int> pself(npol);
+    for (int t = 0; t < npol; t++) {
+        const int i = s.polar_mm[h.pol_order[t]];
+        ppos[t] = site(i);
+        pa[t] = s.thole_damping_mm[i];
+        pself[t] = t;
+    }
+    for (int t = 0; t < nmm; t++) {
+        const int i = h.mm_order[t];
+        mpos[t] = site(i);
+        ma[t] = s.mm_polar[i] >= 0 ? s.thole_damping_mm[i] : inf;
+        for (int c = 0; c < ld; c++) mq[(size_t)t * ld + c] = s.q[(size_t)i * ld + c];
+    }
+    auto up_i = [&](const std::vector<int> &v) {
+        int *p = fmm_alloc<int>(d, std::max<size_t>(v.size(), 1));
+        upload(v.data(), p, sizeof(int) * v.size());
+        return p;
+    };
+    f.pol_order = up_i(h.pol_order);
+    f.pol_leaf = up_i(h.pol_leaf);
+    f.pol_self = up_i(pself);
+    f.pol_mmself = up_i(h.pol_mmself);
+    f.pol_leaf_start = up_i(h.pol_leaf_start);
+    f.mm_leaf_start = up_i(h.mm_leaf_start);
+    f.cnt_pol = up_i(h.cnt_pol);
+    f.cnt_mm = up_i(h.cnt_mm);
+    f.pol_pos = fmm_alloc<double4>(d, npol);
+    upload(ppos.data(), f.pol_pos, sizeof(double4) * npol);
+    f.pol_a = fmm_alloc<double>(d, npol);
+    upload(pa.data(), f.pol_a, sizeof(double) * npol);
+    f.mm_pos = fmm_alloc<double4>(d, nmm);
+    upload(mpos.data(), f.mm_pos, sizeof(double4) * nmm);
+    f.mm_a = fmm_alloc<double>(d, nmm);
+    upload(ma.data(), f.mm_a, sizeof(double) * nmm);
+    f.mm_q = fmm_alloc<double>(d, mq.size());
+    upload(mq.data(), f.mm_q, sizeof(double) * mq.size());
+    auto up_c = [&](const std::vector<double2> &v) {
+        double2 *p = fmm_alloc<double2>(d, std::max<size_t>(v.size(), 1));
+        upload(v.data(), p, sizeof(double2) * v.size());
+        return p;
+    };
+    f.tab_m2m = up_c(h.tab_m2m);
+    f.tab_l2l = up_c(h.tab_l2l);
+    f.tab_m2l = up_c(h.tab_m2l);
+    const size_t ncell = (size_t)h.cell_off.back();
+    f.mexp = fmm_alloc<double2>(d, ncell * f.ncoef_tab);
+    f.lexp = fmm_alloc<double2>(d, ncell * f.ncoef_tab);
+    f.xs = fmm_alloc<double>(d, 6L * npol);
+    d.fmm_part = fmm_alloc<double>(d, (size_t)(kFmmNearSplits + 1) * 2 * 3 * npol);
+    d.fmm_rhs_part = fmm_alloc<double>(d, (size_t)(kFmmNearSplits + 1) * 3 * npol);
+    d.fmm_valid = true;
+}
+
 void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double *y, bool dodiag,
             double *dotpart, const CgState *skip) {
     const long n = d.n;
+    if (kind == kFmm) {
+        ensure_fmm(s, d);
+        fmm_launch_matvec_fields(d.stream, d.fmm, s.amoeba, M, x, n, d.fmm_part);
+        launch_tmv_finalize64(d.stream, s.amoeba, s.npol, M, d.fmm_part, kFmmNearSplits + 1, d.ps64, d.pa64, d.alpha,
+                              d.pp, x, n, y, n, dodiag, dotpart, reinterpret_cast<const int *>(skip));
+        return;
+    }
     if (kind == kOtf64 || kind == kOtf32) {
         const int pi = prof_begin(d, kind);
         launch_tmv_otf(d.stream, kind == kOtf32 ? 32 : 64, s.amoeba, s.npol, M, d.ps64, d.pa64, d.ps32, d.pa32,
@@ -649,14 +756,26 @@ void invalidate_geometry(System &s) {
     if (s.dev) {
         s.dev->tpack_valid = false;
         s.dev->lu_valid = false;
+        s.dev->fmm_valid = false;
     }
+}
+
+void fmm_settings_changed(System &s) {
+    if (s.dev) s.dev->fmm_valid = false;
+    s.m2d_done = false;
+    s.ipd_done = false;
 }
 
 // prepare_polelec (mod_electrostatics.F90:1487-1596), cached until the geometry changes.
 void prepare_polelec(System &s) {
     if (s.m2d_done) return;
     DeviceState &d = ensure_device(s);
-    if (s.npol > 0) {
+    if (s.npol > 0 && s.fmm_requested) {
+        ensure_fmm(s, d);
+        fmm_launch_static_fields(d.stream, d.fmm, s.amoeba, s.ld_cart, d.fmm_rhs_part);
+        launch_rhs_finalize(d.stream, s.amoeba, s.npol, kFmmNearSplits + 1, d.fmm_rhs_part, d.ms64, d.ma64, d.mq,
+                            d.ps64, d.pa64, d.sp, d.e_m2d);
+    } else if (s.npol > 0) {
         const int pi = prof_begin(d, 4);
         launch_rhs_m2d(d.stream, s.amoeba, s.nmm, s.npol, d.ms64, d.ma64, d.mq, d.ps64, d.pa64, d.polar_mm, d.sp,
                        d.rhs_part, d.rhs_split, d.e_m2d, pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
@@ -690,8 +809,12 @@ void polarization(System &s, const double *d_field, int solver, int matv, int ma
         fatal("Unknown solver for calculation of the induced point dipoles");
     if (solver != kSolverInversion && matv != kMatvIncore && matv != kMatvDirect)
         fatal("Unknown matrix-vector method requested");
-    const bool mixed = s.precision == 1;
-    const int kind = matv == kMatvIncore ? (mixed ? kPack32 : kPack64) : (mixed ? kOtf32 : kOtf64);
+    // with fast multipoles every matvec is the FP64 multipole one (the mixed
+    // refinement is for the direct pair sweep)
+    const bool use_fmm = s.fmm_requested && solver != kSolverInversion;
+    const bool mixed = s.precision == 1 && !use_fmm;
+    const int kind = use_fmm ? kFmm
+                             : (matv == kMatvIncore ? (mixed ? kPack32 : kPack64) : (mixed ? kOtf32 : kOtf64));
     // ipd0 = 0, then the guess for the requested sets
     HCHECK(hipMemsetAsync(d.vx, 0, sizeof(double) * 2 * n, d.stream));
     const bool guess = s.ipd_use_guess && solver != kSolverInversion;
@@ -735,7 +858,7 @@ void polarization(System &s, const double *d_field, int solver, int matv, int ma
             if (k == 1 && mD && !s.ipd_use_guess) launch_copy(d.stream, n, d.vx, d.vx + n);
             int it = 0;
             // DIIS runs the FP64 matvec of the requested kind
-            const int dk = matv == kMatvIncore ? kPack64 : kOtf64;
+            const int dk = use_fmm ? kFmm : (matv == kMatvIncore ? kPack64 : kOtf64);
             int bad = diis_solve(s, d, dk, d.vb + k * n, d.vx + k * n, s.tol, s.max_iter, &it);
             s.last_iters[k] = it;
             if (bad) fatal("Iterative solver did not converged");
@@ -1116,7 +1239,7 @@ const double *device_ipd(System &s) {
 
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat) {
     if (ncols < 1 || ncols > 2) fatal("tmatvec supports 1 or 2 columns");
-    if (kind < 0 || kind > 3) fatal("Unknown matvec kind");
+    if (kind < 0 || kind > 4) fatal("Unknown matvec kind");
     if (s.npol == 0) return 0.0;
     DeviceState &d = ensure_device(s);
     const long n = d.n;

@@ -475,6 +475,43 @@ void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, cons
     }
 }
 
+// Finalize of externally computed FP64 partial fields (the fast multipole
+// path: near and far field as two splits).
+void launch_tmv_finalize64(hipStream_t st, bool amoeba, int npol, int M, const double *part, int nsplit,
+                           const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
+                           const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
+                           const int *skip_flag) {
+    if (npol == 0) return;
+    const CgState *skip = reinterpret_cast<const CgState *>(skip_flag);
+    const dim3 g(site_blocks(npol)), b(kBlock);
+    if (amoeba && M == 1)
+        hipLaunchKernelGGL((k_tmv_finalize<double, 1, true>), g, b, 0, st, npol, nsplit, part, ps64, pa64, alpha,
+                           pp.ri, pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
+    else if (amoeba)
+        hipLaunchKernelGGL((k_tmv_finalize<double, 2, true>), g, b, 0, st, npol, nsplit, part, ps64, pa64, alpha,
+                           pp.ri, pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
+    else if (M == 1)
+        hipLaunchKernelGGL((k_tmv_finalize<double, 1, false>), g, b, 0, st, npol, nsplit, part, ps64, pa64, alpha,
+                           pp.ri, pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
+    else
+        hipLaunchKernelGGL((k_tmv_finalize<double, 2, false>), g, b, 0, st, npol, nsplit, part, ps64, pa64, alpha,
+                           pp.ri, pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_rhs_finalize(hipStream_t st, bool amoeba, int npol, int nsplit, const double *part, const double4 *ms64,
+                         const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
+                         const CsrDev &sp, double *e_m2d) {
+    if (npol == 0) return;
+    if (amoeba)
+        hipLaunchKernelGGL(k_rhs_finalize<true>, dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol, nsplit, part,
+                           ms64, ma64, mq, ps64, pa64, sp.ri, sp.ci, sp.f2, e_m2d);
+    else
+        hipLaunchKernelGGL(k_rhs_finalize<false>, dim3(site_blocks(npol)), dim3(kBlock), 0, st, npol, nsplit, part,
+                           ms64, ma64, mq, ps64, pa64, sp.ri, sp.ci, sp.f2, e_m2d);
+    HIP_CHECK(hipGetLastError());
+}
+
 // ===================================================== packed in-core T
 // Upper tile pairs (I <= J) of 64x64 sites, 6 tensor planes (xx, xy, xz, yy,
 // yz, zz).  Thread t of the 256-thread block owns the 4x4 sub-block of rows

@@ -61,6 +61,9 @@ struct System {
     // ------------------------------------------------------------ API state
     int def_solver = kSolverCG, def_matv = kMatvDirect;
     bool fmm_requested = false;
+    // fast multipoles (ommp_set_fmm_*; defaults openmmpol_const.h.in:46-49)
+    int fmm_lmax_pol = 8, fmm_lmax_static = 8;
+    double fmm_distance = 5.0 * kAng2Au, fmm_min_cell = 7.0 * kAng2Au;
     bool m2d_done = false, ipd_done = false, ipd_use_guess = false;
     bool host_ipd_stale = false;
     std::vector<double> ipd;       // [n_ipd][npol][3] host mirror
@@ -121,6 +124,7 @@ void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *i
                  double *epol);
 void charges_field_host(System &s, int nq, const double *charges, double *out);
 void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V, double *E);
+void fmm_settings_changed(System &s);  // drops the tree, the static field and the dipoles
 void polelec_geomgrad_host(System &s, double *grd);
 void rotation_geomgrad_host(System &s, const double *E, const double *Egrd, double *grd);
 

@@ -233,6 +233,25 @@ class OMMPSystem:
             raise ValueError("ext_field should be shaped [mm_atoms, 3]")
         lib().ommp_update_coordinates(self.handler, c.ctypes.data_as(c_dp))
 
+    # fast multipoles (ommp_enable_fmm / ommp_set_fmm_*; lengths in bohr)
+    def enable_fmm(self, lmax=None, lmax_pol=None, distance=None, min_cell_size=None):
+        if lmax is not None:
+            lib().ommp_set_fmm_lmax(self.handler, int(lmax))
+        if lmax_pol is not None:
+            lib().ommp_set_fmm_lmax_pol(self.handler, int(lmax_pol))
+        if distance is not None:
+            lib().ommp_set_fmm_distance(self.handler, float(distance))
+        if min_cell_size is not None:
+            lib().ommp_set_fmm_min_cell_size(self.handler, float(min_cell_size))
+        lib().ommp_enable_fmm(self.handler)
+
+    def disable_fmm(self):
+        lib().ommp_disable_fmm(self.handler)
+
+    @property
+    def use_fmm(self):
+        return bool(lib().ommp_use_fmm(self.handler))
+
     def set_default_solver(self, solver):
         lib().ommp_set_default_solver(self.handler, available_solvers[solver])
 
@@ -296,7 +315,7 @@ class OMMPSystem:
 
     def tmatvec(self, x, kind="otf64", repeat=1):
         """y = T x for x [ncols, pol_atoms, 3]; returns (y, mean ms per launch)."""
-        kinds = {"otf64": 0, "otf32": 1, "incore32": 2, "incore64": 3}
+        kinds = {"otf64": 0, "otf32": 1, "incore32": 2, "incore64": 3, "fmm": 4}
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, self.pol_atoms, 3)
         y = np.zeros_like(x)
         ms = lib().ommp_amd_tmatvec(self.handler, kinds[kind], x.shape[0], x.ctypes.data_as(c_dp),

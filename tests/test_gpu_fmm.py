@@ -1,0 +1,91 @@
+"""Fast multipoles on the GPU (ommp_enable_fmm, ommp_set_fmm_*): the FMM
+T matvec and static field against the direct FP64 sweeps of the same system
+(themselves pinned to the oracle), the error falling with the expansion order;
+solves with FMM against the reference's own 1UBQ fixtures, which its test suite
+produces with fast multipoles of order 18 (tests/1ubq_amoeba_mmp.json:9-11) and
+compares at rtol 1e-6, atol 1e-5."""
+import numpy as np
+import pytest
+
+import mmp_io
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def ubq(fixture_file):
+    from openmmpol_amd import OMMPSystem
+    s = OMMPSystem(fixture_file("1ubq", "input_AMOEBA.mmp"))
+    yield s
+    s.terminate()
+
+
+def test_fmm_matvec_converges_to_direct_1ubq(ubq):
+    """Error of the FMM matvec (largest component difference over the largest
+    component) against the direct FP64 one falls steadily with the order."""
+    s = ubq
+    x = np.random.default_rng(3).normal(size=(2, s.pol_atoms, 3))
+    want, _ = s.tmatvec(x, kind="otf64")
+    errs = []
+    for lmax, tol in ((4, 3e-3), (8, 3e-4), (12, 3e-5), (18, 2e-6)):
+        s.enable_fmm(lmax_pol=lmax)
+        got, _ = s.tmatvec(x, kind="fmm")
+        err = _rel(got, want)
+        print(f"1ubq FMM matvec lmax {lmax}: rel err {err:.3e}", flush=True)
+        assert err < tol
+        errs.append(err)
+    s.disable_fmm()
+    assert all(b < 0.5 * a for a, b in zip(errs, errs[1:]))
+
+
+@pytest.mark.parametrize("lmax,tol", [(8, 2e-3), (18, 5e-6)])
+def test_fmm_static_field_matches_direct_1ubq(ubq, lmax, tol):
+    s = ubq
+    s.disable_fmm()
+    want = np.array(s.e_m2d)
+    s.enable_fmm(lmax=lmax)
+    got = np.array(s.e_m2d)
+    s.disable_fmm()
+    err = _rel(got, want)
+    print(f"1ubq FMM static field lmax {lmax}: rel err {err:.3e}", flush=True)
+    assert err < tol
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_fmm_solve_matches_reference_fixture_1ubq(ff, fixture_file):
+    from openmmpol_amd import OMMPSystem
+    s = OMMPSystem(fixture_file("1ubq", f"input_{ff}.mmp"))
+    s.enable_fmm(lmax=18, lmax_pol=18, distance=8.0 * mmp_io.ANG2AU)
+    assert s.use_fmm
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    ep = s.get_polelec_energy()
+    ref = mmp_io.read_ipd(mmp_io.fixture_path("1ubq", f"IPD_0_{ff}.ref"))
+    got = s.ipd.reshape(-1, 3)
+    print(f"1ubq {ff} FMM solve: max |d mu| {np.abs(got - ref).max():.3e}", flush=True)
+    assert np.allclose(got, ref, rtol=1e-6, atol=1e-5)
+    eref = mmp_io.read_ene(mmp_io.fixture_path("1ubq", f"ENE_0_{ff}.ref"))["EP"]
+    assert abs(ep - eref) <= 1e-6 * abs(eref)
+    s.terminate()
+
+
+@pytest.mark.parametrize("n,radius", [(15000, 33.0), (100000, 62.0)])
+def test_fmm_matvec_matches_direct_sphere(n, radius):
+    from openmmpol_amd import synth
+    spec = synth.sphere_system(n, radius, seed=5)
+    s = synth.make_system(spec)
+    x = np.random.default_rng(4).normal(size=(2, s.pol_atoms, 3))
+    want, ms_direct = s.tmatvec(x, kind="otf64")
+    for lmax, tol in ((8, 5e-4), (14, 5e-5)):
+        s.enable_fmm(lmax_pol=lmax)
+        got, ms = s.tmatvec(x, kind="fmm", repeat=2)
+        got2, _ = s.tmatvec(x, kind="fmm")
+        assert np.array_equal(got, got2)  # fixed summation order
+        err = _rel(got, want)
+        print(f"sphere {n} FMM matvec lmax {lmax}: rel err {err:.3e}, {ms:.3f} ms vs direct {ms_direct:.3f} ms",
+              flush=True)
+        assert err < tol
+    s.terminate()
