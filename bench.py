@@ -421,6 +421,73 @@ def run_grad(spec, tol, repeat, cpu):
     return res
 
 
+def fmm_m2l_flops(cmm, p):
+    """Complex multiply-adds of the M2L stage as 8 FLOP each: for every level
+    l >= 2 and every occupied target cell, (p+1)^4 per occupied source cell in
+    its interaction list (children of the parent's neighbours that are not
+    neighbours).  Same octree as the library (leaves >= 7 A)."""
+    lo, hi = cmm.min(0), cmm.max(0)
+    side = float((hi - lo).max()) * (1 + 1e-9) + 1e-6
+    origin = lo - 0.5 * (side - (hi - lo))
+    leaf_min = 7.0 * 1.8897261245650
+    L = 0
+    while L < 7 and side / 2 ** (L + 1) >= leaf_min:
+        L += 1
+    total = 0
+    for lev in range(2, L + 1):
+        ns = 2 ** lev
+        idx = np.clip(np.floor((cmm - origin) / (side / ns)).astype(int), 0, ns - 1)
+        occ = np.zeros((ns, ns, ns), dtype=bool)
+        occ[idx[:, 2], idx[:, 1], idx[:, 0]] = True
+        cells = np.argwhere(occ)
+        for c in cells:
+            base = 2 * (c // 2) - 2
+            lo3, hi3 = np.maximum(base, 0), np.minimum(base + 5, ns - 1)
+            sub = occ[lo3[0]:hi3[0] + 1, lo3[1]:hi3[1] + 1, lo3[2]:hi3[2] + 1]
+            near = occ[max(c[0] - 1, 0):c[0] + 2, max(c[1] - 1, 0):c[1] + 2, max(c[2] - 1, 0):c[2] + 2]
+            total += int(sub.sum()) - int(near.sum())
+    return 8 * total * (p + 1) ** 4, L
+
+
+def run_fmm(tol):
+    """250,000 AMOEBA sites in an 84 A sphere: the fast-multipole T matvec
+    (order 8, the reference's default) against the direct FP64 on-the-fly one
+    (time, largest-component error), and a complete FP64 solve with fast
+    multipoles against the mixed direct solve of the same system."""
+    from openmmpol_amd import synth
+    n_sites, radius, seed, p = 250000, 84.0, 5, 8
+    spec = synth.sphere_system(n_sites, radius, seed=seed)
+    s = synth.make_system(spec)
+    n = s.pol_atoms
+    x = np.random.default_rng(4).normal(size=(2, n, 3))
+    y_dir, ms_dir = s.tmatvec(x, kind="otf64", repeat=2)
+    _, ms_dir32 = s.tmatvec(x, kind="otf32", repeat=2)
+    s.enable_fmm(lmax=p, lmax_pol=p)
+    y_fmm, ms_fmm = s.tmatvec(x, kind="fmm", repeat=5)
+    err = float(np.abs(y_fmm - y_dir).max() / np.abs(y_dir).max())
+    s.set_solver_tolerance(tol, 500)
+    t = time.perf_counter()
+    s.set_external_field(np.zeros((n, 3)))
+    e_fmm = s.get_polelec_energy()
+    t_fmm = time.perf_counter() - t
+    it_fmm = s.last_iterations[:2]
+    s.disable_fmm()
+    s.set_precision("mixed")
+    t = time.perf_counter()
+    s.set_external_field(np.zeros((n, 3)))
+    e_dir = s.get_polelec_energy()
+    t_dir = time.perf_counter() - t
+    s.terminate()
+    flops, levels = fmm_m2l_flops(np.asarray(spec["cmm"]), p)
+    return {"workload": f"{n_sites}-site AMOEBA sphere (R {radius} A, seed {seed}), two dipole columns",
+            "order": p, "levels": levels,
+            "fmm_matvec_ms": ms_fmm, "direct_fp64_matvec_ms": ms_dir, "direct_fp32_matvec_ms": ms_dir32,
+            "speedup_vs_direct_fp64": ms_dir / ms_fmm, "matvec_rel_err": err,
+            "m2l_flop_per_matvec": 2 * flops,
+            "solve_fmm_s": t_fmm, "solve_direct_mixed_s": t_dir, "iterations_fmm": list(it_fmm),
+            "epol_fmm": e_fmm, "epol_direct": e_dir, "epol_rel_diff": abs(e_fmm - e_dir) / abs(e_dir)}
+
+
 def cpu_baseline(spec, fp64_iters):
     """Oracle (FP64 C/OpenMP restatement of the reference) on the host cores:
     static field once + 8 T matvecs of the same system, extrapolated to one
@@ -463,7 +530,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=16)
     ap.add_argument("--serving-gap-ms", type=float, default=10.0)
     ap.add_argument("--serving-slots", type=int, default=16)
-    ap.add_argument("--legs", default="headline,fp64,multi,serving,incore,large,grad,cpu",
+    ap.add_argument("--legs", default="headline,fp64,multi,serving,incore,large,grad,fmm,cpu",
                     help="comma-separated subset to run (profiling one leg); the default runs all")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -517,6 +584,11 @@ def main():
     if "grad" in legs:
         grad = run_grad(spec, args.tol, 10, "cpu" in legs)
         print(f"[bench] gradient: {grad['ms_per_gradient']:.2f} ms per call", file=sys.stderr, flush=True)
+    fmm = None
+    if "fmm" in legs:
+        fmm = run_fmm(args.tol)
+        print(f"[bench] FMM matvec: {fmm['fmm_matvec_ms']:.2f} ms vs direct {fmm['direct_fp64_matvec_ms']:.2f} ms",
+              file=sys.stderr, flush=True)
     incore = None
     if "incore" in legs:
         incore = run_incore(spec, 20)
@@ -533,7 +605,8 @@ def main():
     if mixed is None:
         # a subset of the legs (profiling): no headline figure
         out = {"legs": sorted(legs)}
-        for key, val in (("multi_rhs", multi), ("serving", serving), ("large_60k", large), ("geomgrad", grad)):
+        for key, val in (("multi_rhs", multi), ("serving", serving), ("large_60k", large), ("geomgrad", grad),
+                         ("fmm_250k", fmm)):
             if val is not None:
                 out[key] = val
         print(json.dumps(out))
@@ -582,6 +655,8 @@ def main():
         out["serving"] = serving
     if grad is not None:
         out["geomgrad"] = grad
+    if fmm is not None:
+        out["fmm_250k"] = fmm
     if incore is not None:
         b = incore_bytes(n, 2, 4)
         out["incore_fp32_matvec"] = {

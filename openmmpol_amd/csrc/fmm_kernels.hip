@@ -122,64 +122,93 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(int p, int nside, double h, dou
     for (int t = s0; t < s1; t++) {
         const double4 ps = pos[t];
         const double vx = ps.x - cx, vy = ps.y - cy, vz = ps.z - cz;
-        double ux[7], uy[7], uz[7], wv[7], w1[7], w2[7];
-        int nd;
+        const double r2v = vx * vx + vy * vy + vz * vz;
+        const double2 w = make_double2(vx, vy);
         if (KIND == 0) {
+            // first derivative along the dipole u: (value, derivative) pairs
             const double *d = src + 3L * t;
-            ux[0] = d[0];
-            uy[0] = d[1];
-            uz[0] = d[2];
-            wv[0] = 0.0;
-            w1[0] = 1.0;
-            w2[0] = 0.0;
-            nd = 1;
+            const double ux = d[0], uy = d[1], uz = d[2];
+            const double2 wd = make_double2(ux, uy);
+            const double r2d = 2.0 * (vx * ux + vy * uy + vz * uz);
+            double2 cv = make_double2(1.0, 0.0), cd = make_double2(0.0, 0.0);
+            for (int k = 1; k <= m; k++) {
+                const double sk = 1.0 / (2.0 * k);
+                const double2 nv = cscale(cmul(cv, w), sk);
+                cd = cscale(cadd(cmul(cd, w), cmul(cv, wd)), sk);
+                cv = nv;
+            }
+            double2 pv = make_double2(0.0, 0.0), pd = make_double2(0.0, 0.0);
+            for (int k = m + 1; k <= n; k++) {
+                double2 nv, nd;
+                if (k == m + 1) {
+                    nv = cscale(cv, vz);
+                    nd = cadd(cscale(cd, vz), cscale(cv, uz));
+                } else {
+                    const double a = 2.0 * k - 1.0, c = 1.0 / ((double)k * k - (double)m * m);
+                    nv = make_double2(c * (a * vz * cv.x - r2v * pv.x), c * (a * vz * cv.y - r2v * pv.y));
+                    nd = make_double2(c * (a * (uz * cv.x + vz * cd.x) - (r2d * pv.x + r2v * pd.x)),
+                                      c * (a * (uz * cv.y + vz * cd.y) - (r2d * pv.y + r2v * pd.y)));
+                }
+                pv = cv;
+                pd = cd;
+                cv = nv;
+                cd = nd;
+            }
+            accp = cadd(accp, cd);
         } else {
             const double *q = src + (long)ld * t;
-            // direction 0: the charge (value) and the dipole (first derivative along mu)
-            ux[0] = ld > 1 ? q[1] : 0.0;
-            uy[0] = ld > 1 ? q[2] : 0.0;
-            uz[0] = ld > 1 ? q[3] : 0.0;
-            wv[0] = q[0];
-            w1[0] = ld > 1 ? 1.0 : 0.0;
-            w2[0] = 0.0;
-            nd = 1;
-            if (ld > 1) {
-                // Q : H from second derivatives along x, y, z, x+y, x+z, y+z
-                const double Qxx = q[4], Qxy = q[5], Qyy = q[6], Qxz = q[7], Qyz = q[8], Qzz = q[9];
-                const double dirs[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
-                const double w[6] = {Qxx - Qxy - Qxz, Qyy - Qxy - Qyz, Qzz - Qxz - Qyz, Qxy, Qxz, Qyz};
-                for (int k = 0; k < 6; k++) {
-                    ux[1 + k] = dirs[k][0];
-                    uy[1 + k] = dirs[k][1];
-                    uz[1 + k] = dirs[k][2];
-                    wv[1 + k] = 0.0;
-                    w1[1 + k] = 0.0;
-                    w2[1 + k] = w[k];
+            // direction 0: the charge (value) and the dipole (first derivative along
+            // mu); directions 1-6: Q : H from second derivatives along x, y, z,
+            // x+y, x+z, y+z
+            const int nd = ld > 1 ? 7 : 1;
+            for (int dd = 0; dd < nd; dd++) {
+                double ux, uy, uz, wv, w1, w2;
+                if (dd == 0) {
+                    ux = ld > 1 ? q[1] : 0.0;
+                    uy = ld > 1 ? q[2] : 0.0;
+                    uz = ld > 1 ? q[3] : 0.0;
+                    wv = q[0];
+                    w1 = ld > 1 ? 1.0 : 0.0;
+                    w2 = 0.0;
+                } else {
+                    const int k = dd - 1;
+                    ux = (k == 0 || k == 3 || k == 4) ? 1.0 : 0.0;
+                    uy = (k == 1 || k == 3 || k == 5) ? 1.0 : 0.0;
+                    uz = (k == 2 || k == 4 || k == 5) ? 1.0 : 0.0;
+                    wv = 0.0;
+                    w1 = 0.0;
+                    const double Qxx = q[4], Qxy = q[5], Qyy = q[6], Qxz = q[7], Qyz = q[8], Qzz = q[9];
+                    w2 = k == 0 ? Qxx - Qxy - Qxz
+                                : k == 1 ? Qyy - Qxy - Qyz
+                                         : k == 2 ? Qzz - Qxz - Qyz : k == 3 ? Qxy : k == 4 ? Qxz : Qyz;
+                    if (w2 == 0.0) continue;
                 }
-                nd = 7;
+                const Jet wj = {w, make_double2(ux, uy), make_double2(0.0, 0.0)};
+                const double r2d = 2.0 * (vx * ux + vy * uy + vz * uz), r2dd = 2.0 * (ux * ux + uy * uy + uz * uz);
+                Jet cur = jreal(1.0, 0.0, 0.0);
+                for (int k = 1; k <= m; k++) cur = jscale(jmul(cur, wj), 1.0 / (2.0 * k));
+                Jet prev = jreal(0.0, 0.0, 0.0);
+                for (int k = m + 1; k <= n; k++) {
+                    // z = (vz, uz, 0), rho^2 = (r2v, r2d, r2dd): real jets
+                    Jet zc;
+                    zc.v = cscale(cur.v, vz);
+                    zc.d1 = cadd(cscale(cur.d1, vz), cscale(cur.v, uz));
+                    zc.d2 = cadd(cscale(cur.d2, vz), cscale(cur.d1, 2.0 * uz));
+                    Jet next;
+                    if (k == m + 1) {
+                        next = zc;
+                    } else {
+                        Jet rp;
+                        rp.v = cscale(prev.v, r2v);
+                        rp.d1 = cadd(cscale(prev.d1, r2v), cscale(prev.v, r2d));
+                        rp.d2 = cadd(cadd(cscale(prev.d2, r2v), cscale(prev.d1, 2.0 * r2d)), cscale(prev.v, r2dd));
+                        next = jscale(jsub(jscale(zc, 2.0 * k - 1.0), rp), 1.0 / ((double)k * k - (double)m * m));
+                    }
+                    prev = cur;
+                    cur = next;
+                }
+                accp = cadd(accp, cadd(cadd(cscale(cur.v, wv), cscale(cur.d1, w1)), cscale(cur.d2, w2)));
             }
-        }
-#pragma unroll
-        for (int dd = 0; dd < 7; dd++) {
-            if (dd >= nd) break;
-            // (x + iy), z and rho^2 as jets along u (x + iy and z have no second derivative)
-            const Jet w = {make_double2(vx, vy), make_double2(ux[dd], uy[dd]), make_double2(0.0, 0.0)};
-            const Jet z = jreal(vz, uz[dd], 0.0);
-            const Jet r2 = jreal(vx * vx + vy * vy + vz * vz, 2.0 * (vx * ux[dd] + vy * uy[dd] + vz * uz[dd]),
-                                 2.0 * (ux[dd] * ux[dd] + uy[dd] * uy[dd] + uz[dd] * uz[dd]));
-            Jet cur = jreal(1.0, 0.0, 0.0);
-            for (int k = 1; k <= m; k++) cur = jscale(jmul(cur, w), 1.0 / (2.0 * k));
-            Jet prev = jreal(0.0, 0.0, 0.0);
-            for (int k = m + 1; k <= n; k++) {
-                Jet next;
-                if (k == m + 1) next = jmul(z, cur);
-                else
-                    next = jscale(jsub(jscale(jmul(z, cur), 2.0 * k - 1.0), jmul(r2, prev)),
-                                  1.0 / ((double)k * k - (double)m * m));
-                prev = cur;
-                cur = next;
-            }
-            accp = cadd(accp, cadd(cadd(cscale(cur.v, wv[dd]), cscale(cur.d1, w1[dd])), cscale(cur.d2, w2[dd])));
         }
     }
     // R_n^m = (-i)^m conj(Phi), R_n^-m = (-i)^m Phi
