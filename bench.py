@@ -639,7 +639,9 @@ def main():
         "epol_oracle_golden": ref_e,
         "epol_rel_err_mixed": abs(mixed["epol"] - ref_e) / abs(ref_e) if ref_e else None,
         "mu_maxabs_err_mixed": mu_err(mixed["ipd"]),
-        "roofline": {"bound": "valu", "kernel": "k_tmv_pairs<float,2,AMOEBA> (on-the-fly FP32 pair kernel)",
+        "roofline": {"bound": "mfma", "compute": "vector ALU: the FP32 pair kernel is compute-bound, priced "
+                                                 "against the FP32 compute peak",
+                     "kernel": "k_tmv_pairs<float,2,AMOEBA> (on-the-fly FP32 pair kernel)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
                      "traffic": profile_traffic("k_tmv_pairs<float,2,AMOEBA>"),

@@ -265,13 +265,15 @@ __global__ __launch_bounds__(256) void k_fmm_m2m(int p, int nside, const int *__
 // --------------------------------------------------------------- M2L
 // Target cells at level l: sources are the children of the parent's neighbours
 // that are not neighbours of the target (offsets -3..3).  tab[(dz+3)*49+(dy+3)*7+dx+3][coef2]
-// = T(c_source - c_target), coef2 over orders 0..2p.  The source list is built
-// in LDS first (fixed order); with at most 128 coefficients the workgroup runs
-// 256 / ncoef groups, each on every G-th source, and the group partials are
-// added in group order at the end.
+// = T(c_source - c_target), coef2 over orders 0..2p.  Real sources give
+// L_j^-k = (-1)^k conj(L_j^k) in this scaling, so only the (p+1)(p+2)/2
+// outputs with k >= 0 are summed.  The source list is built in LDS first (fixed
+// order); the workgroup runs G = 256 / nout groups (at most 8), each on every
+// G-th source, and the group partials are added in group order at the end.
+static __host__ __device__ inline int m2l_outputs(int p) { return (p + 1) * (p + 2) / 2; }
 static __host__ __device__ inline int m2l_groups(int p) {
-    const int ncoef = (p + 1) * (p + 1);
-    return ncoef <= 128 ? 256 / ncoef : 1;
+    const int g = 256 / m2l_outputs(p);
+    return g < 1 ? 1 : (g > 8 ? 8 : g);
 }
 
 __global__ __launch_bounds__(256) void k_fmm_m2l(int p, int nside, const int *__restrict__ cnt_src,
@@ -285,6 +287,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(int p, int nside, const int *__
     __shared__ int s_n;
     const int ncoef = (p + 1) * (p + 1);
     const int ncoef2 = (2 * p + 1) * (2 * p + 1);
+    const int nout = m2l_outputs(p);
     const int G = m2l_groups(p);
     const int cell = blockIdx.x;
     const int tid = threadIdx.x;
@@ -318,23 +321,24 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(int p, int nside, const int *__
     }
     __syncthreads();
     const int nsrc = s_n;
-    // thread -> (group, output); with G == 1 a thread may own outputs tid and tid + 256
-    const int g = G > 1 ? tid / ncoef : 0;
-    const int out0 = G > 1 ? tid - g * ncoef : tid;
+    // thread -> (group, output r); output r -> (j, k >= 0); a thread may own
+    // outputs r and r + 256 when one group covers more than 256 outputs
+    const int gsize = G > 1 ? nout : 256;
+    const int g = tid / gsize;
+    const int r0 = tid - g * gsize;
     const bool active = g < G;
     double2 *s_o = s_m2l + (long)(active ? g : 0) * (ncoef + ncoef2);
     double2 *s_t = s_o + ncoef;
-    const int gsize = G > 1 ? ncoef : 256;
-    const int lt = G > 1 ? out0 : tid;  // loader index within the group
     double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
     int jj[2], kk[2];
-    for (int r = 0; r < 2; r++) {
-        const int out = out0 + 256 * r;
+    for (int q = 0; q < 2; q++) {
+        const int rr = r0 + 256 * q;
         int j = 0;
-        while ((j + 1) * (j + 1) <= out) j++;
-        jj[r] = j;
-        kk[r] = out - j * j - j;
+        while ((j + 1) * (j + 2) / 2 <= rr) j++;
+        jj[q] = j;
+        kk[q] = rr - j * (j + 1) / 2;
     }
+    const int nq = G > 1 ? 1 : 2;
     for (int it = 0; it < nsrc; it += G) {
         const int idx = it + g;
         const bool has = active && idx < nsrc;
@@ -342,36 +346,40 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(int p, int nside, const int *__
         if (has) {
             const double2 *O = msrc + (long)s_list[idx] * ncoef;
             const double2 *T = tab + (long)s_code[idx] * tstride;
-            for (int c = lt; c < ncoef; c += gsize) s_o[c] = O[c];
-            for (int c = lt; c < ncoef2; c += gsize) s_t[c] = T[c];
+            for (int c = r0; c < ncoef; c += gsize) s_o[c] = O[c];
+            for (int c = r0; c < ncoef2; c += gsize) s_t[c] = T[c];
         }
         __syncthreads();
         if (has) {
-            for (int r = 0; r < (G > 1 ? 1 : 2); r++) {
-                if (out0 + 256 * r >= ncoef) break;
-                const int j = jj[r], k = kk[r];
-                double2 a = acc[r];
-                for (int n = 0; n <= p; n++)
-                    for (int mm = -n; mm <= n; mm++) cmac(a, s_o[hidx(n, mm)], s_t[hidx(j + n, mm - k)]);
-                acc[r] = a;
+            for (int q = 0; q < nq; q++) {
+                if (r0 + 256 * q >= nout) break;
+                const int j = jj[q], k = kk[q];
+                double2 a = acc[q];
+                for (int n = 0; n <= p; n++) {
+                    const double2 *On = s_o + n * n + n;
+                    const double2 *Tn = s_t + (j + n) * (j + n) + (j + n) - k;
+                    for (int mm = -n; mm <= n; mm++) cmac(a, On[mm], Tn[mm]);
+                }
+                acc[q] = a;
             }
         }
     }
     if (G > 1) {
         __syncthreads();
         double2 *red = s_m2l;
-        if (active) red[g * ncoef + out0] = acc[0];
+        if (active) red[g * nout + r0] = acc[0];
         __syncthreads();
         if (g == 0) {
-            double2 a = red[out0];
-            for (int q = 1; q < G; q++) a = cadd(a, red[q * ncoef + out0]);
-            o[out0] = a;
+            double2 a = red[r0];
+            for (int q = 1; q < G; q++) a = cadd(a, red[q * nout + r0]);
+            acc[0] = a;
         }
-    } else {
-        for (int r = 0; r < 2; r++) {
-            const int out = out0 + 256 * r;
-            if (out < ncoef) o[out] = acc[r];
-        }
+    }
+    for (int q = 0; q < nq; q++) {
+        if (g != 0 || r0 + 256 * q >= nout) break;
+        const int j = jj[q], k = kk[q];
+        o[hidx(j, k)] = acc[q];
+        if (k > 0) o[hidx(j, -k)] = (k & 1) ? make_double2(-acc[q].x, acc[q].y) : make_double2(acc[q].x, -acc[q].y);
     }
 }
 
