@@ -89,3 +89,37 @@ def test_fmm_matvec_matches_direct_sphere(n, radius):
               flush=True)
         assert err < tol
     s.terminate()
+
+
+def test_fmm_on_a_small_system_is_the_direct_sum(fixture_file):
+    """N-methylacetamide fits in one leaf: with fast multipoles on, every pair is
+    in the near field and the solve equals the direct one up to summation order."""
+    from openmmpol_amd import OMMPSystem
+    path = fixture_file("N-methylacetamide", "input_AMOEBA.mmp")
+    a = OMMPSystem(path)
+    b = OMMPSystem(path)
+    b.set_default_solver("cg")
+    b.enable_fmm()
+    ext = mmp_io.read_ef(mmp_io.fixture_path("N-methylacetamide", "EF_1.txt"))[np.asarray(a.polar_mm)]
+    a.set_external_field(ext)
+    b.set_external_field(ext)
+    assert np.abs(a.ipd - b.ipd).max() < 1e-12
+    assert abs(a.get_polelec_energy() - b.get_polelec_energy()) <= 1e-13 * abs(a.get_polelec_energy())
+    a.terminate()
+    b.terminate()
+
+
+def test_fmm_settings_take_effect(ubq):
+    s = ubq
+    s.disable_fmm()
+    assert not s.use_fmm
+    direct = np.array(s.e_m2d)
+    s.enable_fmm(lmax=2)
+    assert s.use_fmm
+    low = np.array(s.e_m2d)
+    s.enable_fmm(lmax=16)
+    high = np.array(s.e_m2d)
+    s.disable_fmm()
+    again = np.array(s.e_m2d)
+    assert np.array_equal(again, direct)
+    assert _rel(high, direct) < 0.1 * _rel(low, direct)
