@@ -539,8 +539,8 @@ def main():
         if flag:
             legs.discard(leg)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(one process drives one GPU)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes
