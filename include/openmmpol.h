@@ -115,7 +115,10 @@ void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c);
    pairs through complex solid-harmonic expansions of order lmax (static) and
    lmax_pol (dipoles), default 8, at most 20.  Any change drops the static field
    and the dipoles.  With fast multipoles the solve runs in FP64 whatever
-   ommp_amd_set_precision says; the gradient and the batched solvers stay direct. */
+   ommp_amd_set_precision says; the gradient and the batched solvers stay direct.
+   Unlike the reference, which switches them on by itself above 1,000 MM atoms
+   (mod_electrostatics.F90:294-300), this library computes every interaction
+   directly until ommp_enable_fmm is called. */
 void ommp_enable_fmm(OMMP_SYSTEM_PRT s);
 void ommp_disable_fmm(OMMP_SYSTEM_PRT s);
 bool ommp_use_fmm(OMMP_SYSTEM_PRT s);
