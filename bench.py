@@ -452,8 +452,10 @@ def fmm_m2l_flops(cmm, p):
 def run_fmm(tol):
     """250,000 AMOEBA sites in an 84 A sphere: the fast-multipole T matvec
     (order 8, the reference's default) against the direct FP64 on-the-fly one
-    (time, largest-component error), and a complete FP64 solve with fast
-    multipoles against the mixed direct solve of the same system."""
+    (time, largest-component error), a complete FP64 solve with fast
+    multipoles against the mixed direct solve of the same system, and the
+    gradient of E_pol through the tree against the direct gradient (each after
+    its own solve, so their difference includes that of the dipoles)."""
     from openmmpol_amd import synth
     n_sites, radius, seed, p = 250000, 84.0, 5, 8
     spec = synth.sphere_system(n_sites, radius, seed=seed)
@@ -471,12 +473,25 @@ def run_fmm(tol):
     e_fmm = s.get_polelec_energy()
     t_fmm = time.perf_counter() - t
     it_fmm = s.last_iterations[:2]
+    # gradient through the tree (first call builds the gradient's lists)
+    g_fmm = s.polelec_geomgrad()
+    s.profile_kernels(True)
+    s.kernel_stats(reset=True)
+    t = time.perf_counter()
+    for _ in range(3):
+        g_fmm = s.polelec_geomgrad()
+    t_gfmm = (time.perf_counter() - t) / 3
+    gcalls, gms = s.kernel_stats(reset=True)["geomgrad"]
+    s.profile_kernels(False)
     s.disable_fmm()
     s.set_precision("mixed")
     t = time.perf_counter()
     s.set_external_field(np.zeros((n, 3)))
     e_dir = s.get_polelec_energy()
     t_dir = time.perf_counter() - t
+    t = time.perf_counter()
+    g_dir = s.polelec_geomgrad()
+    t_gdir = time.perf_counter() - t
     s.terminate()
     flops, levels = fmm_m2l_flops(np.asarray(spec["cmm"]), p)
     return {"workload": f"{n_sites}-site AMOEBA sphere (R {radius} A, seed {seed}), two dipole columns",
@@ -485,7 +500,10 @@ def run_fmm(tol):
             "speedup_vs_direct_fp64": ms_dir / ms_fmm, "matvec_rel_err": err,
             "m2l_flop_per_matvec": 2 * flops,
             "solve_fmm_s": t_fmm, "solve_direct_mixed_s": t_dir, "iterations_fmm": list(it_fmm),
-            "epol_fmm": e_fmm, "epol_direct": e_dir, "epol_rel_diff": abs(e_fmm - e_dir) / abs(e_dir)}
+            "epol_fmm": e_fmm, "epol_direct": e_dir, "epol_rel_diff": abs(e_fmm - e_dir) / abs(e_dir),
+            "grad_fmm_ms": t_gfmm * 1e3, "grad_fmm_pair_sums_ms": gms / max(gcalls, 1),
+            "grad_direct_ms": t_gdir * 1e3, "grad_speedup_vs_direct": t_gdir / t_gfmm,
+            "grad_rel_diff_vs_direct": float(np.abs(g_fmm - g_dir).max() / np.abs(g_dir).max())}
 
 
 def cpu_baseline(spec, fp64_iters):
@@ -587,7 +605,8 @@ def main():
     fmm = None
     if "fmm" in legs:
         fmm = run_fmm(args.tol)
-        print(f"[bench] FMM matvec: {fmm['fmm_matvec_ms']:.2f} ms vs direct {fmm['direct_fp64_matvec_ms']:.2f} ms",
+        print(f"[bench] FMM matvec: {fmm['fmm_matvec_ms']:.2f} ms vs direct {fmm['direct_fp64_matvec_ms']:.2f} ms; "
+              f"FMM gradient {fmm['grad_fmm_ms']:.2f} ms vs direct {fmm['grad_direct_ms']:.2f} ms",
               file=sys.stderr, flush=True)
     incore = None
     if "incore" in legs:

@@ -119,6 +119,10 @@ struct GradArgs {
 // out[nmm][3] = d E_pol / d r (pair sweeps + rotation term); emm/egrd receive
 // E_D2M and Egrd_D2M.  evb/eve bracket the three pair sweeps.
 void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
+// The same after the pair sums: finalize (sum of nparts partials in the layouts
+// above, screened-pair corrections), rotation term and per-atom assembly, for
+// partials computed elsewhere (fast multipoles).
+void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm);
 // out[nmm][3] = rotation term only, for the field a.emm and gradient a.egrd.
 void launch_rotation_grad(hipStream_t st, const GradArgs &a);
 

@@ -105,6 +105,7 @@ struct DeviceState {
     bool fmm_valid = false;
     std::vector<void *> fmm_ptrs;
     double *fmm_part = nullptr, *fmm_rhs_part = nullptr;
+    double *fmm_gpart_pol = nullptr, *fmm_gpart_mm = nullptr;  // gradient partials, on first use
 };
 
 namespace {
@@ -380,7 +381,7 @@ void fmm_release(DeviceState &d) {
     for (void *p : d.fmm_ptrs) dfree(d, p);
     d.fmm_ptrs.clear();
     d.fmm = FmmDev();
-    d.fmm_part = d.fmm_rhs_part = nullptr;
+    d.fmm_part = d.fmm_rhs_part = d.fmm_gpart_pol = d.fmm_gpart_mm = nullptr;
     d.fmm_valid = false;
 }
 
@@ -433,6 +434,8 @@ void ensure_fmm(System &s, DeviceState &d) {
     f.pol_self = up_i(pself);
     f.pol_mmself = up_i(h.pol_mmself);
     f.pol_leaf_start = up_i(h.pol_leaf_start);
+    f.mm_order = up_i(h.mm_order);
+    f.mm_leaf = up_i(h.mm_leaf);
     f.mm_leaf_start = up_i(h.mm_leaf_start);
     f.cnt_pol = up_i(h.cnt_pol);
     f.cnt_mm = up_i(h.cnt_mm);
@@ -457,7 +460,7 @@ void ensure_fmm(System &s, DeviceState &d) {
     const size_t ncell = (size_t)h.cell_off.back();
     f.mexp = fmm_alloc<double2>(d, ncell * f.ncoef_tab);
     f.lexp = fmm_alloc<double2>(d, ncell * f.ncoef_tab);
-    f.xs = fmm_alloc<double>(d, 6L * npol);
+    f.xs = fmm_alloc<double>(d, 9L * npol);
     d.fmm_part = fmm_alloc<double>(d, (size_t)(kFmmNearSplits + 1) * 2 * 3 * npol);
     d.fmm_rhs_part = fmm_alloc<double>(d, (size_t)(kFmmNearSplits + 1) * 3 * npol);
     d.fmm_valid = true;
@@ -1207,10 +1210,26 @@ void polelec_geomgrad_host(System &s, double *grd) {
     }
     DeviceState &d = ensure_device(s);
     ensure_grad(s, d);
-    const GradArgs a = grad_args(s, d);
+    GradArgs a = grad_args(s, d);
     const int pi = prof_begin(d, 5);
-    launch_geomgrad(d.stream, a, pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
-                    pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
+    hipEvent_t evb = pi >= 0 ? d.prof_pool[2 * pi] : nullptr, eve = pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr;
+    if (s.fmm_requested) {
+        // pair sums by fast multipoles (near field + far field), then the same
+        // finalize, rotation and assembly as the direct sweeps
+        ensure_fmm(s, d);
+        if (!d.fmm_gpart_pol) {
+            d.fmm_gpart_pol = fmm_alloc<double>(d, (size_t)fmm_grad_parts_pol(s.amoeba) * 3 * s.npol);
+            d.fmm_gpart_mm = fmm_alloc<double>(d, (size_t)fmm_grad_parts_mm() * (s.amoeba ? 12 : 3) * s.nmm);
+        }
+        a.part_pol = d.fmm_gpart_pol;
+        a.part_mm = d.fmm_gpart_mm;
+        if (evb) HCHECK(hipEventRecord(evb, d.stream));
+        fmm_launch_grad_fields(d.stream, d.fmm, s.amoeba, s.ld_cart, d.ipd, a.part_pol, a.part_mm);
+        if (eve) HCHECK(hipEventRecord(eve, d.stream));
+        launch_geomgrad_finish(d.stream, a, fmm_grad_parts_pol(s.amoeba), fmm_grad_parts_mm());
+    } else {
+        launch_geomgrad(d.stream, a, evb, eve);
+    }
     HCHECK(hipMemcpyAsync(grd, d.g_out, sizeof(double) * 3L * s.nmm, hipMemcpyDeviceToHost, d.stream));
     HCHECK(hipStreamSynchronize(d.stream));
 }

@@ -153,10 +153,12 @@ FmmHost fmm_build(const System &s, int p_pol, int p_static, double min_cell, dou
     }
     for (long k = 0; k < nleaf; k++) f.mm_leaf_start[k + 1] += f.mm_leaf_start[k];
     f.mm_order.assign(nmm, 0);
+    f.mm_leaf.assign(nmm, 0);
     std::vector<int> mm_rank(nmm), fill(f.mm_leaf_start.begin(), f.mm_leaf_start.end() - 1);
     for (int i = 0; i < nmm; i++) {
         const int t = fill[mleaf[i]]++;
         f.mm_order[t] = i;
+        f.mm_leaf[t] = (int)mleaf[i];
         mm_rank[i] = t;
     }
     // polarizable sites by leaf

@@ -31,7 +31,9 @@ struct FmmDev {
     int *pol_leaf_start = nullptr; // [nleaf + 1]
     double4 *pol_pos = nullptr;    // (x, y, z, 1/thole)
     double *pol_a = nullptr;       // damping a
-    // sorted MM atoms (static sources)
+    // sorted MM atoms (static sources, gradient targets)
+    int *mm_order = nullptr;       // sorted -> MM index
+    int *mm_leaf = nullptr;        // sorted -> leaf
     int *mm_leaf_start = nullptr;
     double4 *mm_pos = nullptr;
     double *mm_a = nullptr;
@@ -43,7 +45,7 @@ struct FmmDev {
     double2 *tab_m2m = nullptr;    // [level][8][ncoef_tab]   R(c_child - c_parent), parent level
     double2 *tab_l2l = nullptr;    // [level][8][ncoef_tab]   S(c_child - c_parent), parent level
     double2 *tab_m2l = nullptr;    // [level][343][ncoef2_tab] T(c_source - c_target)
-    double *xs = nullptr;          // sorted dipole columns [2][npol][3]
+    double *xs = nullptr;          // sorted dipole columns [3][npol][3]
 };
 
 // Near (splits 0 .. 26, one per neighbour leaf) and far (split 27) dipole fields
@@ -54,5 +56,19 @@ void fmm_launch_matvec_fields(hipStream_t st, const FmmDev &f, bool amoeba, int 
 // Near (splits 0 .. 26) and far (split 27) fields of the static multipoles at
 // the polarizable sites: part[split * 3 npol + 3 j + c].
 void fmm_launch_static_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, double *part);
+// Pair sums of the polarization-energy gradient, near (splits 0 .. 26) and far,
+// in the partial layouts of the direct gradient sweeps:
+//   part_pol[split][3][npol]: sum_b w_b Egrd_ab at the polarizable sites, far
+//     splits 27 and 28 (AMOEBA: static + mu_P sources against 1/2 mu_D, static +
+//     mu_D against 1/2 mu_P) or 27 (AMBER: static + mu against mu);
+//   part_mm[split][na][nmm]: E, Egrd and Q : EHes (na = 12, AMOEBA) or E (na = 3)
+//     of the dipoles nu = 1/2 (mu_D + mu_P) (AMOEBA) or mu at the MM atoms, far
+//     split 27.
+// Every pair within the 27 leaves around a target is in the near field with
+// scale 1 and the damped kernels; ipd = [D or mu][npol][3] (+ [P][npol][3]).
+void fmm_launch_grad_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, const double *ipd,
+                            double *part_pol, double *part_mm);
+constexpr int fmm_grad_parts_pol(bool amoeba) { return kFmmNearSplits + (amoeba ? 2 : 1); }
+constexpr int fmm_grad_parts_mm() { return kFmmNearSplits + 1; }
 
 }  // namespace ommp

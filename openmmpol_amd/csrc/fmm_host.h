@@ -17,7 +17,7 @@ struct FmmHost {
     double side = 0.0, leaf_h = 0.0, origin[3] = {0.0, 0.0, 0.0};
     std::vector<long> cell_off;
     std::vector<int> pol_order, pol_leaf, pol_mmself, pol_leaf_start;
-    std::vector<int> mm_order, mm_leaf_start;
+    std::vector<int> mm_order, mm_leaf, mm_leaf_start;
     std::vector<int> cnt_pol, cnt_mm;
     std::vector<double2> tab_m2m, tab_l2l, tab_m2l;
 };

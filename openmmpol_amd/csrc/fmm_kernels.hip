@@ -36,6 +36,7 @@
 
 #include "device.h"
 #include "fmm.h"
+#include "grad_math.h"
 
 namespace ommp {
 
@@ -101,11 +102,12 @@ __device__ __forceinline__ int hidx(int n, int m) { return n * n + n + m; }
 // accumulates the +-m coefficients in registers; (p+1)(p+2)/2 threads per leaf,
 // as many leaves per 256-thread workgroup as fit.
 // KIND 0: dipoles xs[t][3] (sorted);  KIND 1: multipoles q[t][ld] (sorted MM atoms).
+// add != 0 adds to the expansions already in out (two source sets in one tree).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_fmm_p2m(int p, int nside, double h, double3 origin, int nleaf,
                                                 const int *__restrict__ leaf_start,
                                                 const double4 *__restrict__ pos, const double *__restrict__ src,
-                                                int ld, double2 *__restrict__ out) {
+                                                int ld, int add, double2 *__restrict__ out) {
     const int npair = (p + 1) * (p + 2) / 2;
     const int per = 256 / npair;
     const int g = threadIdx.x / npair, r = threadIdx.x - g * npair;
@@ -215,8 +217,13 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(int p, int nside, double h, dou
     const int ncoef = (p + 1) * (p + 1);
     double2 *o = out + (long)leaf * ncoef;
     const int e = (-m) & 3;
-    o[hidx(n, m)] = ipow(e, cconj(accp));
-    if (m > 0) o[hidx(n, -m)] = ipow(e, accp);
+    double2 vp = ipow(e, cconj(accp)), vm = ipow(e, accp);
+    if (add) {
+        vp = cadd(vp, o[hidx(n, m)]);
+        if (m > 0) vm = cadd(vm, o[hidx(n, -m)]);
+    }
+    o[hidx(n, m)] = vp;
+    if (m > 0) o[hidx(n, -m)] = vm;
 }
 
 // --------------------------------------------------------------- M2M
@@ -599,6 +606,314 @@ __global__ void k_fmm_gather(int n, int ncol, const int *__restrict__ order, con
         for (int k = 0; k < 3; k++) xs[(long)c * 3 * n + 3L * t + k] = x[c * xstride + 3 * j + k];
 }
 
+// ------------------------------------------------------------ gradient
+// Derivatives of a local expansion.  With Phi^_n^-m = (-1)^m conj(Phi_n^m) the
+// harmonics obey d/dz Phi^_n^m = Phi^_{n-1}^m, (d/dx + i d/dy) Phi^_n^m =
+// -Phi^_{n-1}^{m+1} and (d/dx - i d/dy) Phi^_n^m = Phi^_{n-1}^{m-1}, and
+// S_n^m = (-1)^n i^-m Phi^_n^m for every m.  Writing V = Re sum c_n^m Phi^_n^m
+// (c_n^m = (-1)^n i^-m L_n^m), d/dx shifts the coefficients to (Y - X) / 2,
+// d/dy to i (X + Y) / 2 and d/dz to c_{n+1}^m, with X c_n^m = c_{n+1}^{m-1} and
+// Y c_n^m = c_{n+1}^{m+1}.  Hence for the multi-index (A, B, C), ord = A + B + C:
+//   d^ord V = Re sum_{j,k} (-1)^ord i^-A [sum_s w_s (-1)^s L_{j+ord}^{k+A+B-2s}] S_j^k,
+// w_s = coefficient of X^s Y^(A+B-s) in (Y - X)^A (X + Y)^B / 2^(A+B).  Real
+// expansions pair the terms of k and -k into 2 Re(...) over k > 0.
+__host__ __device__ constexpr double fmm_binom(int n, int k) {
+    double r = 1.0;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+__host__ __device__ constexpr double fmm_dweight(int a, int b, int s) {
+    double r = 0.0;
+    for (int i = 0; i <= a; i++) {
+        const int l = s - i;
+        if (l < 0 || l > b) continue;
+        r += fmm_binom(a, i) * ((i & 1) ? -1.0 : 1.0) * fmm_binom(b, l);
+    }
+    for (int t = 0; t < a + b; t++) r *= 0.5;
+    return (s & 1) ? -r : r;
+}
+template <int A, int B, int S>
+__device__ __forceinline__ void fmm_dterm(const double2 *__restrict__ Lc, double2 &c) {
+    constexpr double w = fmm_dweight(A, B, S);
+    if constexpr (w != 0.0) {
+        const double2 l = Lc[-2 * S];
+        c.x = fma(w, l.x, c.x);
+        c.y = fma(w, l.y, c.y);
+    }
+}
+// acc += f * d^(A,B,C) of the (j, k) term, sv = S_j^k(v); L = the cell's expansion.
+template <int A, int B, int C>
+__device__ __forceinline__ void fmm_dacc(const double2 *__restrict__ L, int p, int j, int k, double2 sv, double f,
+                                         double &acc) {
+    constexpr int ord = A + B + C;
+    if (j + ord > p) return;
+    const int n = j + ord;
+    const double2 *Lc = L + n * n + n + k + A + B;  // L_n^(k + A + B - 2s) = Lc[-2s]
+    double2 c = make_double2(0.0, 0.0);
+    fmm_dterm<A, B, 0>(Lc, c);
+    if constexpr (A + B >= 1) fmm_dterm<A, B, 1>(Lc, c);
+    if constexpr (A + B >= 2) fmm_dterm<A, B, 2>(Lc, c);
+    if constexpr (A + B >= 3) fmm_dterm<A, B, 3>(Lc, c);
+    const double zr = c.x * sv.x - c.y * sv.y, zi = c.x * sv.y + c.y * sv.x;
+    // Re((-1)^ord i^-A z)
+    double v = (A & 1) ? zi : zr;
+    if (A >= 2) v = -v;
+    if (ord & 1) v = -v;
+    acc = fma(f, v, acc);
+}
+
+// One thread per target (sorted): derivatives of the far potential at the target.
+// MODE 0 (polarizable sites): Hessian H, out[c][j] = (H w)_c with w = c0 ipd[j] +
+//   c1 ipd[n3 + j] (j = order[t]; the second set only when c1 != 0), i.e.
+//   sum_b w_b Egrd_cb;
+// MODE 1 (MM atoms, AMOEBA): E = -grad V, Egrd = H and Q : EHes = -Q_bc d_abc V
+//   (q = sorted multipoles, quadrupole at 4..9), out[c][m], c < 12;
+// MODE 2 (MM atoms, AMBER): E only, c < 3.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fmm_l2p_grad(int p, int ntgt, int nside, double h, double3 origin,
+                                                      const double4 *__restrict__ pos, const int *__restrict__ leaf_of,
+                                                      const int *__restrict__ order, const double2 *__restrict__ loc,
+                                                      const double *__restrict__ ipd, long n3, double c0, double c1,
+                                                      const double *__restrict__ q, double *__restrict__ out,
+                                                      long ostride) {
+    constexpr int NV = MODE == 0 ? 6 : (MODE == 1 ? 19 : 3);
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntgt) return;
+    const double4 ps = pos[t];
+    const long cell = leaf_of[t];
+    const int ix = (int)(cell % nside), iy = (int)((cell / nside) % nside), iz = (int)(cell / ((long)nside * nside));
+    const double vx = ps.x - (origin.x + (ix + 0.5) * h), vy = ps.y - (origin.y + (iy + 0.5) * h),
+                 vz = ps.z - (origin.z + (iz + 0.5) * h);
+    const int ncoef = (p + 1) * (p + 1);
+    const double2 *L = loc + cell * ncoef;
+    double acc[NV];
+#pragma unroll
+    for (int a = 0; a < NV; a++) acc[a] = 0.0;
+    const double r2 = vx * vx + vy * vy + vz * vz;
+    const double2 w = make_double2(vx, vy);
+    double2 pmm = make_double2(1.0, 0.0);
+    for (int m = 0; m < p; m++) {
+        if (m > 0) pmm = cscale(cmul(pmm, w), 1.0 / (2.0 * m));
+        double2 p2 = make_double2(0.0, 0.0), p1 = pmm;
+        const double f = m > 0 ? 2.0 : 1.0;
+        const int e = (-m) & 3;
+        for (int n = m; n < p; n++) {
+            double2 cur;
+            if (n == m) {
+                cur = pmm;
+            } else if (n == m + 1) {
+                cur = cscale(pmm, vz);
+            } else {
+                const double a1 = 2.0 * n - 1.0, den = 1.0 / ((double)n * n - (double)m * m);
+                cur = make_double2(den * (a1 * vz * p1.x - r2 * p2.x), den * (a1 * vz * p1.y - r2 * p2.y));
+            }
+            if (n > m) {
+                p2 = p1;
+                p1 = cur;
+            }
+            double2 sv = ipow(e, cur);
+            if (n & 1) sv = make_double2(-sv.x, -sv.y);
+            if (MODE == 0) {
+                fmm_dacc<2, 0, 0>(L, p, n, m, sv, f, acc[0]);
+                fmm_dacc<1, 1, 0>(L, p, n, m, sv, f, acc[1]);
+                fmm_dacc<0, 2, 0>(L, p, n, m, sv, f, acc[2]);
+                fmm_dacc<1, 0, 1>(L, p, n, m, sv, f, acc[3]);
+                fmm_dacc<0, 1, 1>(L, p, n, m, sv, f, acc[4]);
+                fmm_dacc<0, 0, 2>(L, p, n, m, sv, f, acc[5]);
+            } else {
+                fmm_dacc<1, 0, 0>(L, p, n, m, sv, f, acc[0]);
+                fmm_dacc<0, 1, 0>(L, p, n, m, sv, f, acc[1]);
+                fmm_dacc<0, 0, 1>(L, p, n, m, sv, f, acc[2]);
+                if (MODE == 1) {
+                    fmm_dacc<2, 0, 0>(L, p, n, m, sv, f, acc[3]);
+                    fmm_dacc<1, 1, 0>(L, p, n, m, sv, f, acc[4]);
+                    fmm_dacc<0, 2, 0>(L, p, n, m, sv, f, acc[5]);
+                    fmm_dacc<1, 0, 1>(L, p, n, m, sv, f, acc[6]);
+                    fmm_dacc<0, 1, 1>(L, p, n, m, sv, f, acc[7]);
+                    fmm_dacc<0, 0, 2>(L, p, n, m, sv, f, acc[8]);
+                    fmm_dacc<3, 0, 0>(L, p, n, m, sv, f, acc[9]);
+                    fmm_dacc<2, 1, 0>(L, p, n, m, sv, f, acc[10]);
+                    fmm_dacc<1, 2, 0>(L, p, n, m, sv, f, acc[11]);
+                    fmm_dacc<0, 3, 0>(L, p, n, m, sv, f, acc[12]);
+                    fmm_dacc<2, 0, 1>(L, p, n, m, sv, f, acc[13]);
+                    fmm_dacc<1, 1, 1>(L, p, n, m, sv, f, acc[14]);
+                    fmm_dacc<0, 2, 1>(L, p, n, m, sv, f, acc[15]);
+                    fmm_dacc<1, 0, 2>(L, p, n, m, sv, f, acc[16]);
+                    fmm_dacc<0, 1, 2>(L, p, n, m, sv, f, acc[17]);
+                    fmm_dacc<0, 0, 3>(L, p, n, m, sv, f, acc[18]);
+                }
+            }
+        }
+    }
+    const long j = order[t];
+    if (MODE == 0) {
+        // the second set is read only when it is weighted (AMBER has one set)
+        double wx = c0 * ipd[3 * j], wy = c0 * ipd[3 * j + 1], wz = c0 * ipd[3 * j + 2];
+        if (c1 != 0.0) {
+            wx += c1 * ipd[n3 + 3 * j];
+            wy += c1 * ipd[n3 + 3 * j + 1];
+            wz += c1 * ipd[n3 + 3 * j + 2];
+        }
+        // acc: xx xy yy xz yz zz
+        out[j] = acc[0] * wx + acc[1] * wy + acc[3] * wz;
+        out[ostride + j] = acc[1] * wx + acc[2] * wy + acc[4] * wz;
+        out[2 * ostride + j] = acc[3] * wx + acc[4] * wy + acc[5] * wz;
+    } else {
+        out[j] = -acc[0];
+        out[ostride + j] = -acc[1];
+        out[2 * ostride + j] = -acc[2];
+        if (MODE == 1) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) out[(3 + c) * ostride + j] = acc[3 + c];
+            const double *Q = q + 10L * t + 4;  // xx xy yy xz yz zz
+            // third derivatives: 9 xxx, 10 xxy, 11 xyy, 12 yyy, 13 xxz, 14 xyz, 15 yyz, 16 xzz, 17 yzz, 18 zzz
+            const double *T = acc + 9;
+            const double hx = Q[0] * T[0] + Q[2] * T[2] + Q[5] * T[7] + 2.0 * (Q[1] * T[1] + Q[3] * T[4] + Q[4] * T[5]);
+            const double hy = Q[0] * T[1] + Q[2] * T[3] + Q[5] * T[8] + 2.0 * (Q[1] * T[2] + Q[3] * T[5] + Q[4] * T[6]);
+            const double hz = Q[0] * T[4] + Q[2] * T[6] + Q[5] * T[9] + 2.0 * (Q[1] * T[5] + Q[3] * T[7] + Q[4] * T[8]);
+            out[9 * ostride + j] = -hx;
+            out[10 * ostride + j] = -hy;
+            out[11 * ostride + j] = -hz;
+        }
+    }
+}
+
+// Near field of the gradient at the polarizable sites (sorted target t,
+// blockIdx.y = neighbour leaf): static multipoles contracted with w = 1/2 (mu_D +
+// mu_P) (AMOEBA) or mu, and the dipole pairs 1/2 [mu_D,j . Egrd(mu_P) + mu_P,j .
+// Egrd(mu_D)] (mu_j . Egrd(mu)); xs = sorted [D or mu][npol][3] (+ [P][npol][3]).
+// Output part[nb][3][npol] at the site's own index.
+template <bool AMOEBA>
+__global__ __launch_bounds__(256) void k_fmm_gnear_pol(int npol, int nside, const double4 *__restrict__ ppos,
+                                                       const double *__restrict__ pa, const int *__restrict__ pleaf,
+                                                       const int *__restrict__ porder,
+                                                       const int *__restrict__ pmmself,
+                                                       const int *__restrict__ pleaf_start,
+                                                       const int *__restrict__ mleaf_start,
+                                                       const double4 *__restrict__ mpos,
+                                                       const double *__restrict__ ma, const double *__restrict__ mq,
+                                                       const double *__restrict__ xs, double *__restrict__ part) {
+    constexpr int LD = AMOEBA ? 10 : 1;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= npol) return;
+    const int nb = blockIdx.y;
+    const long n3 = 3L * npol;
+    const long cell = pleaf[t];
+    const int ix = (int)(cell % nside) + nb % 3 - 1, iy = (int)((cell / nside) % nside) + (nb / 3) % 3 - 1,
+              iz = (int)(cell / ((long)nside * nside)) + nb / 9 - 1;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < nside && iy < nside && iz < nside) {
+        const long leaf = ((long)iz * nside + iy) * nside + ix;
+        const double4 pj = ppos[t];
+        const double aj = pa[t];
+        const double ax = xs[3L * t], ay = xs[3L * t + 1], az = xs[3L * t + 2];
+        const double bx = AMOEBA ? xs[n3 + 3L * t] : ax, by = AMOEBA ? xs[n3 + 3L * t + 1] : ay,
+                     bz = AMOEBA ? xs[n3 + 3L * t + 2] : az;
+        const double wx = AMOEBA ? 0.5 * (ax + bx) : ax, wy = AMOEBA ? 0.5 * (ay + by) : ay,
+                     wz = AMOEBA ? 0.5 * (az + bz) : az;
+        const int mself = pmmself[t];
+        for (int s = mleaf_start[leaf]; s < mleaf_start[leaf + 1]; s++) {
+            if (s == mself) continue;
+            const double4 pi = mpos[s];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, ma[s], aj, k2, k3, k4, k5);
+            double q[LD];
+#pragma unroll
+            for (int c = 0; c < LD; c++) q[c] = mq[(long)LD * s + c];
+            static_grad_pair<AMOEBA>(dx, dy, dz, k2, k3, k4, k5, q, wx, wy, wz, gx, gy, gz);
+        }
+        const double f = AMOEBA ? 0.5 : 1.0;
+        for (int s = pleaf_start[leaf]; s < pleaf_start[leaf + 1]; s++) {
+            if (s == t) continue;
+            const double4 pi = ppos[s];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, pa[s], aj, k2, k3, k4, k5);
+            if (AMOEBA) {
+                dip_grad_pair(dx, dy, dz, k3, k4, ax, ay, az, xs[n3 + 3L * s], xs[n3 + 3L * s + 1],
+                              xs[n3 + 3L * s + 2], f, gx, gy, gz);
+                dip_grad_pair(dx, dy, dz, k3, k4, bx, by, bz, xs[3L * s], xs[3L * s + 1], xs[3L * s + 2], f, gx, gy,
+                              gz);
+            } else {
+                dip_grad_pair(dx, dy, dz, k3, k4, ax, ay, az, xs[3L * s], xs[3L * s + 1], xs[3L * s + 2], f, gx, gy,
+                              gz);
+            }
+        }
+    }
+    const long j = porder[t];
+    double *o = part + (long)nb * n3;
+    o[j] = gx;
+    o[npol + j] = gy;
+    o[2L * npol + j] = gz;
+}
+
+// Near field at the MM atoms (sorted target t, blockIdx.y = neighbour leaf): E,
+// Egrd and Q : EHes (AMOEBA) or E of the dipoles nu (sorted, nu[npol][3]) of the
+// polarizable sites around; the atom's own site is skipped.  Output
+// part[nb][na][nmm] at the atom's own index.
+template <bool AMOEBA>
+__global__ __launch_bounds__(256) void k_fmm_gnear_mm(int nmm, int nside, const double4 *__restrict__ mpos,
+                                                      const double *__restrict__ ma, const double *__restrict__ mq,
+                                                      const int *__restrict__ mleaf, const int *__restrict__ morder,
+                                                      const int *__restrict__ pleaf_start,
+                                                      const double4 *__restrict__ ppos, const double *__restrict__ pa,
+                                                      const int *__restrict__ pmmself, const double *__restrict__ nu,
+                                                      double *__restrict__ part) {
+    constexpr int NA = AMOEBA ? 12 : 3;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nmm) return;
+    const int nb = blockIdx.y;
+    const long cell = mleaf[t];
+    const int ix = (int)(cell % nside) + nb % 3 - 1, iy = (int)((cell / nside) % nside) + (nb / 3) % 3 - 1,
+              iz = (int)(cell / ((long)nside * nside)) + nb / 9 - 1;
+    double acc[NA];
+#pragma unroll
+    for (int c = 0; c < NA; c++) acc[c] = 0.0;
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ix < nside && iy < nside && iz < nside) {
+        const long leaf = ((long)iz * nside + iy) * nside + ix;
+        const double4 pm = mpos[t];
+        const double am = ma[t];
+        double Q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (AMOEBA)
+            for (int c = 0; c < 6; c++) Q[c] = mq[10L * t + 4 + c];
+        const double trQ = Q[0] + Q[2] + Q[5];
+        for (int s = pleaf_start[leaf]; s < pleaf_start[leaf + 1]; s++) {
+            if (pmmself[s] == t) continue;
+            const double4 pi = ppos[s];
+            const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pm.w, pa[s], am, k2, k3, k4, k5);
+            dip_at_mm<AMOEBA>(ex, ey, ez, k2, k3, k4, k5, nu[3L * s], nu[3L * s + 1], nu[3L * s + 2], Q, trQ, acc);
+        }
+    }
+    const long m = morder[t];
+    double *o = part + (long)nb * NA * nmm;
+#pragma unroll
+    for (int c = 0; c < NA; c++) o[(long)c * nmm + m] = acc[c];
+}
+
+// Sorted dipole columns of the gradient: xs[0] = mu_D (or mu), xs[1] = mu_P,
+// xs[2] = 1/2 (mu_D + mu_P) (AMOEBA); ipd = [set][npol][3].
+__global__ void k_fmm_gather_ipd(int n, bool amoeba, const int *__restrict__ order, const double *__restrict__ ipd,
+                                 double *__restrict__ xs) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    const long j = order[t], n3 = 3L * n;
+    for (int k = 0; k < 3; k++) {
+        const double d = ipd[3 * j + k];
+        xs[3L * t + k] = d;
+        if (amoeba) {
+            const double pv = ipd[n3 + 3 * j + k];
+            xs[n3 + 3L * t + k] = pv;
+            xs[2 * n3 + 3L * t + k] = 0.5 * (d + pv);
+        }
+    }
+}
+
 // ------------------------------------------------------------- driver
 static int p2m_blocks(int p, int nleaf) {
     const int per = 256 / ((p + 1) * (p + 2) / 2);
@@ -609,33 +924,43 @@ static size_t lds_m2l(int p) {
     return sizeof(double2) * m2l_groups(p) * ((p + 1) * (p + 1) + (2 * p + 1) * (2 * p + 1));
 }
 
-// Far field of one source set (already expanded into the leaf multipoles at
-// level L) at the polarizable sites: M2M, M2L, L2L, L2P into field[npol][3].
+// Local expansions at the leaves of one source set (already expanded into the
+// leaf multipoles at level L): M2M, M2L, L2L; cells with no target (cnt_tgt)
+// are skipped.  False when the tree has no far field (L < 2).
+static bool far_locals(hipStream_t st, const FmmDev &f, int p, const int *cnt_src, const int *cnt_tgt) {
+    const int L = f.levels;
+    const int ncoef = (p + 1) * (p + 1);
+    if (L < 2) return false;
+    for (int l = L - 1; l >= 2; l--) {
+        const int ns = 1 << l;
+        hipLaunchKernelGGL(k_fmm_m2m, dim3(ns * ns * ns), dim3(256), sizeof(double2) * ncoef, st, p, ns,
+                           cnt_src + f.cell_off[l + 1], f.mexp + f.cell_off[l + 1] * ncoef,
+                           f.tab_m2m + (long)l * 8 * f.ncoef_tab, f.ncoef_tab, f.mexp + f.cell_off[l] * ncoef);
+        HIP_CHECK(hipGetLastError());
+    }
+    for (int l = 2; l <= L; l++) {
+        const int ns = 1 << l;
+        hipLaunchKernelGGL(k_fmm_m2l, dim3(ns * ns * ns), dim3(256), lds_m2l(p), st, p, ns, cnt_src + f.cell_off[l],
+                           cnt_tgt + f.cell_off[l], f.mexp + f.cell_off[l] * ncoef,
+                           f.tab_m2l + (long)l * 343 * f.ncoef2_tab, f.ncoef2_tab, f.lexp + f.cell_off[l] * ncoef);
+        HIP_CHECK(hipGetLastError());
+    }
+    for (int l = 2; l < L; l++) {
+        const int ns = 1 << (l + 1);
+        hipLaunchKernelGGL(k_fmm_l2l, dim3(ns * ns * ns), dim3(256), sizeof(double2) * ncoef, st, p, ns,
+                           cnt_tgt + f.cell_off[l + 1], f.lexp + f.cell_off[l] * ncoef,
+                           f.tab_l2l + (long)l * 8 * f.ncoef_tab, f.ncoef_tab, f.lexp + f.cell_off[l + 1] * ncoef);
+        HIP_CHECK(hipGetLastError());
+    }
+    return true;
+}
+
+// Far field of one source set at the polarizable sites: local expansions, then
+// L2P into field[npol][3].
 static void far_field(hipStream_t st, const FmmDev &f, int p, const int *cnt_src, double *field) {
     const int L = f.levels;
     const int ncoef = (p + 1) * (p + 1);
-    if (L >= 2) {
-        for (int l = L - 1; l >= 2; l--) {
-            const int ns = 1 << l;
-            hipLaunchKernelGGL(k_fmm_m2m, dim3(ns * ns * ns), dim3(256), sizeof(double2) * ncoef, st, p, ns,
-                               cnt_src + f.cell_off[l + 1], f.mexp + f.cell_off[l + 1] * ncoef,
-                               f.tab_m2m + (long)l * 8 * f.ncoef_tab, f.ncoef_tab, f.mexp + f.cell_off[l] * ncoef);
-            HIP_CHECK(hipGetLastError());
-        }
-        for (int l = 2; l <= L; l++) {
-            const int ns = 1 << l;
-            hipLaunchKernelGGL(k_fmm_m2l, dim3(ns * ns * ns), dim3(256), lds_m2l(p), st, p, ns,
-                               cnt_src + f.cell_off[l], f.cnt_pol + f.cell_off[l], f.mexp + f.cell_off[l] * ncoef,
-                               f.tab_m2l + (long)l * 343 * f.ncoef2_tab, f.ncoef2_tab, f.lexp + f.cell_off[l] * ncoef);
-            HIP_CHECK(hipGetLastError());
-        }
-        for (int l = 2; l < L; l++) {
-            const int ns = 1 << (l + 1);
-            hipLaunchKernelGGL(k_fmm_l2l, dim3(ns * ns * ns), dim3(256), sizeof(double2) * ncoef, st, p, ns,
-                               f.cnt_pol + f.cell_off[l + 1], f.lexp + f.cell_off[l] * ncoef,
-                               f.tab_l2l + (long)l * 8 * f.ncoef_tab, f.ncoef_tab, f.lexp + f.cell_off[l + 1] * ncoef);
-            HIP_CHECK(hipGetLastError());
-        }
+    if (far_locals(st, f, p, cnt_src, f.cnt_pol)) {
         hipLaunchKernelGGL(k_fmm_l2p, dim3((f.npol + 255) / 256), dim3(256), 0, st, p, f.npol, 1 << L, f.leaf_h,
                            f.origin, f.pol_pos, f.pol_leaf, f.pol_order, f.lexp + f.cell_off[L] * ncoef, field);
         HIP_CHECK(hipGetLastError());
@@ -666,7 +991,7 @@ void fmm_launch_matvec_fields(hipStream_t st, const FmmDev &f, bool amoeba, int 
     for (int c = 0; c < M; c++) {
         if (L >= 2) {
             hipLaunchKernelGGL(k_fmm_p2m<0>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h,
-                               f.origin, nleaf, f.pol_leaf_start, f.pol_pos, f.xs + c * n3, 3,
+                               f.origin, nleaf, f.pol_leaf_start, f.pol_pos, f.xs + c * n3, 3, 0,
                                f.mexp + f.cell_off[L] * ncoef);
             HIP_CHECK(hipGetLastError());
         }
@@ -689,10 +1014,81 @@ void fmm_launch_static_fields(hipStream_t st, const FmmDev &f, bool amoeba, int 
     HIP_CHECK(hipGetLastError());
     if (L >= 2) {
         hipLaunchKernelGGL(k_fmm_p2m<1>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin,
-                           nleaf, f.mm_leaf_start, f.mm_pos, f.mm_q, ld, f.mexp + f.cell_off[L] * ncoef);
+                           nleaf, f.mm_leaf_start, f.mm_pos, f.mm_q, ld, 0, f.mexp + f.cell_off[L] * ncoef);
         HIP_CHECK(hipGetLastError());
     }
     far_field(st, f, p, f.cnt_mm, part + (long)kFmmNearSplits * n3);
+}
+
+void fmm_launch_grad_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, const double *ipd,
+                            double *part_pol, double *part_mm) {
+    const int L = f.levels, ns = 1 << L, nleaf = ns * ns * ns;
+    const int npol = f.npol, nmm = f.nmm;
+    const long n3 = 3L * npol;
+    const int na = amoeba ? 12 : 3;
+    if (ld != (amoeba ? 10 : 1)) {
+        std::fprintf(stderr, "fast multipole gradient: unexpected multipole width %d\n", ld);
+        std::abort();
+    }
+    hipLaunchKernelGGL(k_fmm_gather_ipd, dim3((npol + 255) / 256), dim3(256), 0, st, npol, amoeba, f.pol_order, ipd,
+                       f.xs);
+    HIP_CHECK(hipGetLastError());
+    const double *nu = amoeba ? f.xs + 2 * n3 : f.xs;
+    const dim3 gp((npol + 255) / 256, kFmmNearSplits), gm((nmm + 255) / 256, kFmmNearSplits);
+    if (amoeba) {
+        hipLaunchKernelGGL(k_fmm_gnear_pol<true>, gp, dim3(256), 0, st, npol, ns, f.pol_pos, f.pol_a, f.pol_leaf,
+                           f.pol_order, f.pol_mmself, f.pol_leaf_start, f.mm_leaf_start, f.mm_pos, f.mm_a, f.mm_q,
+                           f.xs, part_pol);
+        hipLaunchKernelGGL(k_fmm_gnear_mm<true>, gm, dim3(256), 0, st, nmm, ns, f.mm_pos, f.mm_a, f.mm_q, f.mm_leaf,
+                           f.mm_order, f.pol_leaf_start, f.pol_pos, f.pol_a, f.pol_mmself, nu, part_mm);
+    } else {
+        hipLaunchKernelGGL(k_fmm_gnear_pol<false>, gp, dim3(256), 0, st, npol, ns, f.pol_pos, f.pol_a, f.pol_leaf,
+                           f.pol_order, f.pol_mmself, f.pol_leaf_start, f.mm_leaf_start, f.mm_pos, f.mm_a, f.mm_q,
+                           f.xs, part_pol);
+        hipLaunchKernelGGL(k_fmm_gnear_mm<false>, gm, dim3(256), 0, st, nmm, ns, f.mm_pos, f.mm_a, f.mm_q, f.mm_leaf,
+                           f.mm_order, f.pol_leaf_start, f.pol_pos, f.pol_a, f.pol_mmself, nu, part_mm);
+    }
+    HIP_CHECK(hipGetLastError());
+    const int npass = amoeba ? 2 : 1;
+    double *far_pol = part_pol + (long)kFmmNearSplits * n3;
+    double *far_mm = part_mm + (long)kFmmNearSplits * na * nmm;
+    if (L < 2) {
+        HIP_CHECK(hipMemsetAsync(far_pol, 0, sizeof(double) * npass * n3, st));
+        HIP_CHECK(hipMemsetAsync(far_mm, 0, sizeof(double) * na * nmm, st));
+        return;
+    }
+    const int p = f.p_tab;
+    const int ncoef = (p + 1) * (p + 1);
+    double2 *leafm = f.mexp + f.cell_off[L] * ncoef;
+    const double2 *leafl = f.lexp + f.cell_off[L] * ncoef;
+    const dim3 lp((npol + 255) / 256), lm((nmm + 255) / 256);
+    // polarizable targets: static multipoles + one dipole set per pass, contracted
+    // with half of the other set (AMOEBA: P sources against D / 2, D against P / 2)
+    for (int pass = 0; pass < npass; pass++) {
+        hipLaunchKernelGGL(k_fmm_p2m<1>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin,
+                           nleaf, f.mm_leaf_start, f.mm_pos, f.mm_q, ld, 0, leafm);
+        const double *dip = amoeba ? f.xs + (pass == 0 ? n3 : 0) : f.xs;
+        hipLaunchKernelGGL(k_fmm_p2m<0>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin,
+                           nleaf, f.pol_leaf_start, f.pol_pos, dip, 3, 1, leafm);
+        HIP_CHECK(hipGetLastError());
+        far_locals(st, f, p, f.cnt_mm, f.cnt_pol);
+        const double c0 = amoeba ? (pass == 0 ? 0.5 : 0.0) : 1.0, c1 = amoeba ? (pass == 0 ? 0.0 : 0.5) : 0.0;
+        hipLaunchKernelGGL(k_fmm_l2p_grad<0>, lp, dim3(256), 0, st, p, npol, ns, f.leaf_h, f.origin, f.pol_pos,
+                           f.pol_leaf, f.pol_order, leafl, ipd, n3, c0, c1, f.mm_q, far_pol + pass * n3, (long)npol);
+        HIP_CHECK(hipGetLastError());
+    }
+    // MM targets: the dipoles nu
+    hipLaunchKernelGGL(k_fmm_p2m<0>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin, nleaf,
+                       f.pol_leaf_start, f.pol_pos, nu, 3, 0, leafm);
+    HIP_CHECK(hipGetLastError());
+    far_locals(st, f, p, f.cnt_pol, f.cnt_mm);
+    if (amoeba)
+        hipLaunchKernelGGL(k_fmm_l2p_grad<1>, lm, dim3(256), 0, st, p, nmm, ns, f.leaf_h, f.origin, f.mm_pos,
+                           f.mm_leaf, f.mm_order, leafl, ipd, n3, 0.0, 0.0, f.mm_q, far_mm, (long)nmm);
+    else
+        hipLaunchKernelGGL(k_fmm_l2p_grad<2>, lm, dim3(256), 0, st, p, nmm, ns, f.leaf_h, f.origin, f.mm_pos,
+                           f.mm_leaf, f.mm_order, leafl, ipd, n3, 0.0, 0.0, f.mm_q, far_mm, (long)nmm);
+    HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace ommp

@@ -550,20 +550,25 @@ void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb, hipEvent
         HIP_CHECK(hipGetLastError());
     }
     if (eve) HIP_CHECK(hipEventRecord(eve, st));
+    launch_geomgrad_finish(st, a, a.split_s + a.split_p, a.split_m);
+}
+
+void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm) {
+    const int nmm = a.nmm, npol = a.npol;
+    const int pb = (npol + kBlock - 1) / kBlock, mb = (nmm + kBlock - 1) / kBlock;
     if (npol > 0) {
-        const int np = a.split_s + a.split_p;
         if (a.amoeba) {
-            hipLaunchKernelGGL(k_gpol_finalize<true>, dim3(pb), dim3(kBlock), 0, st, npol, np, a.part_pol, a.ms,
-                               a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci, a.pp.f,
-                               a.gpol);
-            hipLaunchKernelGGL(k_gmm_finalize<true>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, a.split_m, a.part_mm,
+            hipLaunchKernelGGL(k_gpol_finalize<true>, dim3(pb), dim3(kBlock), 0, st, npol, nparts_pol, a.part_pol,
+                               a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci,
+                               a.pp.f, a.gpol);
+            hipLaunchKernelGGL(k_gmm_finalize<true>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, nparts_mm, a.part_mm,
                                a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci, a.mp.f2, a.emm, a.egrd,
                                a.gtr);
         } else {
-            hipLaunchKernelGGL(k_gpol_finalize<false>, dim3(pb), dim3(kBlock), 0, st, npol, np, a.part_pol, a.ms,
-                               a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci, a.pp.f,
-                               a.gpol);
-            hipLaunchKernelGGL(k_gmm_finalize<false>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, a.split_m,
+            hipLaunchKernelGGL(k_gpol_finalize<false>, dim3(pb), dim3(kBlock), 0, st, npol, nparts_pol, a.part_pol,
+                               a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci,
+                               a.pp.f, a.gpol);
+            hipLaunchKernelGGL(k_gmm_finalize<false>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, nparts_mm,
                                a.part_mm, a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci, a.mp.f2, a.emm,
                                a.egrd, a.gtr);
         }

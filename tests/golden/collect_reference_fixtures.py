@@ -2,8 +2,9 @@
 
 These are data files the reference's ctest suite holds (inputs, external
 fields, golden induced dipoles and energies, prepared-state summaries, and the
-FULL_POTENTIAL.ref energy/gradient listings of the AMOEBA inputs), stored
-gzipped.  Run once in a container that has the reference tree mounted:
+FULL_POTENTIAL.ref energy/gradient listings of the AMOEBA inputs; the 1UBQ one
+is what the reference's 1UBQ_AMOEBA_XYZ_geomgrad test checks its fast multipole
+gradient against), stored gzipped.  Run once in a container that has the reference tree mounted:
 
     python tests/golden/collect_reference_fixtures.py /root/reference
 
@@ -35,6 +36,7 @@ FILES = {
         "input_AMOEBA.mmp", "input_WANG_AL.mmp", "EF_1.txt",
         "IPD_0_AMOEBA.ref", "IPD_1_AMOEBA.ref", "IPD_0_WANG_AL.ref", "IPD_1_WANG_AL.ref",
         "ENE_0_AMOEBA.ref", "ENE_1_AMOEBA.ref", "ENE_0_WANG_AL.ref", "ENE_1_WANG_AL.ref",
+        "FULL_POTENTIAL.ref",
     ],
 }
 

@@ -123,3 +123,87 @@ def test_fmm_settings_take_effect(ubq):
     again = np.array(s.e_m2d)
     assert np.array_equal(again, direct)
     assert _rel(high, direct) < 0.1 * _rel(low, direct)
+
+
+def _oracle_grad(prep, ipd):
+    import oracle_api as oa
+    return oa.OracleSystem(prep).polelec_geomgrad(ipd)
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_fmm_gradient_converges_1ubq(ff, fixture_file):
+    """With fast multipoles on, ommp_polelec_geomgrad sums the pair terms through
+    the tree (near field direct, far field from derivatives of the local
+    expansions up to third order).  Against the oracle's gradient for the same
+    dipoles, the error falls steadily with the expansion order."""
+    from openmmpol_amd import OMMPSystem
+    path = fixture_file("1ubq", f"input_{ff}.mmp")
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path("1ubq", f"input_{ff}.mmp"))
+    s = OMMPSystem(path)
+    errs = []
+    for lmax, tol in ((8, 1e-3), (12, 1e-4), (18, 5e-6)):
+        s.enable_fmm(lmax=lmax, lmax_pol=lmax)
+        s.set_external_field(np.zeros((s.pol_atoms, 3)))
+        g = s.polelec_geomgrad()
+        want = _oracle_grad(prep, s.ipd.copy())
+        err = _rel(g, want)
+        print(f"1ubq {ff} FMM gradient lmax {lmax}: rel err {err:.3e}", flush=True)
+        assert err < tol
+        errs.append(err)
+    assert all(b < 0.5 * a for a, b in zip(errs, errs[1:]))
+    s.terminate()
+
+
+def test_fmm_gradient_matches_reference_listing_1ubq(fixture_file):
+    """The reference's 1UBQ_AMOEBA_XYZ_geomgrad test: the gradient with fast
+    multipoles of order 18 and an 8 angstrom near field (tests/1ubq_amoeba_xyz.json)
+    against the 'Grad EP' listing of tests/1ubq/FULL_POTENTIAL.ref, which it
+    checks at rtol 1e-3, atol 1e-4 (kcal/mol/angstrom); ours also to 1e-5 of the
+    largest component."""
+    from openmmpol_amd import OMMPSystem
+    s = OMMPSystem(fixture_file("1ubq", "input_AMOEBA.mmp"))
+    s.enable_fmm(lmax=18, lmax_pol=18, distance=8.0 * mmp_io.ANG2AU)
+    s.set_solver_tolerance(1e-12, 500)
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    want = mmp_io.read_grad(mmp_io.fixture_path("1ubq", "FULL_POTENTIAL.ref"), "EP")
+    got = s.polelec_geomgrad() * (mmp_io.AU2KCALMOL * mmp_io.ANG2AU)
+    err = _rel(got, want)
+    print(f"1ubq FMM gradient vs reference listing: rel err {err:.3e}", flush=True)
+    assert np.isclose(got, want, rtol=1e-3, atol=1e-4).all()
+    assert err < 1e-5
+    s.terminate()
+
+
+def test_fmm_gradient_on_a_small_system_is_the_direct_sum(fixture_file):
+    """N-methylacetamide fits in one leaf: the FMM gradient is the near field only
+    and equals the direct sweeps up to summation order."""
+    from openmmpol_amd import OMMPSystem
+    path = fixture_file("N-methylacetamide", "input_AMOEBA.mmp")
+    a = OMMPSystem(path)
+    b = OMMPSystem(path)
+    b.enable_fmm()
+    for s in (a, b):
+        s.set_solver_tolerance(1e-11, 500)
+        s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    ga, gb = a.polelec_geomgrad(), b.polelec_geomgrad()
+    assert _rel(gb, ga) < 1e-11
+    a.terminate()
+    b.terminate()
+
+
+def test_fmm_gradient_sphere15000():
+    """15,000 sites (8 x 8 x 8 leaves): FMM gradient of order 12 against the
+    oracle's for the same dipoles, and bitwise repeatable."""
+    from openmmpol_amd import synth
+    spec = synth.sphere_system(15000, 33.0, seed=2)
+    s = synth.make_system(spec)
+    s.enable_fmm(lmax=12, lmax_pol=12)
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    g = s.polelec_geomgrad()
+    assert np.array_equal(g, s.polelec_geomgrad())
+    print("sphere 15000 FMM gradient: computing the oracle's (about 15 s)", flush=True)
+    want = _oracle_grad(synth.oracle_prep(spec), s.ipd.copy())
+    err = _rel(g, want)
+    print(f"sphere 15000 FMM gradient lmax 12: rel err {err:.3e}", flush=True)
+    assert err < 2e-4
+    s.terminate()

@@ -42,7 +42,7 @@ def test_gradient_matches_oracle_on_fixtures(name, ff, fixture_file):
     s.terminate()
 
 
-@pytest.mark.parametrize("name,tol", [("N-methylacetamide", 2e-6), ("1crn", 5e-6)])
+@pytest.mark.parametrize("name,tol", [("N-methylacetamide", 2e-6), ("1crn", 5e-6), ("1ubq", 2e-6)])
 def test_gradient_matches_reference_listing(name, tol, fixture_file):
     from openmmpol_amd import OMMPSystem
     s = OMMPSystem(fixture_file(name, "input_AMOEBA.mmp"))

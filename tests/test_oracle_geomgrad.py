@@ -6,7 +6,7 @@ mod_geomgrad.F90:113-273, with rotation_geomgrad, rotate_multipoles.F90:3-230).
   AMOEBA and Wang-AL inputs and a synthetic cluster that uses every frame
   convention 0-5;
 - the 'Grad EP' listings of the reference's FULL_POTENTIAL.ref fixtures
-  (N-methylacetamide and 1CRN, AMOEBA; kcal/mol/angstrom, converted with the
+  (N-methylacetamide, 1CRN and 1UBQ, AMOEBA; kcal/mol/angstrom, converted with the
   reference's constants); those listings come from a separately converged
   dipole set, so they agree to ~1e-6 relative, which still fails any dropped
   term, wrong sign or wrong factor by orders of magnitude;
@@ -58,7 +58,7 @@ def test_nma_gradient_matches_finite_differences(ff):
     assert np.abs(g - fd).max() < 1e-9 * max(1.0, np.abs(g).max() / 1e-3)
 
 
-@pytest.mark.parametrize("name,tol", [("N-methylacetamide", 2e-6), ("1crn", 5e-6)])
+@pytest.mark.parametrize("name,tol", [("N-methylacetamide", 2e-6), ("1crn", 5e-6), ("1ubq", 2e-6)])
 def test_gradient_matches_reference_listing(name, tol):
     prep = mmp_io.prepare_mmp(mmp_io.fixture_path(name, "input_AMOEBA.mmp"))
     o, ipd, ep = _solve(prep, prep["cmm"], tol=1e-12)
