@@ -461,8 +461,11 @@ void ensure_fmm(System &s, DeviceState &d) {
     f.mexp = fmm_alloc<double2>(d, ncell * f.ncoef_tab);
     f.lexp = fmm_alloc<double2>(d, ncell * f.ncoef_tab);
     f.xs = fmm_alloc<double>(d, 9L * npol);
+    f.mstatic = fmm_alloc<double2>(d, std::max<size_t>((size_t)(h.cell_off.back() - h.cell_off[h.levels]), 1) *
+                                          f.ncoef_tab);
     d.fmm_part = fmm_alloc<double>(d, (size_t)(kFmmNearSplits + 1) * 2 * 3 * npol);
     d.fmm_rhs_part = fmm_alloc<double>(d, (size_t)(kFmmNearSplits + 1) * 3 * npol);
+    fmm_launch_static_moments(d.stream, f, ld);
     d.fmm_valid = true;
 }
 

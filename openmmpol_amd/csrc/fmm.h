@@ -46,7 +46,13 @@ struct FmmDev {
     double2 *tab_l2l = nullptr;    // [level][8][ncoef_tab]   S(c_child - c_parent), parent level
     double2 *tab_m2l = nullptr;    // [level][343][ncoef2_tab] T(c_source - c_target)
     double *xs = nullptr;          // sorted dipole columns [3][npol][3]
+    double2 *mstatic = nullptr;    // [nleaf][ncoef_tab] static multipole moments of the leaves (order p_tab)
 };
+
+// Moments of the static multipoles in every leaf, order p_tab, into f.mstatic:
+// computed once per tree (the multipoles change only with the geometry) and
+// used by the static field and the gradient.  Nothing to do without a far field.
+void fmm_launch_static_moments(hipStream_t st, const FmmDev &f, int ld);
 
 // Near (splits 0 .. 26, one per neighbour leaf) and far (split 27) dipole fields
 // of the M columns x at the polarizable sites, in the partial layout of the
@@ -54,7 +60,7 @@ struct FmmDev {
 void fmm_launch_matvec_fields(hipStream_t st, const FmmDev &f, bool amoeba, int M, const double *x, long xstride,
                               double *part);
 // Near (splits 0 .. 26) and far (split 27) fields of the static multipoles at
-// the polarizable sites: part[split * 3 npol + 3 j + c].
+// the polarizable sites: part[split * 3 npol + 3 j + c].  Needs f.mstatic.
 void fmm_launch_static_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, double *part);
 // Pair sums of the polarization-energy gradient, near (splits 0 .. 26) and far,
 // in the partial layouts of the direct gradient sweeps:
@@ -66,6 +72,7 @@ void fmm_launch_static_fields(hipStream_t st, const FmmDev &f, bool amoeba, int 
 //     split 27.
 // Every pair within the 27 leaves around a target is in the near field with
 // scale 1 and the damped kernels; ipd = [D or mu][npol][3] (+ [P][npol][3]).
+// Needs f.mstatic.
 void fmm_launch_grad_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, const double *ipd,
                             double *part_pol, double *part_mm);
 constexpr int fmm_grad_parts_pol(bool amoeba) { return kFmmNearSplits + (amoeba ? 2 : 1); }

@@ -914,6 +914,17 @@ __global__ void k_fmm_gather_ipd(int n, bool amoeba, const int *__restrict__ ord
     }
 }
 
+// dst[leaf][c] = src[leaf][c] for the first n coefficients (orders up to that of
+// dst); src has sstride coefficients per leaf.
+__global__ void k_fmm_leafcopy(long total, int n, int sstride, const double2 *__restrict__ src,
+                               double2 *__restrict__ dst) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const long leaf = i / n;
+    const int c = (int)(i - leaf * n);
+    dst[i] = src[leaf * sstride + c];
+}
+
 // ------------------------------------------------------------- driver
 static int p2m_blocks(int p, int nleaf) {
     const int per = 256 / ((p + 1) * (p + 2) / 2);
@@ -922,6 +933,15 @@ static int p2m_blocks(int p, int nleaf) {
 
 static size_t lds_m2l(int p) {
     return sizeof(double2) * m2l_groups(p) * ((p + 1) * (p + 1) + (2 * p + 1) * (2 * p + 1));
+}
+
+// Leaf moments of the static multipoles at order p (<= p_tab) from the cache.
+static void leaf_static(hipStream_t st, const FmmDev &f, int p, double2 *leafm) {
+    const int ns = 1 << f.levels;
+    const long n = (long)(p + 1) * (p + 1), total = (long)ns * ns * ns * n;
+    hipLaunchKernelGGL(k_fmm_leafcopy, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, (int)n,
+                       f.ncoef_tab, f.mstatic, leafm);
+    HIP_CHECK(hipGetLastError());
 }
 
 // Local expansions at the leaves of one source set (already expanded into the
@@ -1000,7 +1020,7 @@ void fmm_launch_matvec_fields(hipStream_t st, const FmmDev &f, bool amoeba, int 
 }
 
 void fmm_launch_static_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, double *part) {
-    const int L = f.levels, ns = 1 << L, nleaf = ns * ns * ns;
+    const int L = f.levels, ns = 1 << L;
     const int p = f.p_static;
     const int ncoef = (p + 1) * (p + 1);
     const long n3 = 3L * f.npol;
@@ -1012,12 +1032,16 @@ void fmm_launch_static_fields(hipStream_t st, const FmmDev &f, bool amoeba, int 
         hipLaunchKernelGGL((k_fmm_near<1, false>), ng, dim3(256), 0, st, f.npol, ns, f.pol_pos, f.pol_a, f.pol_leaf,
                            f.pol_order, f.pol_mmself, f.mm_leaf_start, f.mm_pos, f.mm_a, f.mm_q, ld, 1, part);
     HIP_CHECK(hipGetLastError());
-    if (L >= 2) {
-        hipLaunchKernelGGL(k_fmm_p2m<1>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin,
-                           nleaf, f.mm_leaf_start, f.mm_pos, f.mm_q, ld, 0, f.mexp + f.cell_off[L] * ncoef);
-        HIP_CHECK(hipGetLastError());
-    }
+    if (L >= 2) leaf_static(st, f, p, f.mexp + f.cell_off[L] * ncoef);
     far_field(st, f, p, f.cnt_mm, part + (long)kFmmNearSplits * n3);
+}
+
+void fmm_launch_static_moments(hipStream_t st, const FmmDev &f, int ld) {
+    const int L = f.levels, ns = 1 << L, nleaf = ns * ns * ns;
+    if (L < 2) return;
+    hipLaunchKernelGGL(k_fmm_p2m<1>, dim3(p2m_blocks(f.p_tab, nleaf)), dim3(256), 0, st, f.p_tab, ns, f.leaf_h,
+                       f.origin, nleaf, f.mm_leaf_start, f.mm_pos, f.mm_q, ld, 0, f.mstatic);
+    HIP_CHECK(hipGetLastError());
 }
 
 void fmm_launch_grad_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld, const double *ipd,
@@ -1065,8 +1089,7 @@ void fmm_launch_grad_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld
     // polarizable targets: static multipoles + one dipole set per pass, contracted
     // with half of the other set (AMOEBA: P sources against D / 2, D against P / 2)
     for (int pass = 0; pass < npass; pass++) {
-        hipLaunchKernelGGL(k_fmm_p2m<1>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin,
-                           nleaf, f.mm_leaf_start, f.mm_pos, f.mm_q, ld, 0, leafm);
+        leaf_static(st, f, p, leafm);
         const double *dip = amoeba ? f.xs + (pass == 0 ? n3 : 0) : f.xs;
         hipLaunchKernelGGL(k_fmm_p2m<0>, dim3(p2m_blocks(p, nleaf)), dim3(256), 0, st, p, ns, f.leaf_h, f.origin,
                            nleaf, f.pol_leaf_start, f.pol_pos, dip, 3, 1, leafm);
