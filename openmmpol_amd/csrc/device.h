@@ -24,6 +24,7 @@
 namespace ommp {
 
 constexpr int kBlock = 256;      // threads per block for every kernel here
+constexpr int kSymBlock = 256;   // sites per block of the pair-once matvec (= kBlock)
 constexpr int kMaxCols = 2;      // D and P sets
 constexpr double kEpsRpDev = 2.220446049250313080847e-16 * 100.0;
 
@@ -70,6 +71,21 @@ void launch_tmv_finalize64(hipStream_t st, bool amoeba, int npol, int M, const d
                            const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
                            const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
                            const int *skip_flag);
+// Same for FP32 (prec 32) or FP64 partials.
+void launch_tmv_finalize_prec(hipStream_t st, int prec, bool amoeba, int npol, int M, const void *part, int nsplit,
+                              const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
+                              const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
+                              const int *skip_flag);
+// T matvec on the fly with each unordered pair evaluated once (tmv_sym.hip):
+// pairs[] = the block pairs (I <= J) of kSymBlock-site blocks, row-major;
+// part holds tmv_sym_part_bytes(npol, M, prec).
+int tmv_sym_blocks(int npol);
+size_t tmv_sym_part_bytes(int npol, int M, int prec);
+void launch_tmv_sym(hipStream_t st, int prec, bool amoeba, int npol, int M, const int2 *pairs, const double4 *ps64,
+                    const double *pa64, const float4 *ps32, const float *pa32, const double *alpha,
+                    const CsrDev &pp, const double *x, long xstride, double *y, long ystride, bool dodiag,
+                    double *dotpart, void *part, const int *skip_flag, hipEvent_t evb = nullptr,
+                    hipEvent_t eve = nullptr);
 void launch_rhs_finalize(hipStream_t st, bool amoeba, int npol, int nsplit, const double *part, const double4 *ms64,
                          const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
                          const CsrDev &sp, double *e_m2d);

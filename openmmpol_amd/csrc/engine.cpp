@@ -28,7 +28,10 @@ namespace ommp {
             fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
     } while (0)
 
-enum MvKind { kOtf64 = 0, kOtf32 = 1, kPack32 = 2, kPack64 = 3, kFmm = 4 };
+// kOtf64 / kOtf32 pick the pair-once sweep (kSym*) or the per-target one
+// (kPlain*) by size (use_pair_once); the explicit kinds force one of them.
+enum MvKind { kOtf64 = 0, kOtf32 = 1, kPack32 = 2, kPack64 = 3, kFmm = 4, kSym64 = 5, kSym32 = 6, kPlain64 = 7,
+              kPlain32 = 8 };
 
 struct DeviceState {
     int device = 0;
@@ -56,6 +59,8 @@ struct DeviceState {
     CgState *cs_host = nullptr;
     double *scal_host = nullptr;
     void *mvpart = nullptr;
+    int2 *sym_pairs = nullptr;  // block pairs of the pair-once matvec, on first use
+    void *sympart = nullptr;
     size_t mvpart_bytes = 0;
     int nsplit = 1;
     double *rhs_part = nullptr;
@@ -469,6 +474,17 @@ void ensure_fmm(System &s, DeviceState &d) {
     d.fmm_valid = true;
 }
 
+// The pair-once sweep (tmv_sym.hip) evaluates half the pairs, at the price of
+// a workgroup per block pair and of its block-pair partial sums.  Measured on
+// MI355X against the per-target sweep (two columns): FP64 faster from 5,000
+// sites (-7%) to 100,000 (-25%); FP32 slower up to 10,000 sites (+14%), faster
+// from 15,000 (-14%; -15% at 100,000).  Used from those sizes while its partial
+// sums stay under 2 GiB (up to ~100,000 sites); otherwise the per-target sweep.
+bool use_pair_once(int npol, int prec) {
+    if (npol < (prec == 32 ? 12000 : 4000)) return false;
+    return tmv_sym_part_bytes(npol, 2, 64) <= (size_t(2) << 30);
+}
+
 void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double *y, bool dodiag,
             double *dotpart, const CgState *skip) {
     const long n = d.n;
@@ -480,8 +496,30 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
         return;
     }
     if (kind == kOtf64 || kind == kOtf32) {
-        const int pi = prof_begin(d, kind);
-        launch_tmv_otf(d.stream, kind == kOtf32 ? 32 : 64, s.amoeba, s.npol, M, d.ps64, d.pa64, d.ps32, d.pa32,
+        const bool sym = use_pair_once(s.npol, kind == kOtf32 ? 32 : 64);
+        kind = kind == kOtf32 ? (sym ? kSym32 : kPlain32) : (sym ? kSym64 : kPlain64);
+    }
+    if (kind == kSym64 || kind == kSym32) {
+        if (!d.sym_pairs) {
+            const int nb = tmv_sym_blocks(s.npol);
+            std::vector<int2> pr;
+            pr.reserve((size_t)nb * (nb + 1) / 2);
+            for (int I = 0; I < nb; I++)
+                for (int J = I; J < nb; J++) pr.push_back(make_int2(I, J));
+            d.sym_pairs = dalloc<int2>(d, pr.size());
+            upload(pr.data(), d.sym_pairs, sizeof(int2) * pr.size());
+            d.sympart = dmalloc(d, tmv_sym_part_bytes(s.npol, 2, 64));
+        }
+        const int pi = prof_begin(d, kind == kSym32 ? kOtf32 : kOtf64);
+        launch_tmv_sym(d.stream, kind == kSym32 ? 32 : 64, s.amoeba, s.npol, M, d.sym_pairs, d.ps64, d.pa64, d.ps32,
+                       d.pa32, d.alpha, d.pp, x, n, y, n, dodiag, dotpart, d.sympart,
+                       reinterpret_cast<const int *>(skip), pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
+                       pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
+        return;
+    }
+    if (kind == kPlain64 || kind == kPlain32) {
+        const int pi = prof_begin(d, kind == kPlain32 ? kOtf32 : kOtf64);
+        launch_tmv_otf(d.stream, kind == kPlain32 ? 32 : 64, s.amoeba, s.npol, M, d.ps64, d.pa64, d.ps32, d.pa32,
                        d.alpha, d.pp, x, n, y, n, dodiag, dotpart, d.mvpart, d.nsplit,
                        reinterpret_cast<const int *>(skip), pi >= 0 ? d.prof_pool[2 * pi] : nullptr,
                        pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
@@ -932,6 +970,10 @@ size_t device_footprint_estimate(const System &s) {
     b += 3 * 24 * sizeof(double) * (n / kBlock + 16);      // reduction scratch
     const int nsplit = tmv_split(s.npol);
     b += tmv_part_bytes(s.npol, 2, 64, nsplit) + sizeof(double) * n * 16;  // pair and static-field partials
+    if (use_pair_once(s.npol, 64)) {  // block-pair partials and list of the pair-once sweep
+        const size_t nb = tmv_sym_blocks(s.npol);
+        b += tmv_sym_part_bytes(s.npol, 2, 64) + sizeof(int2) * nb * (nb + 1) / 2;
+    }
     b += (size_t)(s.list_S_P_P.nnz() + s.list_S_P_D.nnz() + 2 * s.list_P_P.nnz() + s.list_P_P_dense.nnz()) * 24;
     return b + (1 << 20);
 }
@@ -1261,7 +1303,7 @@ const double *device_ipd(System &s) {
 
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat) {
     if (ncols < 1 || ncols > 2) fatal("tmatvec supports 1 or 2 columns");
-    if (kind < 0 || kind > 4) fatal("Unknown matvec kind");
+    if (kind < 0 || kind > 8) fatal("Unknown matvec kind");
     if (s.npol == 0) return 0.0;
     DeviceState &d = ensure_device(s);
     const long n = d.n;

@@ -89,14 +89,16 @@ __device__ __forceinline__ void dip_kernel(R r2, R ir, R ti, R tj, R ai, R aj, R
 // Geometry of one pair: d = r_j - r_i, undamped k2 = 1/r^3 and k3 = 3/r^5,
 // the damping argument (a u^3 for AMOEBA, u = r/s for Wang) and whether the
 // pair needs damping at all.
-template <typename R, bool AMOEBA>
+template <typename R, bool AMOEBA, bool GUARD = true>
 __device__ __forceinline__ void pair_geom(const typename V4<R>::t &pj, R aj, const typename V4<R>::t &pi, R ai,
                                           R &dx, R &dy, R &dz, R &k2, R &k3, R &arg, bool &near) {
     dx = pj.x - pi.x;
     dy = pj.y - pi.y;
     dz = pj.z - pi.z;
     const R r2 = fmaa(dz, dz, fmaa(dy, dy, dx * dx));
-    const R ir = r2 > R(0) ? fast_rsqrt(r2) : R(0);
+    // GUARD: a zero distance (the self pair) gives zero kernels; without it the
+    // caller guarantees distinct positions
+    const R ir = !GUARD || r2 > R(0) ? fast_rsqrt(r2) : R(0);
     const R ir2 = ir * ir;
     k2 = ir * ir2;
     k3 = (R(3) * k2) * ir2;

@@ -315,7 +315,8 @@ class OMMPSystem:
 
     def tmatvec(self, x, kind="otf64", repeat=1):
         """y = T x for x [ncols, pol_atoms, 3]; returns (y, mean ms per launch)."""
-        kinds = {"otf64": 0, "otf32": 1, "incore32": 2, "incore64": 3, "fmm": 4}
+        kinds = {"otf64": 0, "otf32": 1, "incore32": 2, "incore64": 3, "fmm": 4, "sym64": 5, "sym32": 6,
+                 "plain64": 7, "plain32": 8}
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, self.pol_atoms, 3)
         y = np.zeros_like(x)
         ms = lib().ommp_amd_tmatvec(self.handler, kinds[kind], x.shape[0], x.ctypes.data_as(c_dp),

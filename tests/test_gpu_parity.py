@@ -71,7 +71,8 @@ def test_tmatvec_kinds(name, ff, fixture_file):
     ref = np.stack([o.tmatvec(x[0]), o.tmatvec(x[1])])
     dense = (o.tmat() @ x.reshape(2, -1).T).T.reshape(2, -1, 3)
     scale = np.abs(ref).max()
-    for kind, tol in (("otf64", 1e-13), ("incore64", 1e-13), ("otf32", 2e-6), ("incore32", 2e-6)):
+    for kind, tol in (("otf64", 1e-13), ("incore64", 1e-13), ("otf32", 2e-6), ("incore32", 2e-6),
+                      ("sym64", 1e-13), ("plain64", 1e-13), ("sym32", 2e-6), ("plain32", 2e-6)):
         for ncols in (1, 2):
             y, _ = s.tmatvec(x[:ncols], kind=kind)
             want = dense if kind.startswith("incore") else ref
@@ -157,3 +158,28 @@ def test_energy_getter_solves_when_needed(fixture_file):
     ep = s.get_polelec_energy()
     eo = o.get_polelec_energy()
     assert abs(ep - eo) <= 1e-10 * abs(eo)
+
+
+@pytest.mark.parametrize("n,radius", [(3000, 19.0), (15000, 33.0)])
+def test_pair_once_matvec(n, radius):
+    """The pair-once sweep (each unordered pair evaluated once, both directions
+    accumulated) against the oracle's matvec at 3,000 sites (12 blocks of 256,
+    the last one padded) and against the per-target sweep at 15,000; both
+    precisions, one and two columns, bitwise repeatable."""
+    from openmmpol_amd import synth
+    spec = synth.sphere_system(n, radius, seed=6)
+    s = synth.make_system(spec)
+    x = np.random.default_rng(8).normal(0.0, 0.05, size=(2, s.pol_atoms, 3))
+    if n <= 3000:
+        o = oa.OracleSystem(synth.oracle_prep(spec))
+        ref = np.stack([o.tmatvec(x[0]), o.tmatvec(x[1])])
+    else:
+        ref, _ = s.tmatvec(x, kind="plain64")
+    scale = np.abs(ref).max()
+    for kind, tol in (("sym64", 1e-13), ("sym32", 2e-6)):
+        for ncols in (1, 2):
+            y, _ = s.tmatvec(x[:ncols], kind=kind)
+            assert np.abs(y - ref[:ncols]).max() <= tol * scale, (kind, ncols)
+        y2, _ = s.tmatvec(x, kind=kind)
+        assert np.array_equal(y2, s.tmatvec(x, kind=kind)[0])
+    s.terminate()
