@@ -96,9 +96,11 @@ double *ommp_amd_get_pol(OMMP_SYSTEM_PRT s);
 
 /* One T matvec y = T x on the device for ncols (1 or 2) host vectors
    x, y [ncols][pol_atoms][3] (TMatVec_otf / TMatVec_incore with the diagonal,
-   mod_polarization.F90:437-504).  kind: 0 on-the-fly FP64, 1 on-the-fly FP32,
-   2 packed in-core FP32, 3 packed in-core FP64, 4 fast multipoles (FP64, with
-   the system's FMM settings).  repeat > 1 launches it that
+   mod_polarization.F90:437-504).  kind: 0 on-the-fly FP64, 1 on-the-fly FP32
+   (each the sweep the solvers use at this size), 2 packed in-core FP32, 3 packed
+   in-core FP64, 4 fast multipoles (FP64, with the system's FMM settings),
+   5 / 6 on-the-fly FP64 / FP32 evaluating each unordered pair once, 7 / 8
+   on-the-fly FP64 / FP32 per target.  repeat > 1 launches it that
    many times back to back and returns the mean device time per launch (ms). */
 double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const double *x, double *y,
                         int32_t repeat);

@@ -356,12 +356,17 @@ __global__ __launch_bounds__(kBlock) void k_tmv_finalize(int npol, int nsplit, c
         R sum[M * 3];
 #pragma unroll
         for (int c = 0; c < M * 3; c++) sum[c] = R(0);
+        // unrolled so that the loads of several partials are in flight at once
+        // (the adds keep their order)
+        constexpr int kUnroll = sizeof(R) == 4 ? 8 : 4;
+#pragma unroll kUnroll
         for (int J = b; J < nb; J++) {
             const R *p = part + (row + J - b) * stride + li;
 #pragma unroll
             for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
         }
         const R *part_j = part + np * stride;
+#pragma unroll kUnroll
         for (int I = 0; I <= b; I++) {
             const R *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + li;
 #pragma unroll

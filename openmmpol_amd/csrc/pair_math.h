@@ -104,7 +104,7 @@ __device__ __forceinline__ void pair_geom(const typename V4<R>::t &pj, R aj, con
     k3 = (R(3) * k2) * ir2;
     const R u = (r2 * ir) * (pi.w * pj.w);
     if (AMOEBA) {
-        const R a = ai < aj ? ai : aj;
+        const R a = fmin(ai, aj);  // both finite and positive: one v_min
         arg = a * (u * u * u);
         near = arg < R(50);
     } else {
