@@ -23,7 +23,7 @@ namespace {
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct SegLayout {
-    int nsplit, chunk, tblocks, vblocks;
+    int nsplit, chunk, tblocks, vblocks, sym32, sym64;
     size_t vec, pf, part, red_hp, red_rz, ext, ered, eout, charges, total;
 };
 
@@ -44,7 +44,13 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
     L.chunk = (npol + L.nsplit - 1) / L.nsplit;
     L.vec = align256(sizeof(double) * ncols * n);
     L.pf = m32 == kBatchM32 ? align256(sizeof(float) * ncols * n) : 0;
-    L.part = align256(sizeof(double) * L.nsplit * ncols * n);
+    // the pair-once sweep where the single-system solver would use it (the
+    // matrix-core FP32 matvec keeps its own tiling)
+    L.sym32 = m32 == 2 && tmv_pair_once(npol, 32, ncols);
+    L.sym64 = tmv_pair_once(npol, 64, ncols);
+    size_t part = sizeof(double) * L.nsplit * ncols * n;
+    if (L.sym32 || L.sym64) part = std::max(part, tmv_sym_part_bytes(npol, ncols, 64));
+    L.part = align256(part);
     L.red_hp = align256(sizeof(double) * ncols * L.tblocks);
     L.red_rz = align256(sizeof(double) * ncols * L.vblocks);
     L.ext = align256(sizeof(double) * nrhs * n);
@@ -83,7 +89,7 @@ BatchEngine::~BatchEngine() {
     for (void *p : owned_) (void)hipFree(p);
     (void)hipFree(d_cols);
     (void)hipFree(d_segs_);
-    int4 *lists[] = {d_pair32_, d_fin32_, d_pair64_, d_fin64_, d_vec_};
+    int4 *lists[] = {d_pair32_, d_fin32_, d_pair64_, d_fin64_, d_vec_, d_sym32_, d_sym64_};
     for (auto p : lists)
         if (p) (void)hipFree(p);
     for (int k = 0; k < 2; k++) {
@@ -168,6 +174,8 @@ int BatchEngine::add(const DeviceView &v, int nrhs, int extra) {
     d.ncols = ncols;
     d.nsplit = L.nsplit;
     d.chunk = L.chunk;
+    d.sym32 = L.sym32;
+    d.sym64 = L.sym64;
     d.tblocks = L.tblocks;
     d.vblocks = L.vblocks;
     d.col0 = col0;
@@ -234,11 +242,21 @@ void BatchEngine::commit() {
     // largest systems first: their blocks start early, small ones fill the tail
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return segs[a].dev.npol > segs[b].dev.npol; });
-    std::vector<int4> p32, f32, p64, f64, vec;
+    std::vector<int4> p32, f32, p64, f64, vec, s32, s64;
     for (int si : order) {
         const BSeg &d = segs[si].dev;
         const int g32 = (d.ncols + m32 - 1) / m32, g64 = (d.ncols + m64 - 1) / m64;
-        if (m32 == kBatchM32) {
+        // pair-once items: every block pair (I <= J), row-major (index order)
+        auto sym_items = [&](std::vector<int4> &out, int groups) {
+            for (int g = 0; g < groups; g++) {
+                int idx = 0;
+                for (int I = 0; I < d.tblocks; I++)
+                    for (int J = I; J < d.tblocks; J++) out.push_back(make_int4(si, I | (J << 16), idx++, g));
+            }
+        };
+        if (d.sym32) {
+            sym_items(s32, g32);
+        } else if (m32 == kBatchM32) {
             const int ttiles = (d.npol + kMfmaSites - 1) / kMfmaSites;
             const int gm = (d.ncols + kMfmaCols - 1) / kMfmaCols;
             for (int g = 0; g < gm; g++)
@@ -251,9 +269,12 @@ void BatchEngine::commit() {
         }
         for (int g = 0; g < g32; g++)
             for (int tb = 0; tb < d.tblocks; tb++) f32.push_back(make_int4(si, tb, g, 0));
-        for (int g = 0; g < g64; g++)
-            for (int sp = 0; sp < d.nsplit; sp++)
-                for (int tb = 0; tb < d.tblocks; tb++) p64.push_back(make_int4(si, tb, sp, g));
+        if (d.sym64)
+            sym_items(s64, g64);
+        else
+            for (int g = 0; g < g64; g++)
+                for (int sp = 0; sp < d.nsplit; sp++)
+                    for (int tb = 0; tb < d.tblocks; tb++) p64.push_back(make_int4(si, tb, sp, g));
         for (int g = 0; g < g64; g++)
             for (int tb = 0; tb < d.tblocks; tb++) f64.push_back(make_int4(si, tb, g, 0));
         for (int c = 0; c < d.ncols; c++)
@@ -264,9 +285,11 @@ void BatchEngine::commit() {
     ensure_capacity(d_pair64_, cap_pair64_, (int)p64.size());
     ensure_capacity(d_fin64_, cap_fin64_, (int)f64.size());
     ensure_capacity(d_vec_, cap_vec_, (int)vec.size());
+    ensure_capacity(d_sym32_, cap_sym32_, (int)s32.size());
+    ensure_capacity(d_sym64_, cap_sym64_, (int)s64.size());
     const size_t nseg = segs.size();
-    const size_t need = sizeof(BSeg) * nseg +
-                        sizeof(int4) * (p32.size() + f32.size() + p64.size() + f64.size() + vec.size());
+    const size_t need = sizeof(BSeg) * nseg + sizeof(int4) * (p32.size() + f32.size() + p64.size() + f64.size() +
+                                                              vec.size() + s32.size() + s64.size());
     if (stage_pending_) {
         BCHECK(hipEventSynchronize(stage_ev_));
         stage_pending_ = false;
@@ -293,6 +316,8 @@ void BatchEngine::commit() {
     put(p64.data(), sizeof(int4) * p64.size(), d_pair64_);
     put(f64.data(), sizeof(int4) * f64.size(), d_fin64_);
     put(vec.data(), sizeof(int4) * vec.size(), d_vec_);
+    put(s32.data(), sizeof(int4) * s32.size(), d_sym32_);
+    put(s64.data(), sizeof(int4) * s64.size(), d_sym64_);
     BCHECK(hipEventRecord(stage_ev_, stream));
     stage_pending_ = true;
     tables_.segs = d_segs_;
@@ -307,6 +332,10 @@ void BatchEngine::commit() {
     tables_.nfin64 = (int)f64.size();
     tables_.vec = d_vec_;
     tables_.nvec = (int)vec.size();
+    tables_.sym32 = d_sym32_;
+    tables_.nsym32 = (int)s32.size();
+    tables_.sym64 = d_sym64_;
+    tables_.nsym64 = (int)s64.size();
     dirty_ = false;
 }
 

@@ -74,13 +74,18 @@ struct BSeg {
     const double *pp_f;
     double *b, *x, *r, *z, *p, *h, *d;  // [ncols][3 npol]
     float *pf;                           // [ncols][3 npol] FP32 copy of p (MFMA path)
-    void *part;                          // [nsplit][ncols][3 npol], FP64-sized
+    // partial sums of the matvec: per-target sweep [nsplit][ncols][3 npol]; pair-once
+    // sweep (sym32 / sym64 set for that precision, tmv_sym.h) [block pair][ncols][3][256]
+    // target sides, then as many source sides; FP64-sized
+    void *part;
+    int sym32, sym64;
     double *red_hp;                      // [ncols][tblocks]
     double *red_rz;                      // [ncols][vblocks]
 };
 
 // Work lists (built on the host when the batch composition changes):
 //   pair items     int4 {segment, target block, source split, column group}
+//   pair-once items int4 {segment, I | J << 16, block pair index, column group}
 //   finalize items int4 {segment, target block, column group, 0}
 //   vector items   int4 {segment, column in segment, vector block, batch column}
 struct BatchTables {
@@ -94,6 +99,10 @@ struct BatchTables {
     int npair64;
     const int4 *fin64;
     int nfin64;
+    const int4 *sym32;
+    int nsym32;
+    const int4 *sym64;
+    int nsym64;
     const int4 *vec;
     int nvec;
 };

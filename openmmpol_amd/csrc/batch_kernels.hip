@@ -10,6 +10,7 @@
 
 #include "batch.h"
 #include "pair_math.h"
+#include "tmv_sym.h"
 
 namespace ommp {
 
@@ -163,6 +164,78 @@ __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs
         }
     }
     bpairs_body<R, M, AMOEBA>(S, w, xin, cidx);
+}
+
+// Pair-once items {segment, I | J << 16, block pair index, column group}: the
+// block pair's sweep (tmv_sym.h) for the active columns of the group, into the
+// segment's block-pair partials [pair][ncols][3][256] (target sides, then the
+// source sides).  A group of two with one active column runs the one-column
+// body, as k_bpairs.
+template <typename R, int M, bool AMOEBA>
+__device__ __forceinline__ void bsym_body(const BSeg &S, const int4 w, const double *__restrict__ xin,
+                                          const int (&cidx)[M], char *lds) {
+    const int nb = S.tblocks;
+    const long np = (long)nb * (nb + 1) / 2, n = 3L * S.npol;
+    R *part_i = static_cast<R *>(S.part);
+    R *part_j = part_i + np * S.ncols * 3 * kSymBlock;
+    const double *xcol[M];
+    R *oi[M], *oj[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        xcol[m] = xin + (long)cidx[m] * n;
+        const long o = ((long)w.z * S.ncols + cidx[m]) * 3 * kSymBlock;
+        oi[m] = part_i + o;
+        oj[m] = part_j + o;
+    }
+    sym_block_pair<R, M, AMOEBA>(S.npol, w.y & 0xffff, w.y >> 16, SegSites<R>::site(S), SegSites<R>::damp(S), xcol,
+                                 oi, oj, lds);
+}
+
+template <typename R, int M, bool AMOEBA>
+__global__ __launch_bounds__(kSymBlock) void k_bsym(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                    const int4 *__restrict__ work, int use_p) {
+    constexpr int kPair = M < 2 ? M : 2;  // columns per sweep
+    __shared__ __align__(16) char lds[sym_lds_bytes<R, kPair>()];
+    const int4 w = work[blockIdx.x];
+    const BSeg &S = segs[w.x];
+    const int c0 = w.w * M;
+    const double *xin = use_p ? S.p : S.x;
+    int act[M];
+    int nact = 0;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const bool a = c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
+        act[m] = a ? c0 + m : -1;
+        nact += a;
+    }
+    if (nact == 0) return;  // uniform across the workgroup
+    if (nact == 1) {
+        int one[1] = {-1};
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (act[m] >= 0) one[0] = act[m];
+        bsym_body<R, 1, AMOEBA>(S, w, xin, one, lds);
+        return;
+    }
+    if constexpr (M == 2) {
+        bsym_body<R, 2, AMOEBA>(S, w, xin, act, lds);
+    } else {
+        // every active column of the group in turn, two at a time (an odd one alone)
+        int idx[M];
+        int k = 0;
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (act[m] >= 0) idx[k++] = act[m];
+        int q = 0;
+        for (; q + 1 < k; q += 2) {
+            const int two[2] = {idx[q], idx[q + 1]};
+            bsym_body<R, 2, AMOEBA>(S, w, xin, two, lds);
+        }
+        if (q < k) {
+            const int one[1] = {idx[q]};
+            bsym_body<R, 1, AMOEBA>(S, w, xin, one, lds);
+        }
+    }
 }
 
 // ------------------------------------------------- matrix-core FP32 matvec
@@ -354,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const BSeg *__restrict__ segs
     double acc[M][3];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
-    if (valid) {
+    if (valid && !(use_p ? S.sym32 : S.sym64)) {
         for (int s = 0; s < S.nsplit; s++)
 #pragma unroll
             for (int m = 0; m < M; m++) {
@@ -364,6 +437,38 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const BSeg *__restrict__ segs
                 acc[m][1] -= (double)p[1];
                 acc[m][2] -= (double)p[2];
             }
+    } else if (valid) {
+        // block-pair partials: the row (b, J >= b) of target sides, then the
+        // column (I <= b, b) of source sides; pair (I, J) has index
+        // I nb - I (I - 1) / 2 + J - I
+        const int nb = S.tblocks, b = w.y, li = threadIdx.x;
+        const long np = (long)nb * (nb + 1) / 2, row = (long)b * nb - (long)b * (b - 1) / 2;
+        const R *part_j = part + np * ncols * 3 * kSymBlock;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            if (!act[m]) continue;
+            R sum[3] = {R(0), R(0), R(0)};
+#pragma unroll 4
+            for (int J = b; J < nb; J++) {
+                const R *p = part + (((row + J - b) * ncols + c0 + m) * 3) * kSymBlock + li;
+                sum[0] += p[0];
+                sum[1] += p[kSymBlock];
+                sum[2] += p[2 * kSymBlock];
+            }
+#pragma unroll 4
+            for (int I = 0; I <= b; I++) {
+                const long pidx = (long)I * nb - (long)I * (I - 1) / 2 + b - I;
+                const R *p = part_j + ((pidx * ncols + c0 + m) * 3) * kSymBlock + li;
+                sum[0] += p[0];
+                sum[1] += p[kSymBlock];
+                sum[2] += p[2 * kSymBlock];
+            }
+            acc[m][0] = -(double)sum[0];
+            acc[m][1] = -(double)sum[1];
+            acc[m][2] = -(double)sum[2];
+        }
+    }
+    if (valid) {
         const double4 pj = S.ps64[j];
         const double aj = S.pa64[j];
         for (int k = S.pp_ri[j]; k < S.pp_ri[j + 1]; k++) {
@@ -542,21 +647,25 @@ __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ 
 }
 
 template <typename R, int M, bool AMOEBA>
-static void mv_launch(hipStream_t st, const BatchTables &t, const int4 *pair, int npair, const int4 *fin, int nfin,
-                      int use_p) {
-    if (npair == 0) return;
-    hipLaunchKernelGGL((k_bpairs<R, M, AMOEBA>), dim3(npair), dim3(kBlock), 0, st, t.segs, t.cols, pair, use_p);
+static void mv_launch(hipStream_t st, const BatchTables &t, const int4 *pair, int npair, const int4 *sym, int nsym,
+                      const int4 *fin, int nfin, int use_p) {
+    if (npair == 0 && nsym == 0) return;
+    if (npair)
+        hipLaunchKernelGGL((k_bpairs<R, M, AMOEBA>), dim3(npair), dim3(kBlock), 0, st, t.segs, t.cols, pair, use_p);
+    if (nsym)
+        hipLaunchKernelGGL((k_bsym<R, M, AMOEBA>), dim3(nsym), dim3(kSymBlock), 0, st, t.segs, t.cols, sym, use_p);
     hipLaunchKernelGGL((k_bfinal<R, M, AMOEBA>), dim3(nfin), dim3(kBlock), 0, st, t.segs, t.cols, fin, use_p);
 }
 
 template <bool AMOEBA>
 static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, double eta, double frac) {
-    if (m64 == 2) mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
-    else mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.fin64, t.nfin64, 0);
+    if (m64 == 2) mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0);
+    else
+        mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0);
     hipLaunchKernelGGL(k_bresid, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
     hipLaunchKernelGGL(k_bdecide, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec, eta, frac);
     if (m32 == 2) {
-        mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.fin32, t.nfin32, 1);
+        mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.sym32, t.nsym32, t.fin32, t.nfin32, 1);
     } else if (t.npair32) {
         hipLaunchKernelGGL(k_bconvert, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
         hipLaunchKernelGGL(k_bpairs_mfma<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);

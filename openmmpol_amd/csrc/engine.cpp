@@ -29,7 +29,7 @@ namespace ommp {
     } while (0)
 
 // kOtf64 / kOtf32 pick the pair-once sweep (kSym*) or the per-target one
-// (kPlain*) by size (use_pair_once); the explicit kinds force one of them.
+// (kPlain*) by size (tmv_pair_once); the explicit kinds force one of them.
 enum MvKind { kOtf64 = 0, kOtf32 = 1, kPack32 = 2, kPack64 = 3, kFmm = 4, kSym64 = 5, kSym32 = 6, kPlain64 = 7,
               kPlain32 = 8 };
 
@@ -474,17 +474,6 @@ void ensure_fmm(System &s, DeviceState &d) {
     d.fmm_valid = true;
 }
 
-// The pair-once sweep (tmv_sym.hip) evaluates half the pairs, at the price of
-// a workgroup per block pair and of its block-pair partial sums.  Measured on
-// MI355X against the per-target sweep (two columns): FP64 faster from 5,000
-// sites (-7%) to 100,000 (-25%); FP32 slower up to 10,000 sites (+14%), faster
-// from 15,000 (-14%; -15% at 100,000).  Used from those sizes while its partial
-// sums stay under 2 GiB (up to ~100,000 sites); otherwise the per-target sweep.
-bool use_pair_once(int npol, int prec) {
-    if (npol < (prec == 32 ? 12000 : 4000)) return false;
-    return tmv_sym_part_bytes(npol, 2, 64) <= (size_t(2) << 30);
-}
-
 void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double *y, bool dodiag,
             double *dotpart, const CgState *skip) {
     const long n = d.n;
@@ -496,7 +485,7 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
         return;
     }
     if (kind == kOtf64 || kind == kOtf32) {
-        const bool sym = use_pair_once(s.npol, kind == kOtf32 ? 32 : 64);
+        const bool sym = tmv_pair_once(s.npol, kind == kOtf32 ? 32 : 64, 2);
         kind = kind == kOtf32 ? (sym ? kSym32 : kPlain32) : (sym ? kSym64 : kPlain64);
     }
     if (kind == kSym64 || kind == kSym32) {
@@ -970,7 +959,7 @@ size_t device_footprint_estimate(const System &s) {
     b += 3 * 24 * sizeof(double) * (n / kBlock + 16);      // reduction scratch
     const int nsplit = tmv_split(s.npol);
     b += tmv_part_bytes(s.npol, 2, 64, nsplit) + sizeof(double) * n * 16;  // pair and static-field partials
-    if (use_pair_once(s.npol, 64)) {  // block-pair partials and list of the pair-once sweep
+    if (tmv_pair_once(s.npol, 64, 2)) {  // block-pair partials and list of the pair-once sweep
         const size_t nb = tmv_sym_blocks(s.npol);
         b += tmv_sym_part_bytes(s.npol, 2, 64) + sizeof(int2) * nb * (nb + 1) / 2;
     }

@@ -1,0 +1,180 @@
+// Pair-once sweep of one block pair: the body shared by the single-system T
+// matvec (tmv_sym.hip) and the batched solver (batch_kernels.hip).
+//
+// Sites are grouped in blocks of kSymBlock = 256 (four 64-site sub-tiles, one per
+// wave).  For the block pair (I <= J), wave w holds the targets of sub-tile w of
+// I in registers; the 256 sites of J are staged in LDS.  For sub-tile k of J,
+// lane l meets source (l + s) mod 64 at step s = 0..63, so at every step the 64
+// lanes touch 64 distinct sources: the target side accumulates in registers, and
+// so does the source side, each lane holding the sum of its current source and
+// taking over the next lane's after every step (a DPP wave rotate, on the vector
+// ALU); after 64 steps every source-side sum is back at its source's lane.
+// Every sum has a fixed order (results bitwise reproducible).  Measured
+// alternatives for the source side, both slower: LDS float atomics into a
+// per-wave row (FP32 ones were 14x slower here) and lane shuffles through the
+// LDS crossbar.  In a diagonal block pair (I = J) the sub-tile pairs k < w are
+// the mirror images of pairs handled by wave k and are skipped, and the
+// diagonal sub-tile (k = w) is swept one-sided over s = 1..63.  Scale 1 for every
+// pair (the finalize kernels apply the screening corrections), self excluded.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "pair_math.h"
+
+namespace ommp {
+
+constexpr int kSymWaves = kSymBlock / 64;
+
+// Lane l + 1's value at lane l (wave rotate by one lane, DPP).
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_next(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x134, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x134, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// LDS of sym_block_pair for up to M columns (bytes): the block's sources
+// (position, damping, dipoles) and the per-wave rows of source-side sums.
+template <typename R, int M>
+constexpr int sym_lds_bytes() {
+    return (int)(sizeof(typename V4<R>::t) * kSymBlock * (1 + M) + sizeof(R) * kSymWaves * M * 3 * 64);
+}
+
+// Pair (I, J) of blocks with a kSymBlock-thread workgroup: xcol[m] = column m's
+// dipoles [npol][3] (FP64); the target-side sums of component c of column m go
+// to out_i[m][c * kSymBlock + t] (t = the site's index in I), the source-side
+// ones to out_j[m][c * kSymBlock + t] (t in J).  lds: sym_lds_bytes<R, M>()
+// bytes, 16-byte aligned (the caller's, so that bodies of different widths in
+// one kernel share it); the body ends with a workgroup barrier.
+template <typename R, int M, bool AMOEBA>
+__device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typename V4<R>::t *__restrict__ site,
+                                               const R *__restrict__ adamp, const double *const (&xcol)[M],
+                                               R *const (&out_i)[M], R *const (&out_j)[M], char *lds) {
+    using R4 = typename V4<R>::t;
+    R4 *s_pos = reinterpret_cast<R4 *>(lds);
+    R4(*s_x)[kSymBlock] = reinterpret_cast<R4(*)[kSymBlock]>(s_pos + kSymBlock);  // x, y, z; w of column 0: a
+    R(*s_acc)[M * 3][64] = reinterpret_cast<R(*)[M * 3][64]>(s_x + M);
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+    // Padding sites (past npol, in the last block only) sit far away at distinct
+    // points with zero dipoles, so that no pair but the skipped self pair has a
+    // zero distance.
+    const R far = R(1e15), gap = R(1e9);
+    // sources: the sites of J
+    {
+        const int jj = J * kSymBlock + tid;
+        R4 p, v[M];
+        if (jj < npol) {
+            p = site[jj];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                v[m].x = (R)xcol[m][3L * jj + 0];
+                v[m].y = (R)xcol[m][3L * jj + 1];
+                v[m].z = (R)xcol[m][3L * jj + 2];
+                v[m].w = m == 0 ? adamp[jj] : R(0);
+            }
+        } else {
+            p.x = p.y = p.z = far + gap * (R)tid;
+            p.w = R(1);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                v[m].x = v[m].y = v[m].z = R(0);
+                v[m].w = R(0.39);
+            }
+        }
+        s_pos[tid] = p;
+#pragma unroll
+        for (int m = 0; m < M; m++) s_x[m][tid] = v[m];
+    }
+    // targets: this thread's site of I
+    const int i = I * kSymBlock + tid;
+    R4 pj;
+    R aj, xi[M][3];
+    if (i < npol) {
+        pj = site[i];
+        aj = adamp[i];
+#pragma unroll
+        for (int m = 0; m < M; m++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) xi[m][c] = (R)xcol[m][3L * i + c];
+    } else {
+        pj.x = pj.y = pj.z = far + gap * (R)tid;
+        pj.w = R(1);
+        aj = R(0.39);
+#pragma unroll
+        for (int m = 0; m < M; m++) xi[m][0] = xi[m][1] = xi[m][2] = R(0);
+    }
+    R acc[M][3];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = R(0);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSymWaves; k++) {
+        const bool mirror = I == J && k < w;  // wave-uniform
+        const bool diag = I == J && k == w;
+        R jacc[M * 3];
+#pragma unroll
+        for (int c = 0; c < M * 3; c++) jacc[c] = R(0);
+        if (diag) {
+            // one-sided over the other 63 sites of the sub-tile
+            for (int s = 1; s < 64; s++) {
+                const int js = k * 64 + ((l + s) & 63);
+                R4 xv[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) xv[m] = s_x[m][js];
+                R dx, dy, dz, k2, k3, arg;
+                bool near;
+                pair_geom<R, AMOEBA, false>(pj, aj, s_pos[js], xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
+                pair_accum<R, M>(acc, xv, dx, dy, dz, k2, k3);
+            }
+        } else if (!mirror) {
+            int jl = l;
+            for (int s = 0; s < 64; s++) {
+                const int js = k * 64 + jl;
+                R4 xv[M];
+#pragma unroll
+                for (int m = 0; m < M; m++) xv[m] = s_x[m][js];
+                R dx, dy, dz, k2, k3, arg;
+                bool near;
+                pair_geom<R, AMOEBA, false>(pj, aj, s_pos[js], xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
+                pair_accum<R, M>(acc, xv, dx, dy, dz, k2, k3);
+                const R nk2 = -k2;
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const R c = k3 * fmaa(xi[m][2], dz, fmaa(xi[m][1], dy, xi[m][0] * dx));
+                    jacc[3 * m + 0] = fmaa(c, dx, fmaa(nk2, xi[m][0], jacc[3 * m + 0]));
+                    jacc[3 * m + 1] = fmaa(c, dy, fmaa(nk2, xi[m][1], jacc[3 * m + 1]));
+                    jacc[3 * m + 2] = fmaa(c, dz, fmaa(nk2, xi[m][2], jacc[3 * m + 2]));
+                }
+#pragma unroll
+                for (int c = 0; c < M * 3; c++) jacc[c] = lane_next(jacc[c]);
+                jl = (jl + 1) & 63;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < M * 3; c++) s_acc[w][c][l] = jacc[c];
+        __syncthreads();
+        // source side of sub-tile k: the waves' rows added in wave order
+        for (int e = tid; e < M * 3 * 64; e += kSymBlock) {
+            const int c = e >> 6, jl = e & 63;
+            R sum = s_acc[0][c][jl];
+#pragma unroll
+            for (int q = 1; q < kSymWaves; q++) sum += s_acc[q][c][jl];
+            out_j[c / 3][(c % 3) * kSymBlock + k * 64 + jl] = sum;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) out_i[m][c * kSymBlock + tid] = acc[m][c];
+}
+
+}  // namespace ommp

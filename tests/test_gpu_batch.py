@@ -183,3 +183,54 @@ def test_server_admission_waits_for_memory(requests):
     assert stats["deferrals"] > 0 and stats["max_admitted"] == 1
     for k in which:
         assert np.array_equal(mix[k][0], tight[k][0]) and np.array_equal(mix[k][1], tight[k][1])
+
+
+# ------------------------------------------------ pair-once sweep in batches
+def test_multi_rhs_pair_once_matches_oracle(tight_oracle):
+    """A 5,000-site system (20 blocks of 256): its FP64 refinement matvec in
+    the batch runs the pair-once sweep, two columns per pass over a group of
+    four; every field against the oracle's exact solve."""
+    from openmmpol_amd import synth
+    spec = _sphere(5000, 31)
+    s = synth.make_system(spec)
+    s.set_solver_tolerance(TOL, 500)
+    o = oa.OracleSystem(synth.oracle_prep(spec))
+    cpol = spec["cmm"][o.polar_mm]
+    rng = np.random.default_rng(9)
+    ext = []
+    for k in range(3):
+        q = _qm(rng)
+        ext.append(synth.point_charge_field(cpol, q[:, :3], q[:, 3]))
+    ext = np.stack(ext)
+    ipd, ep = s.set_external_fields_multi(ext)
+    for k in range(len(ext)):
+        want, ew = _exact(o, ext[k])
+        assert np.abs(ipd[k] - want).max() < 1e-7, k
+        assert abs(ep[k] - ew) <= 1e-9 * abs(ew), k
+    s.terminate()
+
+
+def test_server_pair_once_sizes_alone_and_in_mix(tight_oracle):
+    """Requests of 5,000 sites (FP64 matvec pair-once), 13,000 (both matvecs
+    pair-once) and 700 (per-target) together: each request's results alone and
+    in the mix are identical, and the 5,000-site one matches the oracle."""
+    specs = [_sphere(n, sd) for n, sd in ((5000, 41), (13000, 42), (700, 43))]
+    charges = [_charges(200 + k, 2) for k in range(len(specs))]
+    alone = {}
+    for k in range(len(specs)):
+        r, _ = _run(specs, charges, [k])
+        alone[k] = r[k]
+    mix, stats = _run(specs, charges, list(range(len(specs))))
+    assert stats["max_admitted"] >= 2
+    for k in range(len(specs)):
+        assert np.array_equal(alone[k][0], mix[k][0]), k
+        assert np.array_equal(alone[k][1], mix[k][1]), k
+    from openmmpol_amd import synth
+    o = oa.OracleSystem(synth.oracle_prep(specs[0]))
+    cpol = specs[0]["cmm"][o.polar_mm]
+    for step in range(len(charges[0])):
+        q = charges[0][step]
+        ext = synth.point_charge_field(cpol, q[:, :3], q[:, 3])
+        ipd, ew = _exact(o, ext)
+        assert abs(mix[0][0][step] - ew) <= 1e-9 * abs(ew), step
+    assert np.abs(mix[0][1] - ipd).max() < 1e-7
