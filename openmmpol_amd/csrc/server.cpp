@@ -6,7 +6,10 @@
 // worker thread admits requests while batch slots and the HBM budget allow,
 // keeps every admitted system resident (geometry, lists, static field, last
 // dipoles), and runs ticks of the batched mixed solver (batch.h) over all
-// admitted systems at once.  A solve that converges is retired at the next
+// admitted systems at once.  A preparation thread brings a newly admitted
+// system onto the device (upload, static field, on the system's own stream)
+// while the worker keeps ticking; the worker adds it to the batch when it is
+// ready.  A solve that converges is retired at the next
 // tick boundary; the request's next solve starts right away from the converged
 // dipoles (warm start), and a request whose last solve finished releases its
 // slot and memory to the next waiting request.  Column states are read back one
@@ -72,7 +75,7 @@ struct Request {
     int inner_shared = 0;  // its inner iterations (reported with the step)
     long valid_from = 0;
     double t_ready = 0.0;
-    size_t footprint = 0;
+    size_t footprint = 0;  // bytes held against the budget (the estimate until added)
     double *epin = nullptr;  // pinned [nsteps] energies
     hipEvent_t ev = nullptr;
     // results
@@ -96,6 +99,7 @@ class Server {
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
             num_cu_ = prop.multiProcessorCount;
         worker_ = std::thread([this] { run(); });
+        prep_ = std::thread([this] { prepare_loop(); });
     }
 
     ~Server() {
@@ -105,6 +109,12 @@ class Server {
         }
         cv_.notify_all();
         worker_.join();
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_prep_ = true;
+        }
+        cv_.notify_all();
+        prep_.join();
         delete engine_;
         for (auto &kv : reqs_) delete kv.second;
         (void)hipStreamDestroy(stream_);
@@ -176,45 +186,83 @@ class Server {
     }
 
   private:
+    // Requests leave the queue while slots (admitted + in preparation) and the
+    // budget allow; their estimated footprint is held from then on.
     void admit_locked(std::unique_lock<std::mutex> &lk) {
-        while (!pending_.empty() && (int)active_.size() < max_active_) {
+        while (!pending_.empty() && (int)active_.size() + preparing_ < max_active_) {
             Request *r = pending_.front();
             const int nsets = r->sys->n_ipd;
             const size_t fp = device_footprint_estimate(*r->sys) +
                               BatchEngine::segment_bytes(r->sys->npol, 1, nsets, false, num_cu_, nsets > 1);
-            if (used_ + fp > budget_ && !active_.empty()) {
+            if (used_ + fp > budget_ && !(active_.empty() && preparing_ == 0)) {
                 deferred_++;
                 return;  // wait for memory to be released
             }
             pending_.pop_front();
-            lk.unlock();
-            const bool ok = admit(r, fp);
-            lk.lock();
-            if (!ok) {
-                if (active_.empty()) fatal("Cannot allocate device memory for a request of the serving loop");
-                pending_.push_front(r);
-                return;
+            if (r->sys->npol == 0) {
+                // nothing to solve: every step has zero energy
+                r->epin_copy.assign(r->nsteps, 0.0);
+                lk.unlock();
+                finish_now(r);
+                lk.lock();
+                continue;
             }
+            r->footprint = fp;
+            used_ += fp;
+            preparing_++;
+            to_prepare_.push_back(r);
+            cv_.notify_all();
         }
     }
 
-    bool admit(Request *r, size_t fp_estimate) {
+    // Preparation thread: device state and static field of each admitted
+    // system (device_view synchronizes the system's own stream).
+    void prepare_loop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_prep_ || !to_prepare_.empty(); });
+            if (to_prepare_.empty()) break;
+            Request *r = to_prepare_.front();
+            to_prepare_.pop_front();
+            lk.unlock();
+            DeviceView v = device_view(*r->sys);
+            lk.lock();
+            prepared_.push_back({r, v});
+            cv_.notify_all();
+        }
+    }
+
+    // Worker: prepared systems join the batch.  One that does not fit (no free
+    // segment or column range) goes back to the front of the queue.
+    void add_prepared_locked(std::unique_lock<std::mutex> &lk) {
+        if (prepared_.empty()) return;
+        std::vector<std::pair<Request *, DeviceView>> ready;
+        ready.swap(prepared_);
+        lk.unlock();
+        std::vector<Request *> back;
+        for (auto &rv : ready)
+            if (!add_to_batch(rv.first, rv.second)) back.push_back(rv.first);
+        lk.lock();
+        preparing_ -= (int)ready.size();
+        for (auto it = back.rbegin(); it != back.rend(); ++it) {
+            if (active_.empty() && preparing_ == 0) fatal("Cannot allocate device memory for a request of the serving loop");
+            used_ -= (*it)->footprint;
+            (*it)->footprint = 0;
+            pending_.push_front(*it);
+        }
+    }
+
+    bool add_to_batch(Request *r, const DeviceView &v) {
         System &s = *r->sys;
         if (!engine_) engine_ = new BatchEngine(stream_, s.amoeba, false, max_active_, 3 * max_active_);
         if (s.amoeba != engine_->amoeba) fatal("All systems of a server must use the same force field");
-        if (s.npol == 0) {
-            // nothing to solve: every step has zero energy
-            r->epin_copy.assign(r->nsteps, 0.0);
-            finish_now(r);
-            return true;
-        }
-        DeviceView v = device_view(s);  // upload, static field (system stream, synchronized)
         const int seg = engine_->add(v, 1, s.n_ipd > 1 ? 1 : 0);
         if (seg < 0) return false;
         SegInfo &S = engine_->segs[seg];
         r->seg = seg;
-        r->footprint = std::max(fp_estimate, v.bytes + S.bytes);
-        used_ += r->footprint;
+        const size_t held = std::max(r->footprint, v.bytes + S.bytes);
+        used_ += held - r->footprint;
+        r->footprint = held;
         if (used_ > peak_bytes_) peak_bytes_ = used_;
         SCHECK(hipHostMalloc((void **)&r->epin, sizeof(double) * r->nsteps, hipHostMallocDefault));
         SCHECK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
@@ -349,14 +397,19 @@ class Server {
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
             admit_locked(lk);
+            add_prepared_locked(lk);
             if (active_.empty()) {
                 lk.unlock();
                 if (!finishing_.empty()) SCHECK(hipStreamSynchronize(stream_));
                 reap_finished();
                 lk.lock();
                 prev = -1;
-                if (stop_ && pending_.empty() && finishing_.empty()) break;
-                if (pending_.empty()) cv_.wait(lk, [&] { return stop_ || !pending_.empty(); });
+                if (stop_ && pending_.empty() && finishing_.empty() && preparing_ == 0) break;
+                // nothing to tick: wait for a prepared system, or for a request
+                // when none is in preparation (one that is ready wakes us first)
+                cv_.wait(lk, [&] {
+                    return !prepared_.empty() || (preparing_ == 0 && (stop_ || !pending_.empty()));
+                });
                 continue;
             }
             lk.unlock();
@@ -415,6 +468,13 @@ class Server {
     std::vector<Request *> active_;     // worker thread
     std::vector<Request *> finishing_;  // worker thread
     size_t used_ = 0;  // worker thread
+    // preparation (under mu_): queued for the preparation thread, done, and the
+    // number of requests between admission and joining the batch
+    std::thread prep_;
+    bool stop_prep_ = false;
+    std::deque<Request *> to_prepare_;
+    std::vector<std::pair<Request *, DeviceView>> prepared_;
+    int preparing_ = 0;
     std::atomic<size_t> peak_bytes_{0};
     std::atomic<long> ticks_{0}, solves_{0}, col_ticks_{0}, deferred_{0};
     std::atomic<int> max_active_seen_{0};
