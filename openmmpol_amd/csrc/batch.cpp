@@ -44,10 +44,12 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
     L.chunk = (npol + L.nsplit - 1) / L.nsplit;
     L.vec = align256(sizeof(double) * ncols * n);
     L.pf = m32 == kBatchM32 ? align256(sizeof(float) * ncols * n) : 0;
-    // the pair-once sweep where the single-system solver would use it (the
-    // matrix-core FP32 matvec keeps its own tiling)
-    L.sym32 = m32 == 2 && tmv_pair_once(npol, 32, ncols);
-    L.sym64 = tmv_pair_once(npol, 64, ncols);
+    // the pair-once sweep at the sizes where the single-system solver uses it
+    // (the matrix-core FP32 matvec keeps its own tiling); many columns may take
+    // up to 8 GiB of block-pair partials (15,000 sites x 128 columns: 2.8 GB,
+    // written and read once per FP64 matvec of ~20 ms)
+    L.sym32 = m32 == 2 && tmv_pair_once(npol, 32, ncols, size_t(8) << 30);
+    L.sym64 = tmv_pair_once(npol, 64, ncols, size_t(8) << 30);
     size_t part = sizeof(double) * L.nsplit * ncols * n;
     if (L.sym32 || L.sym64) part = std::max(part, tmv_sym_part_bytes(npol, ncols, 64));
     L.part = align256(part);

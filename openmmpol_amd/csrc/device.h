@@ -82,8 +82,8 @@ void launch_tmv_finalize_prec(hipStream_t st, int prec, bool amoeba, int npol, i
 int tmv_sym_blocks(int npol);
 size_t tmv_sym_part_bytes(int npol, int M, int prec);
 // Whether the on-the-fly matvec of prec (32 / 64) with ncols columns uses the
-// pair-once sweep at npol sites.
-bool tmv_pair_once(int npol, int prec, int ncols);
+// pair-once sweep at npol sites (its partial sums at most max_bytes).
+bool tmv_pair_once(int npol, int prec, int ncols, size_t max_bytes = size_t(2) << 30);
 void launch_tmv_sym(hipStream_t st, int prec, bool amoeba, int npol, int M, const int2 *pairs, const double4 *ps64,
                     const double *pa64, const float4 *ps32, const float *pa32, const double *alpha,
                     const CsrDev &pp, const double *x, long xstride, double *y, long ystride, bool dodiag,

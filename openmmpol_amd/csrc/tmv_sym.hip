@@ -61,10 +61,10 @@ int tmv_sym_blocks(int npol) { return (npol + kSymBlock - 1) / kSymBlock; }
 // per-target sweep (two columns): FP64 faster from 5,000 sites (-7%) to 100,000
 // (-25%); FP32 slower up to 10,000 sites (+14%), faster from 15,000 (-14%; -15%
 // at 100,000).  Used from those sizes while its partial sums for ncols columns
-// stay under 2 GiB (two columns: up to ~100,000 sites).
-bool tmv_pair_once(int npol, int prec, int ncols) {
+// stay under max_bytes (2 GiB for two columns: up to ~100,000 sites).
+bool tmv_pair_once(int npol, int prec, int ncols, size_t max_bytes) {
     if (npol < (prec == 32 ? 12000 : 4000)) return false;
-    return tmv_sym_part_bytes(npol, ncols, 64) <= (size_t(2) << 30);
+    return tmv_sym_part_bytes(npol, ncols, 64) <= max_bytes;
 }
 
 size_t tmv_sym_part_bytes(int npol, int M, int prec) {
