@@ -89,6 +89,14 @@ __device__ __forceinline__ void dip_kernel(R r2, R ir, R ti, R tj, R ai, R aj, R
 // Geometry of one pair: d = r_j - r_i, undamped k2 = 1/r^3 and k3 = 3/r^5,
 // the damping argument (a u^3 for AMOEBA, u = r/s for Wang) and whether the
 // pair needs damping at all.
+// min of two positive finite values.  FP32: their bit patterns order as
+// integers, so one integer min (no NaN quieting); FP64 keeps the float min
+// (there is no 64-bit integer min instruction).
+__device__ __forceinline__ float pos_min(float a, float b) {
+    return __int_as_float(min(__float_as_int(a), __float_as_int(b)));
+}
+__device__ __forceinline__ double pos_min(double a, double b) { return fmin(a, b); }
+
 template <typename R, bool AMOEBA, bool GUARD = true>
 __device__ __forceinline__ void pair_geom(const typename V4<R>::t &pj, R aj, const typename V4<R>::t &pi, R ai,
                                           R &dx, R &dy, R &dz, R &k2, R &k3, R &arg, bool &near) {
@@ -104,7 +112,7 @@ __device__ __forceinline__ void pair_geom(const typename V4<R>::t &pj, R aj, con
     k3 = (R(3) * k2) * ir2;
     const R u = (r2 * ir) * (pi.w * pj.w);
     if (AMOEBA) {
-        const R a = fmin(ai, aj);  // both finite and positive: one v_min
+        const R a = pos_min(ai, aj);
         arg = a * (u * u * u);
         near = arg < R(50);
     } else {

@@ -94,9 +94,8 @@ class Server {
   public:
     Server(double budget, int max_active) : budget_(budget), max_active_(std::max(1, max_active)) {
         SCHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        int dev = 0;
         hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        if (hipGetDevice(&dev_) == hipSuccess && hipGetDeviceProperties(&prop, dev_) == hipSuccess)
             num_cu_ = prop.multiProcessorCount;
         worker_ = std::thread([this] { run(); });
         prep_ = std::thread([this] { prepare_loop(); });
@@ -218,6 +217,7 @@ class Server {
     // Preparation thread: device state and static field of each admitted
     // system (device_view synchronizes the system's own stream).
     void prepare_loop() {
+        SCHECK(hipSetDevice(dev_));  // the creating thread's device
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
             cv_.wait(lk, [&] { return stop_prep_ || !to_prepare_.empty(); });
@@ -393,6 +393,7 @@ class Server {
     }
 
     void run() {
+        SCHECK(hipSetDevice(dev_));  // the creating thread's device
         long prev = -1;
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
@@ -455,6 +456,7 @@ class Server {
     double budget_;
     int max_active_;
     int num_cu_ = 256;
+    int dev_ = 0;
     hipStream_t stream_ = nullptr;
     BatchEngine *engine_ = nullptr;
     std::thread worker_;

@@ -134,15 +134,21 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
                 pair_accum<R, M>(acc, xv, dx, dy, dz, k2, k3);
             }
         } else if (!mirror) {
-            int jl = l;
+            // byte offset of the current source within the sub-tile (one add
+            // and one mask per step)
+            constexpr int kRow = 64 * (int)sizeof(R4);
+            const char *pos_k = reinterpret_cast<const char *>(s_pos + k * 64);
+            const char *x_k = reinterpret_cast<const char *>(&s_x[0][k * 64]);
+            int jo = l * (int)sizeof(R4);
             for (int s = 0; s < 64; s++) {
-                const int js = k * 64 + jl;
                 R4 xv[M];
 #pragma unroll
-                for (int m = 0; m < M; m++) xv[m] = s_x[m][js];
+                for (int m = 0; m < M; m++)
+                    xv[m] = *reinterpret_cast<const R4 *>(x_k + m * kSymBlock * (int)sizeof(R4) + jo);
+                const R4 ps = *reinterpret_cast<const R4 *>(pos_k + jo);
                 R dx, dy, dz, k2, k3, arg;
                 bool near;
-                pair_geom<R, AMOEBA, false>(pj, aj, s_pos[js], xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                pair_geom<R, AMOEBA, false>(pj, aj, ps, xv[0].w, dx, dy, dz, k2, k3, arg, near);
                 if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
                 pair_accum<R, M>(acc, xv, dx, dy, dz, k2, k3);
                 const R nk2 = -k2;
@@ -155,7 +161,7 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
                 }
 #pragma unroll
                 for (int c = 0; c < M * 3; c++) jacc[c] = lane_next(jacc[c]);
-                jl = (jl + 1) & 63;
+                jo = (jo + (int)sizeof(R4)) & (kRow - 1);
             }
         }
 #pragma unroll

@@ -183,3 +183,19 @@ def test_pair_once_matvec(n, radius):
         y2, _ = s.tmatvec(x, kind=kind)
         assert np.array_equal(y2, s.tmatvec(x, kind=kind)[0])
     s.terminate()
+
+
+def test_pair_once_matvec_amber():
+    """The AMBER (Wang linear Thole, one dipole set) form of the pair-once
+    sweep against the per-target sweep at 13,000 sites (both precisions; the
+    per-target kernel is pinned to the oracle on the Wang-AL fixtures)."""
+    from openmmpol_amd import OMMPSystem, synth
+    spec = synth.sphere_system(13000, 31.5, seed=12)
+    s = OMMPSystem.from_arrays(False, spec["cmm"], spec["q"][:, :1], spec["pol"], spec["bonds_ri"],
+                               spec["bonds_ci"])
+    x = np.random.default_rng(13).normal(0.0, 0.05, size=(1, s.pol_atoms, 3))
+    want, _ = s.tmatvec(x, kind="plain64")
+    for kind, tol in (("sym64", 1e-13), ("sym32", 2e-6), ("plain32", 2e-6)):
+        got, _ = s.tmatvec(x, kind=kind)
+        assert np.abs(got - want).max() <= tol * np.abs(want).max(), kind
+    s.terminate()
