@@ -71,11 +71,6 @@ void launch_tmv_finalize64(hipStream_t st, bool amoeba, int npol, int M, const d
                            const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
                            const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
                            const int *skip_flag);
-// Same for FP32 (prec 32) or FP64 partials.
-void launch_tmv_finalize_prec(hipStream_t st, int prec, bool amoeba, int npol, int M, const void *part, int nsplit,
-                              const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
-                              const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
-                              const int *skip_flag);
 // T matvec on the fly with each unordered pair evaluated once (tmv_sym.hip):
 // pairs[] = the block pairs (I <= J) of kSymBlock-site blocks, row-major;
 // part holds tmv_sym_part_bytes(npol, M, prec).

@@ -310,9 +310,8 @@ __global__ __launch_bounds__(kBlock) void k_tmv_pairs(int npol, int chunk,
     }
 }
 
-// y = -(sum of partial fields) + corrections (+ x / alpha); optional y.x partials.
-// nsplit > 0: partials part[split][m][3 npol]; nsplit = -nb: the block-pair
-// partials of the pair-once matvec over nb blocks of kSymBlock sites.
+// y = -(sum of partial fields part[split][m][3 npol]) + corrections (+ x / alpha);
+// optional y.x partials.
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kBlock) void k_tmv_finalize(int npol, int nsplit, const R *__restrict__ part,
                                                          const double4 *__restrict__ ps,
@@ -336,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_tmv_finalize(int npol, int nsplit, c
     double acc[M][3];
 #pragma unroll
     for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
-    if (valid && nsplit > 0) {
+    if (valid) {
         for (int s = 0; s < nsplit; s++)
 #pragma unroll
             for (int m = 0; m < M; m++) {
@@ -345,37 +344,6 @@ __global__ __launch_bounds__(kBlock) void k_tmv_finalize(int npol, int nsplit, c
                 acc[m][1] -= (double)p[1];
                 acc[m][2] -= (double)p[2];
             }
-    } else if (valid) {
-        // block-pair partials of the pair-once matvec (tmv_sym.hip), nb = -nsplit
-        // blocks: the row (b, J >= b) of target sides, then the column (I <= b, b)
-        // of source sides; pair (I, J) has index I nb - I (I - 1) / 2 + J - I
-        const int nb = -nsplit;
-        const long np = (long)nb * (nb + 1) / 2, stride = (long)M * 3 * kSymBlock;
-        const int b = j / kSymBlock, li = j - b * kSymBlock;
-        const long row = (long)b * nb - (long)b * (b - 1) / 2;
-        R sum[M * 3];
-#pragma unroll
-        for (int c = 0; c < M * 3; c++) sum[c] = R(0);
-        // unrolled so that the loads of several partials are in flight at once
-        // (the adds keep their order)
-        constexpr int kUnroll = sizeof(R) == 4 ? 8 : 4;
-#pragma unroll kUnroll
-        for (int J = b; J < nb; J++) {
-            const R *p = part + (row + J - b) * stride + li;
-#pragma unroll
-            for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
-        }
-        const R *part_j = part + np * stride;
-#pragma unroll kUnroll
-        for (int I = 0; I <= b; I++) {
-            const R *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + li;
-#pragma unroll
-            for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
-        }
-#pragma unroll
-        for (int m = 0; m < M; m++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) acc[m][c] = -(double)sum[3 * m + c];
     }
     if (valid) {
         const double4 pj = ps[j];
@@ -508,34 +476,6 @@ void launch_tmv_otf(hipStream_t st, int prec, bool amoeba, int npol, int M, cons
             tmv_launch_t<double, 2>(st, amoeba, npol, ps64, pa64, ps64, pa64, alpha, pp, x, xstride, y, ystride,
                                     dodiag, dotpart, (double *)part, nsplit, skip, skipM, evb, eve);
     }
-}
-
-void launch_tmv_finalize_prec(hipStream_t st, int prec, bool amoeba, int npol, int M, const void *part, int nsplit,
-                              const double4 *ps64, const double *pa64, const double *alpha, const CsrDev &pp,
-                              const double *x, long xstride, double *y, long ystride, bool dodiag, double *dotpart,
-                              const int *skip_flag) {
-    if (npol == 0) return;
-    if (prec != 32) {
-        launch_tmv_finalize64(st, amoeba, npol, M, (const double *)part, nsplit, ps64, pa64, alpha, pp, x, xstride, y,
-                              ystride, dodiag, dotpart, skip_flag);
-        return;
-    }
-    const CgState *skip = reinterpret_cast<const CgState *>(skip_flag);
-    const float *pf = (const float *)part;
-    const dim3 g(site_blocks(npol)), b(kBlock);
-    if (amoeba && M == 1)
-        hipLaunchKernelGGL((k_tmv_finalize<float, 1, true>), g, b, 0, st, npol, nsplit, pf, ps64, pa64, alpha, pp.ri,
-                           pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
-    else if (amoeba)
-        hipLaunchKernelGGL((k_tmv_finalize<float, 2, true>), g, b, 0, st, npol, nsplit, pf, ps64, pa64, alpha, pp.ri,
-                           pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
-    else if (M == 1)
-        hipLaunchKernelGGL((k_tmv_finalize<float, 1, false>), g, b, 0, st, npol, nsplit, pf, ps64, pa64, alpha, pp.ri,
-                           pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
-    else
-        hipLaunchKernelGGL((k_tmv_finalize<float, 2, false>), g, b, 0, st, npol, nsplit, pf, ps64, pa64, alpha, pp.ri,
-                           pp.ci, pp.f, x, xstride, y, ystride, (int)dodiag, dotpart, skip, M);
-    HIP_CHECK(hipGetLastError());
 }
 
 // Finalize of externally computed FP64 partial fields (the fast multipole

@@ -5,15 +5,15 @@
 // sym_block_pair (tmv_sym.h).
 //
 // Outputs, per block pair p: target side part_i[p][M * 3][256] (sites of I) and
-// source side part_j[p][M * 3][256] (sites of J); the finalize kernel adds, for
-// every site, the nb partials of its row and column in a fixed order (a separate
-// reduction kernel with one thread per component measured slower).
+// source side part_j[p][M * 3][256] (sites of J); k_tmv_sym_finalize adds, for
+// every site, the nb partials of its row and column in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
 
 #include "device.h"
+#include "pair_math.h"
 #include "tmv_sym.h"
 
 namespace ommp {
@@ -52,6 +52,146 @@ __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__r
         oj[m] = part_j + ((long)blockIdx.x * M * 3 + 3 * m) * kSymBlock;
     }
     sym_block_pair<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
+}
+
+// y = -(row and column block-pair partials) + list_P_P corrections (+ x / alpha),
+// optional y.x partials, for one block of kSymBlock sites per workgroup.  kSymQ
+// groups of kSymBlock threads share each site: group q sums every kSymQ-th
+// partial of the site's row (pairs (b, J >= b): target sides) and column (pairs
+// (I <= b, b): source sides) and every kSymQ-th entry of its correction list; group
+// 0 adds the group results in a fixed order.  (With one thread per site the
+// finalize waited on the latency of 2 nb dependent partial loads per thread.)
+constexpr int kSymQ = 4;
+
+template <typename R, int M, bool AMOEBA>
+__global__ __launch_bounds__(kSymQ *kSymBlock) void k_tmv_sym_finalize(
+    int npol, int nb, const R *__restrict__ part, const double4 *__restrict__ ps, const double *__restrict__ pa,
+    const double *__restrict__ alpha, const int *__restrict__ pp_ri, const int *__restrict__ pp_ci,
+    const double *__restrict__ pp_f, const double *__restrict__ x, long xstride, double *__restrict__ y,
+    long ystride, int dodiag, double *__restrict__ dotpart, const CgState *__restrict__ skip, int skipM) {
+    if (skip) {
+        bool all = true;
+        for (int m = 0; m < skipM; m++) all = all && skip[m].done;
+        if (all) return;
+    }
+    __shared__ R shs[kSymQ - 1][M * 3][kSymBlock];
+    __shared__ double shc[kSymQ - 1][M * 3][kSymBlock];
+    __shared__ double shd[M][kSymBlock / 64];
+    const int q = threadIdx.x / kSymBlock, li = threadIdx.x % kSymBlock;
+    const int b = blockIdx.x, j = b * kSymBlock + li;
+    const bool valid = j < npol;
+    const long np = (long)nb * (nb + 1) / 2, stride = (long)M * 3 * kSymBlock;
+    R sum[M * 3];
+    double corr[M * 3];
+#pragma unroll
+    for (int c = 0; c < M * 3; c++) {
+        sum[c] = R(0);
+        corr[c] = 0.0;
+    }
+    if (valid) {
+        // pair (I, J) has index I nb - I (I - 1) / 2 + J - I
+        const long row = (long)b * nb - (long)b * (b - 1) / 2;
+        constexpr int kUnroll = sizeof(R) == 4 ? 4 : 2;
+#pragma unroll kUnroll
+        for (int J = b + q; J < nb; J += kSymQ) {
+            const R *p = part + (row + J - b) * stride + li;
+#pragma unroll
+            for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
+        }
+        const R *part_j = part + np * stride;
+#pragma unroll kUnroll
+        for (int I = q; I <= b; I += kSymQ) {
+            const R *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + li;
+#pragma unroll
+            for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
+        }
+        const double4 pj = ps[j];
+        const double aj = pa[j];
+        for (int k = pp_ri[j] + q; k < pp_ri[j + 1]; k += kSymQ) {
+            const int i = pp_ci[k];
+            const double f = pp_f[k];
+            const double4 pi = ps[i];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double k2, k3;
+            dip_kernel<double, AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, pa[i], aj, k2, k3);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const double *xi = x + m * xstride + 3L * i;
+                const double c = k3 * (xi[0] * dx + xi[1] * dy + xi[2] * dz);
+                corr[3 * m + 0] += f * (c * dx - k2 * xi[0]);
+                corr[3 * m + 1] += f * (c * dy - k2 * xi[1]);
+                corr[3 * m + 2] += f * (c * dz - k2 * xi[2]);
+            }
+        }
+    }
+    if (q > 0) {
+#pragma unroll
+        for (int c = 0; c < M * 3; c++) {
+            shs[q - 1][c][li] = sum[c];
+            shc[q - 1][c][li] = corr[c];
+        }
+    }
+    __syncthreads();
+    double v[M];
+#pragma unroll
+    for (int m = 0; m < M; m++) v[m] = 0.0;
+    if (q == 0 && valid) {
+        double acc[M * 3];
+#pragma unroll
+        for (int c = 0; c < M * 3; c++) {
+            R t = sum[c];
+            double u = corr[c];
+#pragma unroll
+            for (int g = 0; g < kSymQ - 1; g++) {
+                t += shs[g][c][li];
+                u += shc[g][c][li];
+            }
+            acc[c] = u - (double)t;
+        }
+        if (dodiag) {
+            const double ia = 1.0 / alpha[j];
+#pragma unroll
+            for (int m = 0; m < M; m++)
+                for (int c = 0; c < 3; c++) acc[3 * m + c] += x[m * xstride + 3L * j + c] * ia;
+        }
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            for (int c = 0; c < 3; c++) {
+                y[m * ystride + 3L * j + c] = acc[3 * m + c];
+                v[m] += acc[3 * m + c] * x[m * xstride + 3L * j + c];
+            }
+    }
+    if (!dotpart) return;  // uniform over the workgroup
+    const int w = threadIdx.x >> 6;
+    if (q == 0) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const double t = wave_sum(v[m]);
+            if ((threadIdx.x & 63) == 0) shd[m][w] = t;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        const int m = threadIdx.x;
+        static_assert(kSymBlock / 64 == 4, "four waves per site group");
+        dotpart[m * gridDim.x + blockIdx.x] = (shd[m][0] + shd[m][1]) + (shd[m][2] + shd[m][3]);
+    }
+}
+
+template <typename R, int M>
+static void sym_finalize_t(hipStream_t st, bool amoeba, int npol, const R *part, const double4 *ps64,
+                           const double *pa64, const double *alpha, const CsrDev &pp, const double *x, long xs,
+                           double *y, long ys, bool dodiag, double *dotpart, const CgState *skip, int skipM) {
+    const int nb = tmv_sym_blocks(npol);
+    if (amoeba)
+        hipLaunchKernelGGL((k_tmv_sym_finalize<R, M, true>), dim3(nb), dim3(kSymQ * kSymBlock), 0, st, npol, nb, part,
+                           ps64, pa64, alpha, pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag, dotpart, skip, skipM);
+    else
+        hipLaunchKernelGGL((k_tmv_sym_finalize<R, M, false>), dim3(nb), dim3(kSymQ * kSymBlock), 0, st, npol, nb,
+                           part, ps64, pa64, alpha, pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag, dotpart, skip,
+                           skipM);
+    HIP_CHECK(hipGetLastError());
 }
 
 int tmv_sym_blocks(int npol) { return (npol + kSymBlock - 1) / kSymBlock; }
@@ -97,22 +237,29 @@ void launch_tmv_sym(hipStream_t st, int prec, bool amoeba, int npol, int M, cons
                     double *dotpart, void *part, const int *skip_flag, hipEvent_t evb, hipEvent_t eve) {
     if (npol == 0) return;
     const CgState *skip = reinterpret_cast<const CgState *>(skip_flag);
-    const int nb = tmv_sym_blocks(npol);
     if (prec == 32) {
-        if (M == 1)
-            sym_launch_t<float, 1>(st, amoeba, npol, pairs, ps32, pa32, x, xstride, (float *)part, skip, M, evb, eve);
-        else
-            sym_launch_t<float, 2>(st, amoeba, npol, pairs, ps32, pa32, x, xstride, (float *)part, skip, M, evb, eve);
+        float *pf = (float *)part;
+        if (M == 1) {
+            sym_launch_t<float, 1>(st, amoeba, npol, pairs, ps32, pa32, x, xstride, pf, skip, M, evb, eve);
+            sym_finalize_t<float, 1>(st, amoeba, npol, pf, ps64, pa64, alpha, pp, x, xstride, y, ystride, dodiag,
+                                     dotpart, skip, M);
+        } else {
+            sym_launch_t<float, 2>(st, amoeba, npol, pairs, ps32, pa32, x, xstride, pf, skip, M, evb, eve);
+            sym_finalize_t<float, 2>(st, amoeba, npol, pf, ps64, pa64, alpha, pp, x, xstride, y, ystride, dodiag,
+                                     dotpart, skip, M);
+        }
     } else {
-        if (M == 1)
-            sym_launch_t<double, 1>(st, amoeba, npol, pairs, ps64, pa64, x, xstride, (double *)part, skip, M, evb,
-                                    eve);
-        else
-            sym_launch_t<double, 2>(st, amoeba, npol, pairs, ps64, pa64, x, xstride, (double *)part, skip, M, evb,
-                                    eve);
+        double *pd = (double *)part;
+        if (M == 1) {
+            sym_launch_t<double, 1>(st, amoeba, npol, pairs, ps64, pa64, x, xstride, pd, skip, M, evb, eve);
+            sym_finalize_t<double, 1>(st, amoeba, npol, pd, ps64, pa64, alpha, pp, x, xstride, y, ystride, dodiag,
+                                      dotpart, skip, M);
+        } else {
+            sym_launch_t<double, 2>(st, amoeba, npol, pairs, ps64, pa64, x, xstride, pd, skip, M, evb, eve);
+            sym_finalize_t<double, 2>(st, amoeba, npol, pd, ps64, pa64, alpha, pp, x, xstride, y, ystride, dodiag,
+                                      dotpart, skip, M);
+        }
     }
-    launch_tmv_finalize_prec(st, prec, amoeba, npol, M, part, -nb, ps64, pa64, alpha, pp, x, xstride, y, ystride,
-                             dodiag, dotpart, skip_flag);
 }
 
 }  // namespace ommp
