@@ -113,6 +113,10 @@ double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const do
    stats[2k+1] = total milliseconds.  stats must hold 12 entries; reset clears
    the counters. */
 void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable);
+/* Same, bracketing only every `every`-th launch of each class (each bracketed
+   launch costs a few microseconds of idle GPU for the two event records);
+   every <= 0 disables.  The stats then count the bracketed launches. */
+void ommp_amd_profile_kernels_every(OMMP_SYSTEM_PRT s, int32_t every);
 void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset);
 
 /* Timing of the last solve on the device, in milliseconds: out[0] static field,

@@ -227,7 +227,11 @@ double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const do
     return ommp::tmatvec_host(*sys(s), kind, ncols, x, y, repeat);
 }
 
-void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable) { ommp::profile_enable(*sys(s), enable); }
+void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable) { ommp::profile_enable(*sys(s), enable ? 1 : 0); }
+
+void ommp_amd_profile_kernels_every(OMMP_SYSTEM_PRT s, int32_t every) {
+    ommp::profile_enable(*sys(s), every > 0 ? every : 0);
+}
 
 void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset) {
     ommp::profile_stats(*sys(s), stats, reset);

@@ -118,7 +118,7 @@ void invalidate_geometry(System &s);
 const double *device_ipd(System &s);
 void refresh_host_m2d(System &s);
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat);
-void profile_enable(System &s, bool on);
+void profile_enable(System &s, int every);  // 0 off, n > 0: every n-th launch per class
 void profile_stats(System &s, double *out, bool reset);
 void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *ipd_out, bool out_dev,
                  double *epol);

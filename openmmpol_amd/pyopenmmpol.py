@@ -323,8 +323,10 @@ class OMMPSystem:
                                     y.ctypes.data_as(c_dp), int(repeat))
         return y, ms
 
-    def profile_kernels(self, enable=True):
-        lib().ommp_amd_profile_kernels(self.handler, bool(enable))
+    def profile_kernels(self, enable=True, every=1):
+        """Bracket pair-kernel launches with HIP events (every `every`-th launch of
+        each class when enabled)."""
+        lib().ommp_amd_profile_kernels_every(self.handler, int(every) if enable else 0)
 
     def kernel_stats(self, reset=False):
         """{class: (launches, total ms)} for otf64, otf32, incore32, incore64, static_field,
