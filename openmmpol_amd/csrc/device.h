@@ -144,7 +144,11 @@ void launch_rotation_grad(hipStream_t st, const GradArgs &a);
 void launch_norm2_guess(hipStream_t st, long n, int M, double *x, const double *alpha, double *red);
 void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double *h, double *x,
                     double *r, double *z, double *p, const double *alpha, double *red, CgState *cs,
-                    bool zero_guess, const double *tol_eps /* [tol0, tol1, eps0, eps1] */);
+                    bool zero_guess, const double *tol_eps /* [tol0, tol1, eps0, eps1] */,
+                    const CgState *pre = nullptr /* columns with pre[m].done start done */);
+// Mixed-precision refinement step decided on the device (k_refine_decide).
+void launch_refine_decide(hipStream_t st, int M, const double *rms, CgState *out, double tol, double eta,
+                          double frac, double *inner_tol, int outer);
 void launch_cg_update(hipStream_t st, long n, int M, int it, const double *h, double *x, double *r,
                       double *z, const double *p, const double *alpha, const double *red_hp,
                       int nred_hp, double *red_rz, CgState *cs);

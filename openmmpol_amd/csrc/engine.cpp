@@ -58,6 +58,10 @@ struct DeviceState {
     CgState *cs = nullptr;
     CgState *cs_host = nullptr;
     double *scal_host = nullptr;
+    // mixed solve enqueued ahead: per-column refinement state, then the inner
+    // CG state of each refinement step; pinned copy (allocated on first use)
+    CgState *cs_spec = nullptr;
+    CgState *cs_spec_host = nullptr;
     void *mvpart = nullptr;
     int2 *sym_pairs = nullptr;  // block pairs of the pair-once matvec, on first use
     void *sympart = nullptr;
@@ -722,6 +726,76 @@ struct MixedResult {
     bool ok = false;
 };
 
+// Refinement steps enqueued without host round trips, for a solve shaped like
+// the previous one: K FP64 residuals with K - 1 inner solves of the predicted
+// lengths between them.  Every decision is taken on the device
+// (k_refine_decide): a converged column's later inner solves start done and its
+// FP64 residual matvec is skipped once every column has converged.  An inner
+// solve cut short by its predicted length only leaves a larger residual for the
+// next step.  Returns -1 when there is no prediction (nothing enqueued), 0 when
+// some column has not converged within the K steps (res.inner counts the inner
+// iterations run; the caller goes on from x), K when every column converged.
+constexpr int kSpecMaxOuter = 6;
+
+int mixed_solve_ahead(System &s, DeviceState &d, int M, const double *b, double *x, bool zero_guess, double tol,
+                      int kind32, MixedResult &res) {
+    const int K = s.pred_outer;
+    if (K < 2 || K > kSpecMaxOuter) return -1;
+    for (int o = 0; o < K - 1; o++)
+        if (s.pred_iters[2 + o] <= 0) return -1;
+    const long n = d.n;
+    const int W = 2 * (kSpecMaxOuter + 1);
+    if (!d.cs_spec) {
+        d.cs_spec = dalloc<CgState>(d, W);
+        HCHECK(hipHostMalloc((void **)&d.cs_spec_host, W * sizeof(CgState), hipHostMallocDefault));
+    }
+    CgState *out = d.cs_spec;
+    const int nred_hp = site_blocks(s.npol);
+    bool xzero = zero_guess;
+    for (int o = 0; o < K; o++) {
+        if (xzero) {
+            HCHECK(hipMemsetAsync(d.vh, 0, sizeof(double) * M * n, d.stream));
+        } else {
+            matvec(s, d, kOtf64, M, x, d.vh, true, nullptr, out);
+        }
+        launch_residual_rms(d.stream, n, M, b, d.vh, d.vr2, d.alpha, d.red_a, d.scal);
+        launch_refine_decide(d.stream, M, d.scal, out, tol, refine_eta(), refine_frac(), d.tol_dev, o);
+        if (o == K - 1) break;
+        CgState *cs = d.cs_spec + 2 * (o + 1);
+        launch_cg_init(d.stream, n, M, d.vr2, d.vh, d.vd, d.vr, d.vz, d.vp, d.alpha, d.red_a, cs, true, d.tol_dev,
+                       out);
+        for (int it = 1; it <= s.pred_iters[2 + o]; it++) {
+            matvec(s, d, kind32, M, d.vp, d.vh, true, d.red_b, cs);
+            launch_cg_update(d.stream, n, M, it, d.vh, d.vd, d.vr, d.vz, d.vp, d.alpha, d.red_b, nred_hp, d.red_c,
+                             cs);
+            launch_cg_direction(d.stream, n, M, it, d.vz, d.vp, d.red_c, cs, nullptr);
+        }
+        launch_axpy(d.stream, n, M, 1.0, d.vd, x);
+        xzero = false;
+    }
+    HCHECK(hipMemcpyAsync(d.cs_spec_host, d.cs_spec, sizeof(CgState) * 2 * K, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    const CgState *h = d.cs_spec_host;
+    // next predictions: an inner solve that ran out of launches gets two more
+    for (int o = 0; o < K - 1; o++) {
+        const int ran = s.pred_iters[2 + o];
+        int most = 0;
+        for (int m = 0; m < M; m++)
+            most = std::max(most, h[2 * (o + 1) + m].done ? h[2 * (o + 1) + m].iters : ran + 2);
+        s.pred_iters[2 + o] = most;
+        res.inner += std::min(most, ran);
+    }
+    bool conv = true;
+    res.outer = 0;
+    for (int m = 0; m < M; m++) {
+        conv = conv && h[m].done;
+        res.outer = std::max(res.outer, h[m].iters);
+    }
+    if (!conv) return 0;
+    res.ok = true;
+    return K;
+}
+
 MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, double *x, bool zero_guess, double tol,
                         int maxit, int kind32) {
     const long n = d.n;
@@ -729,8 +803,16 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
     const int max_outer = 30;
     // Inner target per refinement step: max(refine_eta() * rms, refine_frac() * tol).
     const double eta_min = refine_eta(), frac = refine_frac();
-    bool xzero = zero_guess;
-    for (int outer = 0; outer < max_outer; outer++) {
+    const int K = s.pred_outer;
+    const int ahead = mixed_solve_ahead(s, d, M, b, x, zero_guess, tol, kind32, res);
+    if (ahead > 0) {
+        s.pred_outer = res.outer;
+        return res;
+    }
+    // one refinement step per host round trip, from x (after the steps enqueued
+    // ahead, if any: the last of them is redone with its inner solve)
+    bool xzero = zero_guess && ahead < 0;
+    for (int outer = ahead < 0 ? 0 : K - 1; outer < max_outer; outer++) {
         // FP64 residual r = b - T x and its preconditioned rms per column
         if (xzero) {
             HCHECK(hipMemsetAsync(d.vh, 0, sizeof(double) * M * n, d.stream));
@@ -751,6 +833,7 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
         }
         if (conv) {
             res.ok = true;
+            s.pred_outer = res.outer;
             return res;
         }
         // inner FP32 PCG on T delta = r from delta = 0
@@ -782,6 +865,7 @@ void device_release(System &s) {
     d.allocs.clear();
     d.alloc_bytes.clear();
     if (d.cs_host) (void)hipHostFree(d.cs_host);
+    if (d.cs_spec_host) (void)hipHostFree(d.cs_spec_host);
     if (d.scal_host) (void)hipHostFree(d.scal_host);
     for (auto &e : d.ev)
         if (e) (void)hipEventDestroy(e);

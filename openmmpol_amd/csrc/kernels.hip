@@ -951,31 +951,70 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(long n, const double *__rest
     if (threadIdx.x == 0) red[m * gridDim.x + blockIdx.x] = v;
 }
 
+// pre (optional): columns with pre[m].done start done (nothing to solve).
 __global__ __launch_bounds__(kBlock) void k_cg_init2(const double *__restrict__ red, int nred, CgState *cs,
-                                                     const double *__restrict__ tol) {
+                                                     const double *__restrict__ tol,
+                                                     const CgState *__restrict__ pre) {
     __shared__ double sh[4];
     const int m = blockIdx.x;
     const double g = reduce_parts(red + m * nred, nred, sh);
     if (threadIdx.x == 0) {
+        const bool skip = pre && pre[m].done;
         cs[m].g[0] = g;
         cs[m].g[1] = g;
         cs[m].rms = 0.0;
         cs[m].last_beta = 0.0;
         cs[m].tol = tol[m];
         cs[m].eps = tol[2 + m];
-        cs[m].done = 0;
-        cs[m].status = 2;
+        cs[m].done = skip ? 1 : 0;
+        cs[m].status = skip ? 0 : 2;
         cs[m].iters = 0;
     }
 }
 
 void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double *h, double *x, double *r,
                     double *z, double *p, const double *alpha, double *red, CgState *cs, bool zero_guess,
-                    const double *tol) {
+                    const double *tol, const CgState *pre) {
     const int nb = vec_blocks(n);
     hipLaunchKernelGGL(k_cg_init, dim3(nb, M), dim3(kBlock), 0, st, n, b, h, x, r, z, p, alpha, red,
                        (int)zero_guess);
-    hipLaunchKernelGGL(k_cg_init2, dim3(M), dim3(kBlock), 0, st, red, nb, cs, tol);
+    hipLaunchKernelGGL(k_cg_init2, dim3(M), dim3(kBlock), 0, st, red, nb, cs, tol, pre);
+    HIP_CHECK(hipGetLastError());
+}
+
+// One refinement step's decision on the device: the FP64 residual rms of each
+// column (rms[m]) marks the column converged (out[m].done) when below tol; a
+// converged column stays converged (later residuals of it are not computed).
+// The inner solve's tolerances go to inner_tol[0..1] (max(eta rms, frac tol)),
+// its small-step thresholds inner_tol[2..3] = 0.
+__global__ void k_refine_decide(int M, const double *__restrict__ rms, CgState *__restrict__ out, double tol,
+                                double eta, double frac, double *__restrict__ inner_tol, int outer) {
+    if (threadIdx.x != 0) return;
+    for (int m = 0; m < M; m++) {
+        if (outer == 0) {
+            out[m].done = 0;
+            out[m].status = 2;
+        }
+        if (!out[m].done) {
+            out[m].rms = rms[m];
+            out[m].iters = outer + 1;
+            if (rms[m] < tol) {
+                out[m].done = 1;
+                out[m].status = 0;
+            }
+        }
+        inner_tol[m] = out[m].done ? 0.0 : fmax(eta * out[m].rms, frac * tol);
+        inner_tol[2 + m] = 0.0;
+    }
+    if (M == 1) {
+        inner_tol[1] = inner_tol[0];
+        inner_tol[3] = 0.0;
+    }
+}
+
+void launch_refine_decide(hipStream_t st, int M, const double *rms, CgState *out, double tol, double eta,
+                          double frac, double *inner_tol, int outer) {
+    hipLaunchKernelGGL(k_refine_decide, dim3(1), dim3(64), 0, st, M, rms, out, tol, eta, frac, inner_tol, outer);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -79,6 +79,7 @@ struct System {
     // iterations is enqueued without a host round trip (slot: 0/1 FP64 D/P,
     // 2.. mixed inner solves by refinement step)
     int pred_iters[16] = {0};
+    int pred_outer = 0;            // FP64 residuals the last mixed solve needed
     int last_status = 0;
     double last_ms[3] = {0, 0, 0};
     void *user_stream = nullptr;

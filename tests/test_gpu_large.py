@@ -89,6 +89,57 @@ def test_warm_started_scf_sequence(sphere):
     assert inner[-1] < inner[0]
 
 
+def test_mixed_solves_enqueued_ahead(sphere):
+    """From the second mixed solve on, the refinement steps are enqueued without
+    host round trips with the previous solve's step and iteration counts, every
+    decision taken on the device.  Repeated cold solves give the same dipoles bit
+    for bit; a tighter tolerance than the prediction (more steps: the host-driven
+    steps take over) and a looser one (fewer steps: the rest are skipped) stay
+    at the accuracy of a fresh solve."""
+    from openmmpol_amd import synth
+    spec, g, ref = sphere
+    eref = g["cases"]["zero"]["epol"]
+    s = synth.make_system(spec)
+    s.set_precision("mixed")
+    s.set_solver_tolerance(3e-10, 500)
+    zero = np.zeros((s.pol_atoms, 3))
+    runs = []
+    for k in range(4):
+        s.reset_guess(True)
+        s.set_external_field(zero)
+        runs.append((s.get_polelec_energy(), s.ipd.copy(), list(s.last_iterations)))
+        print(f"cold solve {k}: iterations {runs[-1][2]}", flush=True)
+    for e, ipd, _ in runs:
+        assert abs(e - eref) <= 1e-9 * abs(eref)
+        assert np.abs(ipd - ref).max() <= 1e-7
+    for e, ipd, it in runs[2:]:
+        assert e == runs[1][0] and np.array_equal(ipd, runs[1][1]) and it == runs[1][2]
+    for tol in (1e-11, 3e-10, 1e-8, 3e-10):
+        s.set_solver_tolerance(tol, 500)
+        s.reset_guess(True)
+        s.set_external_field(zero)
+        e = s.get_polelec_energy()
+        print(f"tol {tol}: iterations {list(s.last_iterations)}", flush=True)
+        bar = 1e-9 if tol < 1e-9 else 1e-7
+        assert abs(e - eref) <= bar * abs(eref)
+        assert np.abs(s.ipd - ref).max() <= (1e-7 if tol < 1e-9 else 1e-5)
+    # warm-started solves after cold ones, against a fresh system
+    cq = np.array(g["qm_charges"]["pos_bohr"])
+    qq = np.array(g["qm_charges"]["q"])
+    ext = synth.point_charge_field(s.cpol, cq, qq)
+    for _ in range(3):
+        s.set_external_field(ext)
+    e = s.get_polelec_energy()
+    f = synth.make_system(spec)
+    f.set_precision("mixed")
+    f.set_solver_tolerance(3e-10, 500)
+    f.set_external_field(ext)
+    assert abs(e - f.get_polelec_energy()) <= 1e-9 * abs(e)
+    assert np.abs(s.ipd - f.ipd).max() <= 1e-7
+    s.terminate()
+    f.terminate()
+
+
 def test_multi_rhs_64_fields_matches_golden():
     """64 QM-region fields on the 15,000-site cavity system in one batched call
     (matrix-core FP32 block matvec, FP64 refinement per column) against the
