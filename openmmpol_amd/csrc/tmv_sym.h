@@ -183,4 +183,143 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
         for (int c = 0; c < 3; c++) out_i[m][c * kSymBlock + tid] = acc[m][c];
 }
 
+// Off-diagonal block pair (I < J) with two targets per lane: wave w holds the
+// targets of sub-tiles 2 (w & 1) and 2 (w & 1) + 1 of I and sweeps sub-tiles
+// 2 (w >> 1) and 2 (w >> 1) + 1 of J, so that every sub-tile pair is met by one
+// wave and each source read from LDS, and each source-side sum rotated, serves
+// two pairs.  A target's sum is split over the two waves holding it, a source's
+// over the two waves sweeping it; both are added in wave order.  Same interface,
+// LDS and outputs as sym_block_pair.
+template <typename R, int M, bool AMOEBA>
+__device__ __forceinline__ void sym_block_pair2(int npol, int I, int J, const typename V4<R>::t *__restrict__ site,
+                                                const R *__restrict__ adamp, const double *const (&xcol)[M],
+                                                R *const (&out_i)[M], R *const (&out_j)[M], char *lds) {
+    using R4 = typename V4<R>::t;
+    static_assert(kSymWaves == 4, "two target and two source halves");
+    static_assert(M <= 2, "the target-side rows reuse the sources' LDS");
+    R4 *s_pos = reinterpret_cast<R4 *>(lds);
+    R4(*s_x)[kSymBlock] = reinterpret_cast<R4(*)[kSymBlock]>(s_pos + kSymBlock);
+    R(*s_acc)[M * 3][64] = reinterpret_cast<R(*)[M * 3][64]>(s_x + M);
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int th = w & 1, sh = w >> 1;
+    const R far = R(1e15), gap = R(1e9);
+    {
+        const int jj = J * kSymBlock + tid;
+        R4 p, v[M];
+        if (jj < npol) {
+            p = site[jj];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                v[m].x = (R)xcol[m][3L * jj + 0];
+                v[m].y = (R)xcol[m][3L * jj + 1];
+                v[m].z = (R)xcol[m][3L * jj + 2];
+                v[m].w = m == 0 ? adamp[jj] : R(0);
+            }
+        } else {
+            p.x = p.y = p.z = far + gap * (R)tid;
+            p.w = R(1);
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                v[m].x = v[m].y = v[m].z = R(0);
+                v[m].w = R(0.39);
+            }
+        }
+        s_pos[tid] = p;
+#pragma unroll
+        for (int m = 0; m < M; m++) s_x[m][tid] = v[m];
+    }
+    // targets: sites 128 th + l and 128 th + 64 + l of I
+    R4 pt[2];
+    R at[2], xt[2][M][3];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int li = 128 * th + 64 * h + l, i = I * kSymBlock + li;
+        if (i < npol) {
+            pt[h] = site[i];
+            at[h] = adamp[i];
+#pragma unroll
+            for (int m = 0; m < M; m++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) xt[h][m][c] = (R)xcol[m][3L * i + c];
+        } else {
+            pt[h].x = pt[h].y = pt[h].z = far + gap * (R)li;
+            pt[h].w = R(1);
+            at[h] = R(0.39);
+#pragma unroll
+            for (int m = 0; m < M; m++) xt[h][m][0] = xt[h][m][1] = xt[h][m][2] = R(0);
+        }
+    }
+    R acc[2][M][3];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int m = 0; m < M; m++) acc[h][m][0] = acc[h][m][1] = acc[h][m][2] = R(0);
+    __syncthreads();
+    constexpr int kRow = 64 * (int)sizeof(R4);
+#pragma unroll
+    for (int kk = 0; kk < 2; kk++) {
+        const int k = 2 * sh + kk;
+        R jacc[M * 3];
+#pragma unroll
+        for (int c = 0; c < M * 3; c++) jacc[c] = R(0);
+        const char *pos_k = reinterpret_cast<const char *>(s_pos + k * 64);
+        const char *x_k = reinterpret_cast<const char *>(&s_x[0][k * 64]);
+        int jo = l * (int)sizeof(R4);
+        for (int s = 0; s < 64; s++) {
+            R4 xv[M];
+#pragma unroll
+            for (int m = 0; m < M; m++)
+                xv[m] = *reinterpret_cast<const R4 *>(x_k + m * kSymBlock * (int)sizeof(R4) + jo);
+            const R4 ps = *reinterpret_cast<const R4 *>(pos_k + jo);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                R dx, dy, dz, k2, k3, arg;
+                bool near;
+                pair_geom<R, AMOEBA, false>(pt[h], at[h], ps, xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
+                pair_accum<R, M>(acc[h], xv, dx, dy, dz, k2, k3);
+                const R nk2 = -k2;
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const R c = k3 * fmaa(xt[h][m][2], dz, fmaa(xt[h][m][1], dy, xt[h][m][0] * dx));
+                    jacc[3 * m + 0] = fmaa(c, dx, fmaa(nk2, xt[h][m][0], jacc[3 * m + 0]));
+                    jacc[3 * m + 1] = fmaa(c, dy, fmaa(nk2, xt[h][m][1], jacc[3 * m + 1]));
+                    jacc[3 * m + 2] = fmaa(c, dz, fmaa(nk2, xt[h][m][2], jacc[3 * m + 2]));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < M * 3; c++) jacc[c] = lane_next(jacc[c]);
+            jo = (jo + (int)sizeof(R4)) & (kRow - 1);
+        }
+#pragma unroll
+        for (int c = 0; c < M * 3; c++) s_acc[w][c][l] = jacc[c];
+        __syncthreads();
+        // source sides of sub-tiles kk and 2 + kk: rows of waves (0, 1) and (2, 3)
+        for (int e = tid; e < 2 * M * 3 * 64; e += kSymBlock) {
+            const int hh = e / (M * 3 * 64), r = e - hh * (M * 3 * 64), c = r >> 6, jl = r & 63;
+            const R sum = s_acc[2 * hh][c][jl] + s_acc[2 * hh + 1][c][jl];
+            out_j[c / 3][(c % 3) * kSymBlock + (2 * hh + kk) * 64 + jl] = sum;
+        }
+        __syncthreads();
+    }
+    // target sides: waves th and th + 2 hold the same targets (the sources' LDS
+    // is free now): rows [wave][half][component][lane]
+    R(*s_t)[2][M * 3][64] = reinterpret_cast<R(*)[2][M * 3][64]>(lds);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int m = 0; m < M; m++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) s_t[w][h][3 * m + c][l] = acc[h][m][c];
+    __syncthreads();
+    for (int e = tid; e < 4 * M * 3 * 64; e += kSymBlock) {
+        const int ts = e / (M * 3 * 64), r = e - ts * (M * 3 * 64), c = r >> 6, jl = r & 63;
+        const int t2 = ts >> 1, h = ts & 1;
+        const R sum = s_t[t2][h][c][jl] + s_t[t2 + 2][h][c][jl];
+        out_i[c / 3][(c % 3) * kSymBlock + ts * 64 + jl] = sum;
+    }
+    __syncthreads();
+}
+
 }  // namespace ommp

@@ -30,6 +30,9 @@ namespace ommp {
 
 static_assert(kSymBlock == kBlock, "the finalize kernel maps one block of sites per workgroup");
 
+template <typename R, int M>
+constexpr bool kSymTwoTargets = sizeof(R) == 4 && M == 2;
+
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__restrict__ pairs,
                                                        const typename V4<R>::t *__restrict__ site,
@@ -51,7 +54,12 @@ __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__r
         oi[m] = part_i + ((long)blockIdx.x * M * 3 + 3 * m) * kSymBlock;
         oj[m] = part_j + ((long)blockIdx.x * M * 3 + 3 * m) * kSymBlock;
     }
-    sym_block_pair<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
+    // two targets per lane off the diagonal (FP32, two columns: the register
+    // budget of the other widths has no room for the second target)
+    if (kSymTwoTargets<R, M> && pr.x != pr.y)
+        sym_block_pair2<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
+    else
+        sym_block_pair<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
 }
 
 // y = -(row and column block-pair partials) + list_P_P corrections (+ x / alpha),
