@@ -45,6 +45,48 @@ constexpr int sym_lds_bytes() {
     return (int)(sizeof(typename V4<R>::t) * kSymBlock * (1 + M) + sizeof(R) * kSymWaves * M * 3 * 64);
 }
 
+// Pair geometry with the AMOEBA damping argument from cubed polarizability
+// factors (vi = w_i^3, vj = w_j^3): a u^3 = a r^3 w_i^3 w_j^3 with one multiply
+// fewer than cubing u = r w_i w_j.  Wang (AMBER) damping keeps u itself.
+template <typename R, bool AMOEBA>
+__device__ __forceinline__ void pair_geom_v(const typename V4<R>::t &pj, R aj, const typename V4<R>::t &pi, R ai,
+                                            R &dx, R &dy, R &dz, R &k2, R &k3, R &arg, bool &near) {
+    if (!AMOEBA) {
+        pair_geom<R, AMOEBA, false>(pj, aj, pi, ai, dx, dy, dz, k2, k3, arg, near);
+        return;
+    }
+    dx = pj.x - pi.x;
+    dy = pj.y - pi.y;
+    dz = pj.z - pi.z;
+    const R r2 = fmaa(dz, dz, fmaa(dy, dy, dx * dx));
+    const R ir = fast_rsqrt(r2);
+    const R ir2 = ir * ir;
+    k2 = ir * ir2;
+    k3 = (R(3) * k2) * ir2;
+    const R r3 = (r2 * ir) * r2;
+    arg = pos_min(ai, aj) * (r3 * (pi.w * pj.w));
+    near = arg < R(50);
+}
+
+// Both directions' accumulation of one pair: E_i += k3 (x_j.d) d - k2 x_j at the
+// target, E_j += k3 (x_i.d) d - k2 x_i at the source, with q = k3 d formed once.
+template <typename R, int M>
+__device__ __forceinline__ void pair_accum2(R (&acc)[M][3], R *jacc, const typename V4<R>::t *xv,
+                                            const R (&xi)[M][3], R dx, R dy, R dz, R k2, R k3) {
+    const R qx = k3 * dx, qy = k3 * dy, qz = k3 * dz, nk2 = -k2;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const R c = fmaa(xv[m].z, qz, fmaa(xv[m].y, qy, xv[m].x * qx));
+        acc[m][0] = fmaa(c, dx, fmaa(nk2, xv[m].x, acc[m][0]));
+        acc[m][1] = fmaa(c, dy, fmaa(nk2, xv[m].y, acc[m][1]));
+        acc[m][2] = fmaa(c, dz, fmaa(nk2, xv[m].z, acc[m][2]));
+        const R cj = fmaa(xi[m][2], qz, fmaa(xi[m][1], qy, xi[m][0] * qx));
+        jacc[3 * m + 0] = fmaa(cj, dx, fmaa(nk2, xi[m][0], jacc[3 * m + 0]));
+        jacc[3 * m + 1] = fmaa(cj, dy, fmaa(nk2, xi[m][1], jacc[3 * m + 1]));
+        jacc[3 * m + 2] = fmaa(cj, dz, fmaa(nk2, xi[m][2], jacc[3 * m + 2]));
+    }
+}
+
 // Pair (I, J) of blocks with a kSymBlock-thread workgroup: xcol[m] = column m's
 // dipoles [npol][3] (FP64); the target-side sums of component c of column m go
 // to out_i[m][c * kSymBlock + t] (t = the site's index in I), the source-side
@@ -71,6 +113,7 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
         R4 p, v[M];
         if (jj < npol) {
             p = site[jj];
+            if (AMOEBA) p.w = p.w * p.w * p.w;  // w^3 (pair_geom_v)
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 v[m].x = (R)xcol[m][3L * jj + 0];
@@ -97,6 +140,7 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
     R aj, xi[M][3];
     if (i < npol) {
         pj = site[i];
+        if (AMOEBA) pj.w = pj.w * pj.w * pj.w;
         aj = adamp[i];
 #pragma unroll
         for (int m = 0; m < M; m++)
@@ -129,7 +173,7 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
                 for (int m = 0; m < M; m++) xv[m] = s_x[m][js];
                 R dx, dy, dz, k2, k3, arg;
                 bool near;
-                pair_geom<R, AMOEBA, false>(pj, aj, s_pos[js], xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                pair_geom_v<R, AMOEBA>(pj, aj, s_pos[js], xv[0].w, dx, dy, dz, k2, k3, arg, near);
                 if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
                 pair_accum<R, M>(acc, xv, dx, dy, dz, k2, k3);
             }
@@ -148,17 +192,9 @@ __device__ __forceinline__ void sym_block_pair(int npol, int I, int J, const typ
                 const R4 ps = *reinterpret_cast<const R4 *>(pos_k + jo);
                 R dx, dy, dz, k2, k3, arg;
                 bool near;
-                pair_geom<R, AMOEBA, false>(pj, aj, ps, xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                pair_geom_v<R, AMOEBA>(pj, aj, ps, xv[0].w, dx, dy, dz, k2, k3, arg, near);
                 if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
-                pair_accum<R, M>(acc, xv, dx, dy, dz, k2, k3);
-                const R nk2 = -k2;
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    const R c = k3 * fmaa(xi[m][2], dz, fmaa(xi[m][1], dy, xi[m][0] * dx));
-                    jacc[3 * m + 0] = fmaa(c, dx, fmaa(nk2, xi[m][0], jacc[3 * m + 0]));
-                    jacc[3 * m + 1] = fmaa(c, dy, fmaa(nk2, xi[m][1], jacc[3 * m + 1]));
-                    jacc[3 * m + 2] = fmaa(c, dz, fmaa(nk2, xi[m][2], jacc[3 * m + 2]));
-                }
+                pair_accum2<R, M>(acc, jacc, xv, xi, dx, dy, dz, k2, k3);
 #pragma unroll
                 for (int c = 0; c < M * 3; c++) jacc[c] = lane_next(jacc[c]);
                 jo = (jo + (int)sizeof(R4)) & (kRow - 1);
@@ -209,6 +245,7 @@ __device__ __forceinline__ void sym_block_pair2(int npol, int I, int J, const ty
         R4 p, v[M];
         if (jj < npol) {
             p = site[jj];
+            if (AMOEBA) p.w = p.w * p.w * p.w;  // w^3 (pair_geom_v)
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 v[m].x = (R)xcol[m][3L * jj + 0];
@@ -237,6 +274,7 @@ __device__ __forceinline__ void sym_block_pair2(int npol, int I, int J, const ty
         const int li = 128 * th + 64 * h + l, i = I * kSymBlock + li;
         if (i < npol) {
             pt[h] = site[i];
+            if (AMOEBA) pt[h].w = pt[h].w * pt[h].w * pt[h].w;
             at[h] = adamp[i];
 #pragma unroll
             for (int m = 0; m < M; m++)
@@ -276,17 +314,9 @@ __device__ __forceinline__ void sym_block_pair2(int npol, int I, int J, const ty
             for (int h = 0; h < 2; h++) {
                 R dx, dy, dz, k2, k3, arg;
                 bool near;
-                pair_geom<R, AMOEBA, false>(pt[h], at[h], ps, xv[0].w, dx, dy, dz, k2, k3, arg, near);
+                pair_geom_v<R, AMOEBA>(pt[h], at[h], ps, xv[0].w, dx, dy, dz, k2, k3, arg, near);
                 if (__ballot(near)) pair_damp<R, AMOEBA>(arg, near, k2, k3);
-                pair_accum<R, M>(acc[h], xv, dx, dy, dz, k2, k3);
-                const R nk2 = -k2;
-#pragma unroll
-                for (int m = 0; m < M; m++) {
-                    const R c = k3 * fmaa(xt[h][m][2], dz, fmaa(xt[h][m][1], dy, xt[h][m][0] * dx));
-                    jacc[3 * m + 0] = fmaa(c, dx, fmaa(nk2, xt[h][m][0], jacc[3 * m + 0]));
-                    jacc[3 * m + 1] = fmaa(c, dy, fmaa(nk2, xt[h][m][1], jacc[3 * m + 1]));
-                    jacc[3 * m + 2] = fmaa(c, dz, fmaa(nk2, xt[h][m][2], jacc[3 * m + 2]));
-                }
+                pair_accum2<R, M>(acc[h], jacc, xv, xt[h], dx, dy, dz, k2, k3);
             }
 #pragma unroll
             for (int c = 0; c < M * 3; c++) jacc[c] = lane_next(jacc[c]);
