@@ -187,8 +187,8 @@ __device__ __forceinline__ void bsym_body(const BSeg &S, const int4 w, const dou
         oi[m] = part_i + o;
         oj[m] = part_j + o;
     }
-    sym_block_pair<R, M, AMOEBA>(S.npol, w.y & 0xffff, w.y >> 16, SegSites<R>::site(S), SegSites<R>::damp(S), xcol,
-                                 oi, oj, lds);
+    sym_block_pair_any<R, M, AMOEBA>(S.npol, w.y & 0xffff, w.y >> 16, SegSites<R>::site(S), SegSites<R>::damp(S),
+                                     xcol, oi, oj, lds);
 }
 
 template <typename R, int M, bool AMOEBA>

@@ -30,12 +30,6 @@ namespace ommp {
 
 static_assert(kSymBlock == kBlock, "the finalize kernel maps one block of sites per workgroup");
 
-// Two targets per lane: FP32 (one or two columns) and FP64 with one column,
-// 4-6% faster at 15,000 sites; FP64 with two columns would need more than the
-// 128 registers that keep four workgroups per CU.
-template <typename R, int M>
-constexpr bool kSymTwoTargets = sizeof(R) == 4 || M == 1;
-
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__restrict__ pairs,
                                                        const typename V4<R>::t *__restrict__ site,
@@ -57,11 +51,7 @@ __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__r
         oi[m] = part_i + ((long)blockIdx.x * M * 3 + 3 * m) * kSymBlock;
         oj[m] = part_j + ((long)blockIdx.x * M * 3 + 3 * m) * kSymBlock;
     }
-    // two targets per lane off the diagonal
-    if (kSymTwoTargets<R, M> && pr.x != pr.y)
-        sym_block_pair2<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
-    else
-        sym_block_pair<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
+    sym_block_pair_any<R, M, AMOEBA>(npol, pr.x, pr.y, site, adamp, xcol, oi, oj, lds);
 }
 
 // y = -(row and column block-pair partials) + list_P_P corrections (+ x / alpha),
