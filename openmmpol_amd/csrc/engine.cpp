@@ -756,7 +756,8 @@ int mixed_solve_ahead(System &s, DeviceState &d, int M, const double *b, double 
         if (xzero) {
             HCHECK(hipMemsetAsync(d.vh, 0, sizeof(double) * M * n, d.stream));
         } else {
-            matvec(s, d, kOtf64, M, x, d.vh, true, nullptr, out);
+            // out still holds the previous solve's state until step 0 is decided
+            matvec(s, d, kOtf64, M, x, d.vh, true, nullptr, o == 0 ? nullptr : out);
         }
         launch_residual_rms(d.stream, n, M, b, d.vh, d.vr2, d.alpha, d.red_a, d.scal);
         launch_refine_decide(d.stream, M, d.scal, out, tol, refine_eta(), refine_frac(), d.tol_dev, o);

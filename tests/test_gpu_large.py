@@ -123,10 +123,18 @@ def test_mixed_solves_enqueued_ahead(sphere):
         bar = 1e-9 if tol < 1e-9 else 1e-7
         assert abs(e - eref) <= bar * abs(eref)
         assert np.abs(s.ipd - ref).max() <= (1e-7 if tol < 1e-9 else 1e-5)
-    # warm-started solves after cold ones, against a fresh system
+    # a warm-started solve after cold ones and an FP64 CG solve (which leaves its
+    # own work vectors behind): a small field change needs one inner solve
+    # between two FP64 residuals; then larger changes, against a fresh system
     cq = np.array(g["qm_charges"]["pos_bohr"])
     qq = np.array(g["qm_charges"]["q"])
     ext = synth.point_charge_field(s.cpol, cq, qq)
+    s.set_precision("fp64")
+    s.set_external_field(2e-3 * ext)
+    s.set_precision("mixed")
+    s.set_external_field(1e-3 * ext)
+    print(f"warm solve after cold ones: iterations {list(s.last_iterations)}", flush=True)
+    assert s.last_iterations[2] <= 2
     for _ in range(3):
         s.set_external_field(ext)
     e = s.get_polelec_energy()
