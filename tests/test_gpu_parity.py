@@ -185,6 +185,30 @@ def test_pair_once_matvec(n, radius):
     s.terminate()
 
 
+def test_pair_once_matvec_mixed_thole():
+    """AMOEBA with a Thole a that differs between sites (damping with
+    min(a_i, a_j)) at 13,000 sites: the pair-once sweep in both precisions
+    against the per-target FP64 sweep, and a visible change from the uniform
+    a = 0.39 of the other synthetic systems."""
+    from openmmpol_amd import synth
+    spec = synth.sphere_system(13000, 31.5, seed=14)
+    x = np.random.default_rng(15).normal(0.0, 0.05, size=(2, 13000, 3))
+    ys = {}
+    for name, thole in (("uniform", None), ("mixed", np.random.default_rng(16).uniform(0.3, 0.45, 13000))):
+        sp = dict(spec)
+        if thole is not None:
+            sp["thole_a"] = thole
+        s = synth.make_system(sp)
+        want, _ = s.tmatvec(x, kind="plain64")
+        for kind, tol in (("sym64", 1e-13), ("sym32", 2e-6)):
+            for ncols in (1, 2):
+                got, _ = s.tmatvec(x[:ncols], kind=kind)
+                assert np.abs(got - want[:ncols]).max() <= tol * np.abs(want).max(), (name, kind, ncols)
+        ys[name] = want
+        s.terminate()
+    assert np.abs(ys["mixed"] - ys["uniform"]).max() > 1e-6 * np.abs(ys["uniform"]).max()
+
+
 def test_pair_once_matvec_amber():
     """The AMBER (Wang linear Thole, one dipole set) form of the pair-once
     sweep against the per-target sweep at 13,000 sites (both precisions; the
