@@ -500,7 +500,7 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
         return;
     }
     if (kind == kOtf64 || kind == kOtf32) {
-        const bool sym = tmv_pair_once(s.npol, kind == kOtf32 ? 32 : 64, 2);
+        const bool sym = tmv_pair_once(s.npol, kind == kOtf32 ? 32 : 64, M);
         kind = kind == kOtf32 ? (sym ? kSym32 : kPlain32) : (sym ? kSym64 : kPlain64);
     }
     if (kind == kSym64 || kind == kSym32) {
@@ -1055,7 +1055,7 @@ size_t device_footprint_estimate(const System &s) {
     b += 3 * 24 * sizeof(double) * (n / kBlock + 16);      // reduction scratch
     const int nsplit = tmv_split(s.npol);
     b += tmv_part_bytes(s.npol, 2, 64, nsplit) + sizeof(double) * n * 16;  // pair and static-field partials
-    if (tmv_pair_once(s.npol, 64, 2)) {  // block-pair partials and list of the pair-once sweep
+    if (tmv_pair_once(s.npol, 64, 1)) {  // block-pair partials and list of the pair-once sweep (smallest size)
         const size_t nb = tmv_sym_blocks(s.npol);
         b += tmv_sym_part_bytes(s.npol, 2, 64) + sizeof(int2) * nb * (nb + 1) / 2;
     }

@@ -197,14 +197,17 @@ static void sym_finalize_t(hipStream_t st, bool amoeba, int npol, const R *part,
 int tmv_sym_blocks(int npol) { return (npol + kSymBlock - 1) / kSymBlock; }
 
 // The pair-once sweep evaluates half the pairs, at the price of a workgroup per
-// block pair and of its block-pair partial sums.  Measured on MI355X against the
-// per-target sweep (two columns): FP64 faster from 5,000 sites (-7%) to 100,000
-// (-25%); FP32 slower up to 10,000 sites (+14%), faster from 15,000 (-14%; -15%
-// at 100,000).  Used from those sizes while its partial sums for ncols columns
-// stay under max_bytes (2 GiB for two columns: up to ~100,000 sites).
+// block pair and of its block-pair partial sums; below a few thousand sites it
+// has too few workgroups to fill the chip.  Crossovers against the per-target
+// sweep measured on MI355X (whole matvec, two targets per lane): FP32 two
+// columns ~11,000 sites (-5% at 12,000, -22% at 15,000), one column ~7,000;
+// FP64 two columns ~7,000 (-8% at 8,000), one column ~6,000 (-19% at 8,000).
+// Used from those sizes while its partial sums for max(ncols, 2) columns stay
+// under max_bytes (2 GiB: up to ~100,000 sites).
 bool tmv_pair_once(int npol, int prec, int ncols, size_t max_bytes) {
-    if (npol < (prec == 32 ? 12000 : 4000)) return false;
-    return tmv_sym_part_bytes(npol, ncols, 64) <= max_bytes;
+    const int from = prec == 32 ? (ncols >= 2 ? 11000 : 7000) : (ncols >= 2 ? 7000 : 6000);
+    if (npol < from) return false;
+    return tmv_sym_part_bytes(npol, ncols > 2 ? ncols : 2, 64) <= max_bytes;
 }
 
 size_t tmv_sym_part_bytes(int npol, int M, int prec) {

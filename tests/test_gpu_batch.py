@@ -187,11 +187,11 @@ def test_server_admission_waits_for_memory(requests):
 
 # ------------------------------------------------ pair-once sweep in batches
 def test_multi_rhs_pair_once_matches_oracle(tight_oracle):
-    """A 5,000-site system (20 blocks of 256): its FP64 refinement matvec in
-    the batch runs the pair-once sweep, two columns per pass over a group of
-    four; every field against the oracle's exact solve."""
+    """A 7,500-site system (30 blocks of 256, the last one padded): its FP64
+    refinement matvec in the batch runs the pair-once sweep, two columns per
+    pass over a group of four; every field against the oracle's exact solve."""
     from openmmpol_amd import synth
-    spec = _sphere(5000, 31)
+    spec = _sphere(7500, 31)
     s = synth.make_system(spec)
     s.set_solver_tolerance(TOL, 500)
     o = oa.OracleSystem(synth.oracle_prep(spec))
@@ -211,10 +211,10 @@ def test_multi_rhs_pair_once_matches_oracle(tight_oracle):
 
 
 def test_server_pair_once_sizes_alone_and_in_mix(tight_oracle):
-    """Requests of 5,000 sites (FP64 matvec pair-once), 13,000 (both matvecs
+    """Requests of 7,500 sites (FP64 matvec pair-once), 13,000 (both matvecs
     pair-once) and 700 (per-target) together: each request's results alone and
-    in the mix are identical, and the 5,000-site one matches the oracle."""
-    specs = [_sphere(n, sd) for n, sd in ((5000, 41), (13000, 42), (700, 43))]
+    in the mix are identical, and the 7,500-site one matches the oracle."""
+    specs = [_sphere(n, sd) for n, sd in ((7500, 41), (13000, 42), (700, 43))]
     charges = [_charges(200 + k, 2) for k in range(len(specs))]
     alone = {}
     for k in range(len(specs)):
