@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU FP64 restatement of OpenMMPol's induced-point-dipole path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the GPU library is
+ * TEST INFRASTRUCTURE ONLY.  This file holds the reference results the GPU library is
  * compared against.  It is loaded by tests/, by __graft_entry__.smoke() and by
  * the cpu_baseline leg of bench.py -- never by the product library, which has
  * no CPU fallback.

@@ -1,7 +1,8 @@
 """ctypes wrapper over oracle/liboracle.so (CPU FP64 restatement of the reference).
 
 Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker, never as the thing measured.
+cpu_baseline leg of bench.py as the reference results are compared with, never as the thing
+measured.
 """
 import ctypes
 import os
