@@ -50,6 +50,12 @@ void ommp_time_push(void);
 /* :38; mod_interface.F90:343-370 -> energy_MM_pol (mod_electrostatics.F90:943-976).
    Solves with the default solver first if no dipoles are available. */
 double ommp_get_polelec_energy(OMMP_SYSTEM_PRT s);
+/* :39; mod_c_interface.F90:390-400 -> mod_interface.F90:372-383 -> energy_MM_MM
+   (mod_electrostatics.F90:899-941) on prepare_fixedelec / elec_prop_M2M
+   (:1407-1485, 1678-1952): interaction energy of the static multipoles
+   (Hartree), m-scaled by bond distance; cached until the coordinates change.
+   Computed by the exact direct sums also when fast multipoles are enabled. */
+double ommp_get_fixedelec_energy(OMMP_SYSTEM_PRT s);
 /* :40; mod_c_interface.F90:289-305 -> mod_interface.F90:184-223.  ext is
    [pol_atoms][3]; the MM field is added and both D and P sets are solved with
    the default matrix-vector method (the matv argument is ignored, as in the
@@ -115,7 +121,8 @@ void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c);
    pairs through complex solid-harmonic expansions of order lmax (static) and
    lmax_pol (dipoles), default 8, at most 20.  Any change drops the static field
    and the dipoles.  With fast multipoles the solve runs in FP64 whatever
-   ommp_amd_set_precision says; the gradient and the batched solvers stay direct.
+   ommp_amd_set_precision says, and the gradient goes through them too; the
+   batched solvers stay direct.
    Unlike the reference, which switches them on by itself above 1,000 MM atoms
    (mod_electrostatics.F90:294-300), this library computes every interaction
    directly until ommp_enable_fmm is called. */

@@ -29,6 +29,7 @@ SIGNATURES = [
     ("ommp_time_pull", None, [ctypes.c_char_p]),
     ("ommp_time_push", None, []),
     ("ommp_get_polelec_energy", ctypes.c_double, [c_vp]),
+    ("ommp_get_fixedelec_energy", ctypes.c_double, [c_vp]),
     ("ommp_set_external_field", None, [c_vp, c_dp, ctypes.c_int32, ctypes.c_int32]),
     ("ommp_set_external_field_nomm", None, [c_vp, c_dp, ctypes.c_int32, ctypes.c_int32]),
     ("ommp_ff_is_amoeba", ctypes.c_bool, [c_vp]),

@@ -11,12 +11,7 @@
 
 namespace ommp {
 
-#define BCHECK(x)                                                                             \
-    do {                                                                                      \
-        hipError_t e_ = (x);                                                                  \
-        if (e_ != hipSuccess)                                                                 \
-            fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
-    } while (0)
+#define BCHECK(x) HIP_CHECK(x)
 
 namespace {
 

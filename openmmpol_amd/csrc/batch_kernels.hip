@@ -14,15 +14,6 @@
 
 namespace ommp {
 
-#define HIP_CHECK(x)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (x);                                                             \
-        if (e_ != hipSuccess) {                                                          \
-            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
-                         __LINE__);                                                      \
-            std::abort();                                                                \
-        }                                                                                \
-    } while (0)
 
 template <typename R> struct SegSites;
 template <> struct SegSites<float> {

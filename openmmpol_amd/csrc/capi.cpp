@@ -43,6 +43,13 @@ void ommp_terminate(OMMP_SYSTEM_PRT s) {
     delete p;
 }
 
+double ommp_get_fixedelec_energy(OMMP_SYSTEM_PRT s) {
+    ommp_time_push();
+    double e = ommp::fixedelec_energy(*sys(s));
+    ommp_time_pull("Fixed Electrostatic energy");
+    return e;
+}
+
 double ommp_get_polelec_energy(OMMP_SYSTEM_PRT s) {
     ommp_time_push();
     double e = ommp::polelec_energy(*sys(s));
@@ -82,6 +89,7 @@ void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c) {
     for (int i = 0; i < p.npol; i++)
         for (int c = 0; c < 3; c++) p.cpol[3 * i + c] = p.cmm[3 * p.polar_mm[i] + c];
     p.m2d_done = false;
+    p.m2m_done = false;
     p.ipd_done = false;
     p.ipd_use_guess = false;
     if (p.amoeba) ommp::rotate_multipoles(p);
@@ -153,6 +161,7 @@ void ommp_amd_set_screening_parameters(OMMP_SYSTEM_PRT s, const double *m, const
     // device copies of the lists and every cached result depend on the scales
     ommp::device_release(sy);
     sy.m2d_done = false;
+    sy.m2m_done = false;
     sy.ipd_done = false;
     sy.dev_geometry_dirty = true;
 }

@@ -20,8 +20,20 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <string>
 
 namespace ommp {
+
+// Errors of the HIP runtime end the process the way every other error of the
+// library does: "[OMMP]   [stop] ..." and exit(1) (io.cpp, the reference's
+// fatal_error, src/mod_io.F90:155-175).
+[[noreturn]] void fatal(const std::string &msg);
+[[noreturn]] void hip_fatal(hipError_t e, const char *what, const char *file, int line);
+#define HIP_CHECK(x)                                                 \
+    do {                                                             \
+        hipError_t e_ = (x);                                         \
+        if (e_ != hipSuccess) ::ommp::hip_fatal(e_, #x, __FILE__, __LINE__); \
+    } while (0)
 
 constexpr int kBlock = 256;      // threads per block for every kernel here
 constexpr int kSymBlock = 256;   // sites per block of the pair-once matvec (= kBlock)
@@ -154,6 +166,12 @@ void launch_cg_update(hipStream_t st, long n, int M, int it, const double *h, do
                       int nred_hp, double *red_rz, CgState *cs);
 void launch_cg_direction(hipStream_t st, long n, int M, int it, const double *z, double *p,
                          const double *red_rz, CgState *cs, int *all_done);
+
+// Interaction energy of the static multipoles (energy_MM_MM), FP64: part holds
+// nsplit * nmm doubles, red site_blocks(nmm); ss = S-S list by target with
+// f = 1 - s; *bad is set if two sites coincide; *out = the energy.
+void launch_mm_energy(hipStream_t st, bool amoeba, int nmm, const double4 *ms64, const double *mq, const CsrDev &ss,
+                      double *part, int nsplit, double *red, int *bad, double *out);
 
 // Element-wise helpers.
 void launch_add_field(hipStream_t st, long n, int nsets, const double *e_m2d, const double *ext,

@@ -21,12 +21,13 @@
 
 namespace ommp {
 
-#define HCHECK(x)                                                                             \
-    do {                                                                                      \
-        hipError_t e_ = (x);                                                                  \
-        if (e_ != hipSuccess)                                                                 \
-            fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
-    } while (0)
+void hip_fatal(hipError_t e, const char *what, const char *file, int line) {
+    const char *base = std::strrchr(file, '/');
+    fatal(std::string("HIP error: ") + hipGetErrorString(e) + " (" + what + ", " + (base ? base + 1 : file) + ":" +
+          std::to_string(line) + ")");
+}
+
+#define HCHECK(x) HIP_CHECK(x)
 
 // kOtf64 / kOtf32 pick the pair-once sweep (kSym*) or the per-target one
 // (kPlain*) by size (tmv_pair_once); the explicit kinds force one of them.
@@ -48,6 +49,7 @@ struct DeviceState {
     double4 *ms64 = nullptr;
     double *ma64 = nullptr, *mq = nullptr;
     CsrDev sp, pp, pd;
+    CsrDev ss;  // S-S pairs by target, f = 1 - s (fixed-multipole energy)
     // fields, dipoles, work vectors ([2][n] each)
     double *e_m2d = nullptr, *ipd = nullptr, *ext = nullptr;
     double *vb = nullptr, *vx = nullptr, *vh = nullptr, *vr = nullptr, *vz = nullptr, *vp = nullptr;
@@ -81,6 +83,10 @@ struct DeviceState {
     double *lu_work = nullptr;
     // DIIS
     double *diis_x = nullptr, *diis_e = nullptr, *diis_coef = nullptr;
+    // fixed-multipole energy (on first use)
+    double *mm_part = nullptr, *mm_red = nullptr, *mm_out = nullptr;
+    int *mm_bad = nullptr;
+    int mm_split = 1;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<void *> allocs;
     std::vector<size_t> alloc_bytes;
@@ -261,9 +267,21 @@ void upload_lists(System &s, DeviceState &d) {
     free_csr(d, d.pp);
     free_csr(d, d.pd);
     free_csr(d, d.mp);
+    free_csr(d, d.ss);
     const int npol = s.npol;
     auto rows = sp_rows(s);
     upload_csr(d, npol, rows, d.sp, true);
+    // S-S corrections of the fixed-multipole energy (list_S_S, rows = source; transpose)
+    rows.assign(s.nmm, {});
+    for (int i = 0; i < s.nmm; i++)
+        for (int k = s.list_S_S.ri[i]; k < s.list_S_S.ri[i + 1]; k++) {
+            double2 f;
+            f.x = 1.0 - s.list_S_S.val[k];
+            f.y = 0.0;
+            rows[s.list_S_S.ci[k]].push_back({i, f});
+        }
+    for (auto &r : rows) std::sort(r.begin(), r.end(), [](auto &a, auto &b) { return a.first < b.first; });
+    upload_csr(d, s.nmm, rows, d.ss, false);
     // P->P corrections of the on-the-fly matvec (list_P_P, rows = source; transpose)
     rows.assign(npol, {});
     for (int ip = 0; ip < npol; ip++)
@@ -1035,6 +1053,42 @@ double polelec_energy(System &s) {
     HCHECK(hipMemcpyAsync(d.scal_host, d.scal, sizeof(double), hipMemcpyDeviceToHost, d.stream));
     HCHECK(hipStreamSynchronize(d.stream));
     return d.scal_host[0];
+}
+
+// energy_MM_MM (mod_electrostatics.F90:899-941) through prepare_fixedelec /
+// elec_prop_M2M (:1407-1485, 1678-1952): one FP64 sweep over all ordered MM
+// pairs, the S-S screened pairs corrected in the finalize, cached until the
+// geometry or the scale factors change.  Always the exact direct sums (also
+// when fast multipoles are enabled for the solver).
+double fixedelec_energy(System &s) {
+    if (s.nmm == 0) return 0.0;
+    if (s.m2m_done) return s.e_mm;
+    DeviceState &d = ensure_device(s);
+    if (!d.mm_part) {
+        hipDeviceProp_t prop;
+        HCHECK(hipGetDeviceProperties(&prop, d.device));
+        const int tb = site_blocks(s.nmm), ncu = prop.multiProcessorCount;
+        const int max_s = std::max(1, (s.nmm + kBlock - 1) / kBlock);
+        d.mm_split = std::min(max_s, std::max(balanced_split(tb, s.nmm, ncu), (6 * ncu + tb - 1) / tb));
+        d.mm_part = dalloc<double>(d, (size_t)d.mm_split * s.nmm);
+        d.mm_red = dalloc<double>(d, tb);
+        d.mm_out = dalloc<double>(d, 1);
+        d.mm_bad = dalloc<int>(d, 1);
+    }
+    HCHECK(hipMemsetAsync(d.mm_bad, 0, sizeof(int), d.stream));
+    launch_mm_energy(d.stream, s.amoeba, s.nmm, d.ms64, d.mq, d.ss, d.mm_part, d.mm_split, d.mm_red, d.mm_bad,
+                     d.mm_out);
+    HCHECK(hipMemcpyAsync(d.scal_host, d.mm_out, sizeof(double), hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipMemcpyAsync(d.scal_host + 1, d.mm_bad, sizeof(int), hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    int bad = 0;
+    std::memcpy(&bad, d.scal_host + 1, sizeof(int));
+    if (bad)
+        fatal("Requesting Coulomb kernel for two atoms placed in the same point, this could be an internal bug "
+              "or a problem in your input file, please check.");
+    s.e_mm = d.scal_host[0];
+    s.m2m_done = true;
+    return s.e_mm;
 }
 
 void refresh_host_ipd(System &s) {

@@ -40,15 +40,6 @@
 
 namespace ommp {
 
-#define HIP_CHECK(x)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (x);                                                             \
-        if (e_ != hipSuccess) {                                                          \
-            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
-                         __LINE__);                                                      \
-            std::abort();                                                                \
-        }                                                                                \
-    } while (0)
 
 // ------------------------------------------------------- small arithmetic
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -1051,8 +1042,7 @@ void fmm_launch_grad_fields(hipStream_t st, const FmmDev &f, bool amoeba, int ld
     const long n3 = 3L * npol;
     const int na = amoeba ? 12 : 3;
     if (ld != (amoeba ? 10 : 1)) {
-        std::fprintf(stderr, "fast multipole gradient: unexpected multipole width %d\n", ld);
-        std::abort();
+        fatal("fast multipole gradient: unexpected multipole width " + std::to_string(ld));
     }
     hipLaunchKernelGGL(k_fmm_gather_ipd, dim3((npol + 255) / 256), dim3(256), 0, st, npol, amoeba, f.pol_order, ipd,
                        f.xs);

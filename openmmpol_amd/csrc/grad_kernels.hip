@@ -35,15 +35,6 @@
 
 namespace ommp {
 
-#define HIP_CHECK(x)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (x);                                                             \
-        if (e_ != hipSuccess) {                                                          \
-            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
-                         __LINE__);                                                      \
-            std::abort();                                                                \
-        }                                                                                \
-    } while (0)
 
 // ------------------------------------------------- polarizable targets
 template <bool AMOEBA>

@@ -47,7 +47,15 @@ void message(const std::string &msg, int level, const char *prefix) {
     std::fflush(out());
 }
 
+namespace {
+thread_local bool t_fatal_throws = false;
+}
+
+FatalToException::FatalToException() : prev_(t_fatal_throws) { t_fatal_throws = true; }
+FatalToException::~FatalToException() { t_fatal_throws = prev_; }
+
 void fatal(const std::string &msg) {
+    if (t_fatal_throws) throw FatalError{msg};
     int saved = g_verbose;
     if (g_verbose < 1) g_verbose = 1;
     message(msg, 1, "stop");

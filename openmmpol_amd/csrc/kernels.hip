@@ -23,15 +23,6 @@
 
 namespace ommp {
 
-#define HIP_CHECK(x)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (x);                                                             \
-        if (e_ != hipSuccess) {                                                          \
-            std::fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, \
-                         __LINE__);                                                      \
-            std::abort();                                                                \
-        }                                                                                \
-    } while (0)
 
 int vec_blocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 int site_blocks(int npol) { return (npol + kBlock - 1) / kBlock; }
@@ -1203,6 +1194,149 @@ void launch_energy(hipStream_t st, long n, const double *ipd_d, const double *e_
     const int nb = vec_blocks(n);
     hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kBlock), 0, st, n, ipd_d, e_p, red);
     hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, st, red, nb, -0.5, out);
+    HIP_CHECK(hipGetLastError());
+}
+
+// ============================================== fixed multipole energy
+// Interaction of the multipoles of target j with the potential, field and field
+// gradient of source i, undamped (the contraction of energy_MM_MM,
+// mod_electrostatics.F90:915-933, applied to the do_V / do_E / do_grdE props of
+// q_elec_prop, mu_elec_prop and quad_elec_prop, :1119-1141, 1197-1223,
+// 1307-1345): q_j V_i - mu_j . E_i + Theta_j : G_i with the off-diagonal
+// gradient entries counted twice.  d = r_j - r_i, ir = 1 / |d|.  The gradient
+// of every source term has the form g_dd d d^T + g_I I - 3 k3 (mu d^T + d mu^T)
+// - 30 k4 (Theta d d^T + d d^T Theta) + 6 k3 Theta, so its contraction with
+// Theta_j needs d.Theta_j.d, tr Theta_j and Theta_j d only.
+template <bool AMOEBA>
+__device__ __forceinline__ double mm_pair_energy(double dx, double dy, double dz, double ir, const double *qi,
+                                                 const double *qj) {
+    const double k1 = ir;
+    if (!AMOEBA) return qj[0] * qi[0] * k1;
+    const double ir2 = ir * ir;
+    const double k2 = k1 * ir2, k3 = k2 * ir2, k4 = k3 * ir2, k5 = k4 * ir2;
+    const double mdi = qi[1] * dx + qi[2] * dy + qi[3] * dz;
+    const double qix = qi[4] * dx + qi[5] * dy + qi[7] * dz;
+    const double qiy = qi[5] * dx + qi[6] * dy + qi[8] * dz;
+    const double qiz = qi[7] * dx + qi[8] * dy + qi[9] * dz;
+    const double qrri = qix * dx + qiy * dy + qiz * dz;
+    const double mdj = qj[1] * dx + qj[2] * dy + qj[3] * dz;
+    const double qjx = qj[4] * dx + qj[5] * dy + qj[7] * dz;
+    const double qjy = qj[5] * dx + qj[6] * dy + qj[8] * dz;
+    const double qjz = qj[7] * dx + qj[8] * dy + qj[9] * dz;
+    const double qrrj = qjx * dx + qjy * dy + qjz * dz;
+    const double trj = qj[4] + qj[6] + qj[9];
+    const double V = qi[0] * k1 + mdi * k2 + 3.0 * qrri * k3;
+    const double cd = qi[0] * k2 + 3.0 * mdi * k3 + 15.0 * qrri * k4;
+    const double muE = cd * mdj - k2 * (qj[1] * qi[1] + qj[2] * qi[2] + qj[3] * qi[3]) -
+                       6.0 * k3 * (qj[1] * qix + qj[2] * qiy + qj[3] * qiz);
+    const double g_dd = 3.0 * qi[0] * k3 + 15.0 * mdi * k4 + 105.0 * qrri * k5;
+    const double g_I = -qi[0] * k2 - 3.0 * mdi * k3 - 15.0 * qrri * k4;
+    const double tq = qi[4] * qj[4] + qi[6] * qj[6] + qi[9] * qj[9] +
+                      2.0 * (qi[5] * qj[5] + qi[7] * qj[7] + qi[8] * qj[8]);
+    const double thG = g_dd * qrrj + g_I * trj - 6.0 * k3 * (qi[1] * qjx + qi[2] * qjy + qi[3] * qjz) -
+                       60.0 * k4 * (qix * qjx + qiy * qjy + qiz * qjz) + 6.0 * k3 * tq;
+    return qj[0] * V - muE + thG;
+}
+
+// One thread per MM target j, the sources of its split staged through LDS;
+// part[split][nmm] receives the unscreened sum over i != j.  Two distinct
+// sites closer than eps_rp set *bad (the reference's coulomb_kernel stops there).
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_mm_energy_pairs(int nmm, int chunk, const double4 *__restrict__ ms,
+                                                            const double *__restrict__ mq, double *__restrict__ part,
+                                                            int *__restrict__ bad) {
+    constexpr int LD = AMOEBA ? 10 : 1;
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_q[LD][kBlock];
+    const int tid = threadIdx.x;
+    const int j = blockIdx.x * kBlock + tid;
+    const bool valid = j < nmm;
+    const double4 pj = valid ? ms[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double qj[LD];
+#pragma unroll
+    for (int c = 0; c < LD; c++) qj[c] = valid ? mq[(long)LD * j + c] : 0.0;
+    double acc = 0.0;
+    bool clash = false;
+    const int i0 = blockIdx.y * chunk;
+    const int i1 = min(nmm, i0 + chunk);
+    for (int base = i0; base < i1; base += kBlock) {
+        const int cnt = min(kBlock, i1 - base);
+        __syncthreads();
+        if (tid < cnt) {
+            s_pos[tid] = ms[base + tid];
+#pragma unroll
+            for (int c = 0; c < LD; c++) s_q[c][tid] = mq[(long)LD * (base + tid) + c];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; k++) {
+            const double4 pi = s_pos[k];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const bool self = (base + k) == j;
+            const bool close = !self && r2 < kEpsRpDev * kEpsRpDev;
+            clash = clash || close;
+            const double ir = (self || close) ? 0.0 : rsqrt(r2);
+            double qi[LD];
+#pragma unroll
+            for (int c = 0; c < LD; c++) qi[c] = s_q[c][k];
+            acc += mm_pair_energy<AMOEBA>(dx, dy, dz, ir, qi, qj);
+        }
+    }
+    if (valid) {
+        part[(long)blockIdx.y * nmm + j] = acc;
+        if (clash) *bad = 1;
+    }
+}
+
+// Sum of the splits in fixed order, the listed pairs' (1 - s) share removed
+// (f = 1 - s per entry of the S-S list by target; s = 0 removes the pair), and
+// per-block partial sums of the per-site energies.
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_mm_energy_finalize(int nmm, int nsplit, const double *__restrict__ part,
+                                                               const double4 *__restrict__ ms,
+                                                               const double *__restrict__ mq,
+                                                               const int *__restrict__ ri, const int *__restrict__ ci,
+                                                               const double *__restrict__ f,
+                                                               double *__restrict__ red) {
+    constexpr int LD = AMOEBA ? 10 : 1;
+    __shared__ double sh[4];
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    double e = 0.0;
+    if (j < nmm) {
+        for (int s = 0; s < nsplit; s++) e += part[(long)s * nmm + j];
+        const double4 pj = ms[j];
+        double qj[LD];
+        for (int c = 0; c < LD; c++) qj[c] = mq[(long)LD * j + c];
+        for (int k = ri[j]; k < ri[j + 1]; k++) {
+            const int i = ci[k];
+            const double4 pi = ms[i];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            double qi[LD];
+            for (int c = 0; c < LD; c++) qi[c] = mq[(long)LD * i + c];
+            e -= f[k] * mm_pair_energy<AMOEBA>(dx, dy, dz, rsqrt(dx * dx + dy * dy + dz * dz), qi, qj);
+        }
+    }
+    e = block_sum_all(e, sh);
+    if (threadIdx.x == 0) red[blockIdx.x] = e;
+}
+
+void launch_mm_energy(hipStream_t st, bool amoeba, int nmm, const double4 *ms64, const double *mq, const CsrDev &ss,
+                      double *part, int nsplit, double *red, int *bad, double *out) {
+    if (nmm == 0) return;
+    const int chunk = (nmm + nsplit - 1) / nsplit;
+    const int nb = site_blocks(nmm);
+    dim3 grid(nb, nsplit);
+    if (amoeba) {
+        hipLaunchKernelGGL(k_mm_energy_pairs<true>, grid, dim3(kBlock), 0, st, nmm, chunk, ms64, mq, part, bad);
+        hipLaunchKernelGGL(k_mm_energy_finalize<true>, dim3(nb), dim3(kBlock), 0, st, nmm, nsplit, part, ms64, mq,
+                           ss.ri, ss.ci, ss.f, red);
+    } else {
+        hipLaunchKernelGGL(k_mm_energy_pairs<false>, grid, dim3(kBlock), 0, st, nmm, chunk, ms64, mq, part, bad);
+        hipLaunchKernelGGL(k_mm_energy_finalize<false>, dim3(nb), dim3(kBlock), 0, st, nmm, nsplit, part, ms64, mq,
+                           ss.ri, ss.ci, ss.f, red);
+    }
+    // energy_MM_MM: every pair is seen from both ends, hence 1/2
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, st, red, nb, 0.5, out);
     HIP_CHECK(hipGetLastError());
 }
 

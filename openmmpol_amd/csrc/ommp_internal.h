@@ -53,6 +53,8 @@ struct System {
     double mscale[4], pscale[4], dscale[4], uscale[4], pscale_intra[4];
     // screening lists, rows = source (reference layout), val = scale factor
     Csr list_S_P_P, list_S_P_D, list_P_P;
+    // static -> static (m-scale by bond distance), rows = source
+    Csr list_S_S;
     // P-P scale factors the dense/incore T uses: the reference builds that
     // matrix with screening_rules on every pair (mod_polarization.F90:348),
     // which for AMOEBA reaches group neighbours that are not bond neighbours.
@@ -65,6 +67,8 @@ struct System {
     int fmm_lmax_pol = 8, fmm_lmax_static = 8;
     double fmm_distance = 5.0 * kAng2Au, fmm_min_cell = 7.0 * kAng2Au;
     bool m2d_done = false, ipd_done = false, ipd_use_guess = false;
+    bool m2m_done = false;          // fixed-multipole energy cached (prepare_fixedelec)
+    double e_mm = 0.0;
     bool host_ipd_stale = false;
     std::vector<double> ipd;       // [n_ipd][npol][3] host mirror
     std::vector<double> e_m2d;     // [n_ipd][npol][3] host mirror
@@ -102,6 +106,23 @@ void set_default_scales(System &s);
 // io.cpp
 void message(const std::string &msg, int level, const char *prefix = nullptr);
 [[noreturn]] void fatal(const std::string &msg);
+// While a FatalToException lives on a thread, fatal() on that thread throws
+// FatalError instead of printing and exiting: worker threads catch it, record
+// the message and wake the caller's thread, which then calls fatal() itself,
+// so the process exits from the thread that called the library.
+struct FatalError {
+    std::string msg;
+};
+class FatalToException {
+  public:
+    FatalToException();
+    ~FatalToException();
+    FatalToException(const FatalToException &) = delete;
+    FatalToException &operator=(const FatalToException &) = delete;
+
+  private:
+    bool prev_;
+};
 int verbosity();
 
 // engine.cpp
@@ -112,6 +133,7 @@ void device_release(System &s);
 void prepare_polelec(System &s);                              // E_M2D on device
 void polarization(System &s, const double *d_field, int solver, int matv, int mask);
 double polelec_energy(System &s);
+double fixedelec_energy(System &s);                           // energy_MM_MM
 void refresh_host_ipd(System &s);
 void set_external_field(System &s, const double *ext, bool ext_on_device, int solver, int matv,
                         bool nomm);

@@ -430,6 +430,13 @@ void make_screening_lists(System &s) {
     const int n = s.nmm, npol = s.npol;
     std::vector<std::vector<std::pair<int, double>>> rows;
     auto keep = [](double sc) { return std::fabs(sc - 1.0) > kEpsRp; };
+    // static -> static: m-scale by bond distance (:620-656)
+    rows.assign(n, {});
+    for (int i = 0; i < n; i++)
+        for (int l = 0; l < 4; l++)
+            for (int k = s.conn[l].ri[i]; k < s.conn[l].ri[i + 1]; k++)
+                if (keep(s.mscale[l])) rows[i].push_back({s.conn[l].ci[k], s.mscale[l]});
+    rows_to_csr(rows, s.list_S_S, true);
     // static -> polarizable, polarization field ('P'): bond distance
     rows.assign(n, {});
     for (int i = 0; i < n; i++)

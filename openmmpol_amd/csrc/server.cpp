@@ -43,12 +43,7 @@
 
 namespace ommp {
 
-#define SCHECK(x)                                                                             \
-    do {                                                                                      \
-        hipError_t e_ = (x);                                                                  \
-        if (e_ != hipSuccess)                                                                 \
-            fatal(std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ")");          \
-    } while (0)
+#define SCHECK(x) HIP_CHECK(x)
 
 DeviceView device_view(System &s);
 size_t device_footprint_estimate(const System &s);
@@ -114,6 +109,7 @@ class Server {
         }
         cv_.notify_all();
         prep_.join();
+        if (failed_) fatal(error_);  // a worker thread failed: exit from the caller's thread
         delete engine_;
         for (auto &kv : reqs_) delete kv.second;
         (void)hipStreamDestroy(stream_);
@@ -133,7 +129,8 @@ class Server {
         r->outer.assign(nsteps, 0);
         r->inner.assign(nsteps, 0);
         {
-            std::lock_guard<std::mutex> g(mu_);
+            std::unique_lock<std::mutex> g(mu_);
+            check_failed(g);
             if (in_service_.count(s)) fatal("A system can be in only one pending or active request");
             in_service_.insert(s);
             r->id = next_id_++;
@@ -148,6 +145,7 @@ class Server {
     void wait(int id) {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&] {
+            if (failed_) return true;
             if (id >= 0) {
                 auto it = reqs_.find(id);
                 return it == reqs_.end() || it->second->done;
@@ -156,10 +154,12 @@ class Server {
                 if (!kv.second->done) return false;
             return true;
         });
+        check_failed(lk);
     }
 
     int result(int id, double *energies, double *latency_ms, int *iters) {
-        std::lock_guard<std::mutex> g(mu_);
+        std::unique_lock<std::mutex> g(mu_);
+        check_failed(g);
         auto it = reqs_.find(id);
         if (it == reqs_.end() || !it->second->done) return -1;
         Request &r = *it->second;
@@ -185,6 +185,27 @@ class Server {
     }
 
   private:
+    // A failure on the worker or preparation thread (a fatal error there is
+    // thrown as FatalError) is recorded, both threads stop, and every waiting
+    // caller wakes up to report it.
+    void fail(const std::string &msg) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            if (!failed_) error_ = msg;
+            failed_ = true;
+        }
+        cv_.notify_all();
+    }
+
+    // Called with mu_ held by the caller's thread: exits the process with the
+    // worker's message after a failure.
+    void check_failed(std::unique_lock<std::mutex> &lk) {
+        if (!failed_) return;
+        const std::string msg = error_;
+        lk.unlock();
+        fatal(msg);
+    }
+
     // Requests leave the queue while slots (admitted + in preparation) and the
     // budget allow; their estimated footprint is held from then on.
     void admit_locked(std::unique_lock<std::mutex> &lk) {
@@ -217,11 +238,20 @@ class Server {
     // Preparation thread: device state and static field of each admitted
     // system (device_view synchronizes the system's own stream).
     void prepare_loop() {
+        FatalToException guard;
+        try {
+            prepare_body();
+        } catch (const FatalError &e) {
+            fail(e.msg);
+        }
+    }
+
+    void prepare_body() {
         SCHECK(hipSetDevice(dev_));  // the creating thread's device
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
-            cv_.wait(lk, [&] { return stop_prep_ || !to_prepare_.empty(); });
-            if (to_prepare_.empty()) break;
+            cv_.wait(lk, [&] { return stop_prep_ || failed_ || !to_prepare_.empty(); });
+            if (failed_ || to_prepare_.empty()) break;
             Request *r = to_prepare_.front();
             to_prepare_.pop_front();
             lk.unlock();
@@ -393,10 +423,20 @@ class Server {
     }
 
     void run() {
+        FatalToException guard;
+        try {
+            run_body();
+        } catch (const FatalError &e) {
+            fail(e.msg);
+        }
+    }
+
+    void run_body() {
         SCHECK(hipSetDevice(dev_));  // the creating thread's device
         long prev = -1;
         std::unique_lock<std::mutex> lk(mu_);
         for (;;) {
+            if (failed_) break;
             admit_locked(lk);
             add_prepared_locked(lk);
             if (active_.empty()) {
@@ -409,7 +449,7 @@ class Server {
                 // nothing to tick: wait for a prepared system, or for a request
                 // when none is in preparation (one that is ready wakes us first)
                 cv_.wait(lk, [&] {
-                    return !prepared_.empty() || (preparing_ == 0 && (stop_ || !pending_.empty()));
+                    return failed_ || !prepared_.empty() || (preparing_ == 0 && (stop_ || !pending_.empty()));
                 });
                 continue;
             }
@@ -463,6 +503,8 @@ class Server {
     std::mutex mu_;
     std::condition_variable cv_;
     bool stop_ = false;
+    bool failed_ = false;
+    std::string error_;
     int next_id_ = 0;
     std::map<int, Request *> reqs_;
     std::deque<Request *> pending_;

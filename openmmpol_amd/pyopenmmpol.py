@@ -159,6 +159,10 @@ class OMMPSystem:
     def get_polelec_energy(self):
         return lib().ommp_get_polelec_energy(self.handler)
 
+    def get_fixedelec_energy(self):
+        """Interaction energy of the static multipoles (pyommp_interface.cpp:892-894)."""
+        return lib().ommp_get_fixedelec_energy(self.handler)
+
     # potentials and fields at external points (pyommp_interface.cpp:362-480):
     # cext [n, 3] in bohr; new zeroed arrays are returned
     def _x2e(self, fn, cext, field):

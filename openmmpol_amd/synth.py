@@ -326,3 +326,29 @@ def oracle_prep(spec):
                 adj_ri=spec["bonds_ri"], adj_ci=spec["bonds_ci"],
                 grp=spec["pgroup"] if spec.get("pgroup") is not None else np.zeros(n, dtype=np.int32),
                 td=spec.get("thole_a", np.full(n, 0.39)), scales=scales, thole_scale=0.0)
+
+
+def serving_requests(nreq=16, seed=5, mean_gap_ms=10.0):
+    """The serving workload: nreq SCF-like requests on spheres of log-uniform
+    size 500-40,000 sites (density 0.1 / A^3, 6 A cavity, system seed 1000 + k),
+    10-20 solves each, the field of step j that of a 30-charge QM region
+    (qm_charge_sets(1, 2000 + k)) with every charge displaced N(0, 0.02 A), and
+    Poisson arrivals every mean_gap_ms on average.  Returns (sizes, nsteps,
+    arrivals_s, charges, spec) with charges[k] = [nsteps[k], 30, 4] and
+    spec(k) building request k's system (seconds of host time for the largest)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(40000), size=nreq)).astype(int)
+    nsteps = rng.integers(10, 21, size=nreq)
+    gaps = rng.exponential(mean_gap_ms * 1e-3, size=nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    charges = []
+    for k in range(nreq):
+        base = qm_charge_sets(1, 2000 + k)[0]
+        steps = np.repeat(base[None], nsteps[k], axis=0)
+        steps[:, :, :3] += rng.normal(0.0, 0.02 * ANG2AU, size=(nsteps[k], base.shape[0], 3))
+        charges.append(steps)
+
+    def spec(k):
+        radius = (3.0 * sizes[k] / (4.0 * np.pi * 0.1)) ** (1.0 / 3.0) + 3.0
+        return sphere_system(int(sizes[k]), radius, seed=1000 + k, cavity=6.0)
+    return sizes, nsteps, arrivals, charges, spec

@@ -59,7 +59,7 @@ typedef struct {
     csr_t gconn[4];       /* group distance 0..3 */
     double mscale[4], pscale[4], dscale[4], uscale[4], pscale_intra[4];
     /* screening lists, row = source (reference layout) */
-    csr_t list_S_P_P, list_S_P_D, list_P_P;
+    csr_t list_S_P_P, list_S_P_D, list_P_P, list_S_S;
     /* transposed (row = target) for O(1) lookups in the pair loops */
     csr_t tS_P_P, tS_P_D, tP_P;
     int fatal;            /* set instead of exit(1) so tests can observe it */
@@ -229,6 +229,16 @@ static void make_screening_lists(osys *s)
 {
     int n = s->nmm, npol = s->npol;
     rowbuf_t *rows;
+    /* S-S (static-static, m-scale by bond distance; :620-656) */
+    rows = (rowbuf_t *)calloc(n ? n : 1, sizeof(rowbuf_t));
+    for (int i = 0; i < n; i++)
+        for (int l = 0; l < 4; l++)
+            for (int k = s->conn[l].ri[i]; k < s->conn[l].ri[i + 1]; k++) {
+                int j = s->conn[l].ci[k];
+                double sc = screening_rule(s, i, 'S', j, 'S', '-');
+                if (fabs(sc - 1.0) > EPS_RP) rb_push(&rows[i], j, sc);
+            }
+    rows_to_csr(n, rows, &s->list_S_S, 1);
     /* S-P (polarization field) */
     rows = (rowbuf_t *)calloc(n, sizeof(rowbuf_t));
     for (int i = 0; i < n; i++)
@@ -380,7 +390,7 @@ void oracle_destroy(void *h)
     free(s->thole); free(s->thole_damping); free(s->grp);
     for (int l = 0; l < 4; l++) { csr_free(&s->conn[l]); csr_free(&s->gconn[l]); }
     csr_free(&s->grp_members);
-    csr_free(&s->list_S_P_P); csr_free(&s->list_S_P_D); csr_free(&s->list_P_P);
+    csr_free(&s->list_S_P_P); csr_free(&s->list_S_P_D); csr_free(&s->list_P_P); csr_free(&s->list_S_S);
     csr_free(&s->tS_P_P); csr_free(&s->tS_P_D); csr_free(&s->tP_P);
     free(s);
 }
@@ -391,19 +401,21 @@ const char *oracle_fatal_msg(void *h) { return ((osys *)h)->msg; }
 void oracle_get_polar_mm(void *h, int *out) { osys *s = (osys *)h; memcpy(out, s->polar_mm, sizeof(int) * s->npol); }
 void oracle_get_thole(void *h, double *out) { osys *s = (osys *)h; memcpy(out, s->thole, sizeof(double) * s->nmm); }
 
-/* Number of entries / copy of a screening list: which 0=S_P_P 1=S_P_D 2=P_P 3+l=conn level l */
+/* Number of entries / copy of a screening list: which 0=S_P_P 1=S_P_D 2=P_P
+   3+l=conn level l, 7+l=group level l, 11=S_S */
+static const csr_t *list_by_id(osys *s, int which)
+{
+    return which == 0 ? &s->list_S_P_P : which == 1 ? &s->list_S_P_D : which == 2 ? &s->list_P_P :
+           which < 7 ? &s->conn[which - 3] : which < 11 ? &s->gconn[which - 7] : &s->list_S_S;
+}
 int oracle_list_nnz(void *h, int which)
 {
-    osys *s = (osys *)h;
-    const csr_t *c = which == 0 ? &s->list_S_P_P : which == 1 ? &s->list_S_P_D :
-                     which == 2 ? &s->list_P_P : which < 7 ? &s->conn[which - 3] : &s->gconn[which - 7];
+    const csr_t *c = list_by_id((osys *)h, which);
     return c->ri ? c->nnz : 0;
 }
 void oracle_list_get(void *h, int which, int *ri, int *ci, double *val)
 {
-    osys *s = (osys *)h;
-    const csr_t *c = which == 0 ? &s->list_S_P_P : which == 1 ? &s->list_S_P_D :
-                     which == 2 ? &s->list_P_P : which < 7 ? &s->conn[which - 3] : &s->gconn[which - 7];
+    const csr_t *c = list_by_id((osys *)h, which);
     if (!c->ri) return;
     memcpy(ri, c->ri, sizeof(int) * (c->n + 1));
     memcpy(ci, c->ci, sizeof(int) * (c->nnz ? c->nnz : 0));
@@ -1300,6 +1312,114 @@ int oracle_rotation_geomgrad(int nmm, const double *cmm, const double *q0, const
             }
     }
     return 0;
+}
+
+/* ------------------------------------------------ fixed multipole energy */
+/* Potential, field and field gradient (grdE, storage xx, xy, yy, xz, yz, zz) of
+   one static multipole at a target, ADDED to V, E, G: the do_V, do_E and
+   do_grdE branches of q_elec_prop / mu_elec_prop / quad_elec_prop
+   (mod_electrostatics.F90:1119-1141, 1197-1223, 1307-1345), literally.
+   k[0..4] = 1/r, 1/r^3, .., 1/r^9 (coulomb_kernel), dr = r_target - r_source. */
+static void m2m_props(const double *q, int amoeba, const double dr[3], const double *k, double *V, double E[3],
+                      double G[6])
+{
+    const double qq = q[0];
+    *V += k[0] * qq;
+    if (!amoeba) return;
+    for (int c = 0; c < 3; c++) E[c] += qq * k[1] * dr[c];
+    G[0] += qq * 3.0 * k[2] * dr[0] * dr[0] - qq * k[1];
+    G[1] += qq * 3.0 * k[2] * dr[0] * dr[1];
+    G[2] += qq * 3.0 * k[2] * dr[1] * dr[1] - qq * k[1];
+    G[3] += qq * 3.0 * k[2] * dr[0] * dr[2];
+    G[4] += qq * 3.0 * k[2] * dr[1] * dr[2];
+    G[5] += qq * 3.0 * k[2] * dr[2] * dr[2] - qq * k[1];
+    const double *mu = q + 1;
+    const double md = mu[0] * dr[0] + mu[1] * dr[1] + mu[2] * dr[2];
+    *V += md * k[1];
+    for (int c = 0; c < 3; c++) E[c] += 3.0 * md * dr[c] * k[2] - mu[c] * k[1];
+    G[0] += md * k[3] * 15.0 * dr[0] * dr[0] - (md + 2.0 * mu[0] * dr[0]) * 3.0 * k[2];
+    G[1] += md * k[3] * 15.0 * dr[0] * dr[1] - (mu[0] * dr[1] + mu[1] * dr[0]) * 3.0 * k[2];
+    G[2] += md * k[3] * 15.0 * dr[1] * dr[1] - (md + 2.0 * mu[1] * dr[1]) * 3.0 * k[2];
+    G[3] += md * k[3] * 15.0 * dr[0] * dr[2] - (mu[0] * dr[2] + mu[2] * dr[0]) * 3.0 * k[2];
+    G[4] += md * k[3] * 15.0 * dr[1] * dr[2] - (mu[1] * dr[2] + mu[2] * dr[1]) * 3.0 * k[2];
+    G[5] += md * k[3] * 15.0 * dr[2] * dr[2] - (md + 2.0 * mu[2] * dr[2]) * 3.0 * k[2];
+    const double *Q = q + 4;
+    double qr[3];
+    qr[0] = Q[0] * dr[0] + Q[1] * dr[1] + Q[3] * dr[2];
+    qr[1] = Q[1] * dr[0] + Q[2] * dr[1] + Q[4] * dr[2];
+    qr[2] = Q[3] * dr[0] + Q[4] * dr[1] + Q[5] * dr[2];
+    const double qrr = qr[0] * dr[0] + qr[1] * dr[1] + qr[2] * dr[2];
+    *V += 3.0 * qrr * k[2];
+    for (int c = 0; c < 3; c++) E[c] += 15.0 * qrr * dr[c] * k[3] - 6.0 * qr[c] * k[2];
+    G[0] += 105.0 * k[4] * qrr * dr[0] * dr[0] + 6.0 * k[2] * Q[0] - 15.0 * k[3] * (qrr + 4.0 * qr[0] * dr[0]);
+    G[1] += 105.0 * k[4] * qrr * dr[0] * dr[1] + 6.0 * k[2] * Q[1] - 30.0 * k[3] * (qr[0] * dr[1] + qr[1] * dr[0]);
+    G[2] += 105.0 * k[4] * qrr * dr[1] * dr[1] + 6.0 * k[2] * Q[2] - 15.0 * k[3] * (qrr + 4.0 * qr[1] * dr[1]);
+    G[3] += 105.0 * k[4] * qrr * dr[0] * dr[2] + 6.0 * k[2] * Q[3] - 30.0 * k[3] * (qr[0] * dr[2] + qr[2] * dr[0]);
+    G[4] += 105.0 * k[4] * qrr * dr[1] * dr[2] + 6.0 * k[2] * Q[4] - 30.0 * k[3] * (qr[1] * dr[2] + qr[2] * dr[1]);
+    G[5] += 105.0 * k[4] * qrr * dr[2] * dr[2] + 6.0 * k[2] * Q[5] - 15.0 * k[3] * (qrr + 4.0 * qr[2] * dr[2]);
+}
+
+/* energy_MM_MM (mod_electrostatics.F90:899-941) on the fields of
+   prepare_fixedelec -> elec_prop_M2M, direct branch (:1678-1952, AMOEBA
+   :1847-1899, AMBER :1900-1949): at every MM site j the potential (AMOEBA: also
+   field and field gradient) of every other site i, undamped, a pair of
+   list_S_S scaled by its factor and skipped when that is 0; then
+   E = 1/2 sum_j [q_j V_j - mu_j . E_j + Theta_j : Egrd_j] (off-diagonal
+   gradient entries counted twice).  Per-site sums are kept and added in site
+   order, so the result does not depend on the thread count.  Returns 0 and sets
+   the fatal flag if two sites coincide. */
+double oracle_energy_mm(void *h)
+{
+    osys *s = (osys *)h;
+    const int n = s->nmm, amoeba = s->amoeba, ld = s->ld_cart;
+    csr_t tSS;
+    csr_transpose(&s->list_S_S, n, &tSS); /* row = target j: sources i with a scale */
+    double *ej = (double *)calloc(n ? n : 1, sizeof(double));
+    int bad = 0;
+#pragma omp parallel reduction(| : bad)
+    {
+        double *scal = (double *)malloc(sizeof(double) * (n ? n : 1));
+        char *has = (char *)calloc(n ? n : 1, 1);
+#pragma omp for schedule(dynamic, 16)
+        for (int j = 0; j < n; j++) {
+            for (int k = tSS.ri[j]; k < tSS.ri[j + 1]; k++) { has[tSS.ci[k]] = 1; scal[tSS.ci[k]] = tSS.val[k]; }
+            double V = 0.0, E[3] = {0, 0, 0}, G[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < n; i++) {
+                if (i == j) continue;
+                const double scalf = has[i] ? scal[i] : 1.0;
+                if (has[i] && !(fabs(scalf) > EPS_RP)) continue; /* todo_S_S false */
+                double dr[3], k[5];
+                for (int c = 0; c < 3; c++) dr[c] = s->cmm[3 * j + c] - s->cmm[3 * i + c];
+                if (coulomb_kernel(dr, amoeba ? 4 : 0, k)) { bad = 1; continue; }
+                double tV = 0.0, tE[3] = {0, 0, 0}, tG[6] = {0, 0, 0, 0, 0, 0};
+                m2m_props(s->q + (size_t)ld * i, amoeba, dr, k, &tV, tE, tG);
+                V += tV * scalf;
+                for (int c = 0; c < 3; c++) E[c] += tE[c] * scalf;
+                for (int c = 0; c < 6; c++) G[c] += tG[c] * scalf;
+            }
+            for (int k = tSS.ri[j]; k < tSS.ri[j + 1]; k++) has[tSS.ci[k]] = 0;
+            const double *q = s->q + (size_t)ld * j;
+            double e = q[0] * V;
+            if (amoeba) {
+                e -= q[1] * E[0] + q[2] * E[1] + q[3] * E[2];
+                /* xx, yy, zz once; xy, xz, yz twice */
+                e += q[4] * G[0] + q[6] * G[2] + q[9] * G[5];
+                e += 2.0 * (q[5] * G[1] + q[7] * G[3] + q[8] * G[4]);
+            }
+            ej[j] = e;
+        }
+        free(scal);
+        free(has);
+    }
+    double tot = 0.0;
+    for (int j = 0; j < n; j++) tot += ej[j];
+    free(ej);
+    csr_free(&tSS);
+    if (bad) {
+        oracle_fatal(s, "Requesting Coulomb kernel for two atoms placed in the same point");
+        return 0.0;
+    }
+    return 0.5 * tot;
 }
 
 int oracle_num_threads(void)

@@ -36,6 +36,8 @@ program test_fortran_api
             ef(:, i) = efmm(:, pmm(i) + 1)
         end do
     end if
+    ! the fixed-multipole energy first, as the reference's test_SI_potential.c:132 does
+    write(*, '(A, ES24.15)') 'EM ', ommp_get_fixedelec_energy(s)
     call ommp_set_external_field(s, ef, OMMP_SOLVER_NONE, OMMP_MATV_NONE)
     ep = ommp_get_polelec_energy(s)
     write(*, '(A, ES24.15)') 'EP ', ep

@@ -8,7 +8,7 @@ gradient against), stored gzipped.  Run once in a container that has the referen
 
     python tests/golden/collect_reference_fixtures.py /root/reference
 
-Source rows: tests/test_list:3-57 of the reference.
+Source rows: tests/test_list:3-73 of the reference.
 """
 import gzip
 import os
@@ -37,6 +37,14 @@ FILES = {
         "IPD_0_AMOEBA.ref", "IPD_1_AMOEBA.ref", "IPD_0_WANG_AL.ref", "IPD_1_WANG_AL.ref",
         "ENE_0_AMOEBA.ref", "ENE_1_AMOEBA.ref", "ENE_0_WANG_AL.ref", "ENE_1_WANG_AL.ref",
         "FULL_POTENTIAL.ref",
+    ],
+    # 18,331 MM sites, 37 polarizable: the reference's only large case with many
+    # non-polarizable sources (tests/test_list:59-73; Wang-AL only, the AMOEBA
+    # CUT10 input is built from Tinker files this tree cannot load)
+    "1ao6": [
+        "input_WANG_AL_CUT10.mmp", "EF_1.txt",
+        "IPD_0_WANG_AL_CUT10.ref", "IPD_1_WANG_AL_CUT10.ref",
+        "ENE_0_WANG_AL_CUT10.ref", "ENE_1_WANG_AL_CUT10.ref",
     ],
 }
 

@@ -56,6 +56,8 @@ def lib():
         L.oracle_create_tmat.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_polarization.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip]
+        L.oracle_energy_mm.restype = ctypes.c_double
+        L.oracle_energy_mm.argtypes = [ctypes.c_void_p]
         L.oracle_energy_pol.restype = ctypes.c_double
         L.oracle_energy_pol.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_charges_field.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
@@ -130,7 +132,8 @@ class OracleSystem:
     def list_csr(self, which):
         """0=S_P_P 1=S_P_D 2=P_P, 3..6 = conn 1-2..1-5, 7..10 = group distance 0..3."""
         nnz = lib().oracle_list_nnz(self.h, which)
-        nrow = self.npol if which == 2 else (self.nmm if which < 7 else int(self.prep["grp"].max()) + 1)
+        nrow = self.npol if which == 2 else (self.nmm if which < 7 or which == 11
+                                             else int(self.prep["grp"].max()) + 1)
         ri = np.zeros(nrow + 1, dtype=np.int32)
         ci = np.zeros(max(nnz, 1), dtype=np.int32)
         val = np.zeros(max(nnz, 1))
@@ -211,6 +214,12 @@ class OracleSystem:
         ipd = np.ascontiguousarray(self.ipd)
         em = np.ascontiguousarray(self.prepare_polelec())
         return lib().oracle_energy_pol(self.h, _ptr(ipd, _dp), _ptr(em, _dp))
+
+    def energy_mm(self):
+        """energy_MM_MM: interaction energy of the static multipoles (Hartree)."""
+        e = lib().oracle_energy_mm(self.h)
+        self.check()
+        return e
 
     def energy(self, ipd):
         ipd = np.ascontiguousarray(ipd, dtype=np.float64)

@@ -40,6 +40,9 @@ def test_fortran_module_solve(name, ff, k, fixture_file, tmp_path):
     o.set_external_field(ext)
     eo = o.get_polelec_energy()
     assert abs(ep - eo) <= 1e-10 * abs(eo)
+    em = float(next(l for l in lines if l.startswith("EM")).split()[1])
+    emo = o.energy_mm()
+    assert abs(em - emo) <= 1e-12 * abs(emo)
     assert np.abs(ipd - o.ipd.reshape(-1, 3)).max() < 1e-9
     # v = V_mmpol + V_mm, e = E_mmpol + E_pol at two fixed points
     x2e = np.array([[float(v) for v in l.split()[1:]] for l in lines if l.startswith("X2E")])

@@ -1,0 +1,116 @@
+"""GPU parity on the reference's 1AO6 fixtures (18,331 MM sites, 37 of them
+polarizable: the reference's only large case with many non-polarizable
+sources; tests/test_list:59-73) and of the fixed-multipole energy
+(ommp_get_fixedelec_energy) on every fixture that prints one.
+
+Bars: the reference's ctest tolerances (IPD rtol 1e-6 atol 1e-5, energies
+rtol 1e-6 atol 1e-6, generate_cmake_tests.py:7-8, 96-99), and the oracle on the
+same input (FP64 dipoles 1e-9 a.u., energies 1e-10 / 1e-12 relative; mixed
+1e-7 a.u. and 1e-9 relative).  The reference produced the 1AO6 files with its
+automatic fast multipoles (order 8 above 1,000 MM atoms); this library's
+direct sums and its own fast multipoles both meet them.
+"""
+import numpy as np
+import pytest
+
+import mmp_io
+import oracle_api as oa
+
+pytestmark = pytest.mark.gpu
+
+NAME, FF = "1ao6", "WANG_AL_CUT10"
+
+
+@pytest.fixture(scope="module")
+def ao6():
+    o = oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path(NAME, f"input_{FF}.mmp")))
+    ef = mmp_io.read_ef(mmp_io.fixture_path(NAME, "EF_1.txt"))[o.polar_mm]
+    return o, ef
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed", "fmm"])
+@pytest.mark.parametrize("k", [0, 1])
+def test_1ao6_solve_matches_reference(mode, k, ao6, fixture_file):
+    from openmmpol_amd import OMMPSystem
+    o, ef = ao6
+    s = OMMPSystem(fixture_file(NAME, f"input_{FF}.mmp"))
+    assert s.mm_atoms == 18331 and s.pol_atoms == 37
+    if mode == "mixed":
+        s.set_precision("mixed")
+        s.set_solver_tolerance(1e-10, 500)
+    if mode == "fmm":
+        s.enable_fmm()
+    ext = ef if k else np.zeros((s.pol_atoms, 3))
+    s.set_external_field(ext)
+    ipd = s.ipd.copy()
+    ep = s.get_polelec_energy()
+    print(f"1AO6 {mode} k={k}: E_pol {ep:.12e}", flush=True)
+    ref = mmp_io.read_ipd(mmp_io.fixture_path(NAME, f"IPD_{k}_{FF}.ref"))
+    assert np.allclose(ipd.reshape(-1, 3), ref, rtol=1e-6, atol=1e-5)
+    eref = mmp_io.read_ene(mmp_io.fixture_path(NAME, f"ENE_{k}_{FF}.ref"))["EP"]
+    assert abs(ep - eref) <= 1e-6 + 1e-6 * abs(eref)
+    o.ipd_use_guess = False
+    o.set_external_field(ext)
+    eo = o.get_polelec_energy()
+    if mode == "fp64":
+        assert np.abs(ipd - o.ipd).max() < 1e-9
+        assert abs(ep - eo) <= 1e-10 * abs(eo)
+    elif mode == "mixed":
+        assert np.abs(ipd - o.ipd).max() < 1e-7
+        assert abs(ep - eo) <= 1e-9 * abs(eo)
+    s.terminate()
+
+
+@pytest.mark.parametrize("name,ff", [("N-methylacetamide", "AMOEBA"), ("N-methylacetamide", "WANG_AL"),
+                                     ("1crn", "AMOEBA"), ("1crn", "WANG_AL"),
+                                     ("1ubq", "AMOEBA"), ("1ubq", "WANG_AL"),
+                                     ("1ao6", "WANG_AL_CUT10")])
+def test_fixedelec_energy_matches_reference(name, ff, fixture_file):
+    """ommp_get_fixedelec_energy against the 'EM' line of ENE_0 (and ENE_1:
+    the external field does not enter it) and against the oracle."""
+    from openmmpol_amd import OMMPSystem
+    s = OMMPSystem(fixture_file(name, f"input_{ff}.mmp"))
+    em = s.get_fixedelec_energy()
+    print(f"{name} {ff}: E_MM {em:.12e}", flush=True)
+    for k in (0, 1):
+        eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_{k}_{ff}.ref"))["EM"]
+        assert abs(em - eref) <= 1e-6 + 1e-6 * abs(eref)
+        assert abs(em - eref) <= 1e-11 * abs(eref)
+    o = oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path(name, f"input_{ff}.mmp")))
+    emo = o.energy_mm()
+    assert abs(em - emo) <= 1e-12 * abs(emo)
+    # a solve in between does not change it; the cached value is returned again
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    assert s.get_fixedelec_energy() == em
+    s.terminate()
+
+
+def test_fixedelec_energy_follows_coordinates(fixture_file):
+    """update_coordinates drops the cached value (mod_mmpol.F90:356-361): the
+    energy of displaced coordinates (multipoles re-rotated) against the oracle."""
+    from openmmpol_amd import OMMPSystem
+    name, ff = "1crn", "AMOEBA"
+    s = OMMPSystem(fixture_file(name, f"input_{ff}.mmp"))
+    e0 = s.get_fixedelec_energy()
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path(name, f"input_{ff}.mmp"))
+    c = np.asarray(s.cmm).copy() + np.random.default_rng(2).normal(0.0, 0.05, size=(s.mm_atoms, 3))
+    s.update_coordinates(c)
+    e1 = s.get_fixedelec_energy()
+    prep["cmm"] = c
+    prep["q"] = mmp_io.rotate_multipoles(c, prep["q0"], prep["frames"])
+    emo = oa.OracleSystem(prep).energy_mm()
+    assert e1 != e0
+    assert abs(e1 - emo) <= 1e-12 * abs(emo)
+    s.terminate()
+
+
+def test_fixedelec_energy_sphere15000():
+    """The 15,000-site synthetic AMOEBA sphere (m-scale 0, 0, 0.4, 0.8) against the oracle."""
+    from openmmpol_amd import synth
+    spec = synth.sphere_system(15000, 33.0, seed=2)
+    s = synth.make_system(spec)
+    em = s.get_fixedelec_energy()
+    emo = oa.OracleSystem(synth.oracle_prep(spec)).energy_mm()
+    print(f"sphere15000: E_MM {em:.12e} oracle {emo:.12e}", flush=True)
+    assert abs(em - emo) <= 1e-12 * abs(emo)
+    s.terminate()

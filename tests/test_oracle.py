@@ -13,6 +13,7 @@ or a transposed operand fails at least one of them:
      statement of the screening rules).
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -24,6 +25,7 @@ FIXTURES = [
     ("N-methylacetamide", "AMOEBA", ""), ("N-methylacetamide", "WANG_AL", ""),
     ("N-methylacetamide", "AMOEBA", "_CUT"), ("N-methylacetamide", "WANG_AL", "_CUT"),
     ("1crn", "AMOEBA", ""), ("1crn", "WANG_AL", ""),
+    ("1ao6", "WANG_AL", "_CUT10"),
 ]
 
 
@@ -40,38 +42,43 @@ def _ext(o, name, k):
 @pytest.mark.parametrize("name,ff,cut", FIXTURES)
 @pytest.mark.parametrize("k", [0, 1])
 def test_reference_fixture_tolerances(name, ff, cut, k):
-    """Reproduces the reference's own ipd/energy ctest rows (tests/test_list:3-40):
-    IPD rtol 1e-6 atol 1e-5, energy rtol 1e-6 (generate_cmake_tests.py:7-8, 96-99)."""
+    """Reproduces the reference's own ipd/energy ctest rows (tests/test_list:3-73):
+    IPD rtol 1e-6 atol 1e-5, energy rtol 1e-6 (generate_cmake_tests.py:7-8, 96-99).
+    1AO6 (18,331 MM sites, 37 polarizable) ran with the reference's automatic
+    fast multipoles; the direct sums reproduce it."""
     o = _system(name, ff, cut)
     o.set_external_field(_ext(o, name, k))
     ep = o.get_polelec_energy()
     ref = mmp_io.read_ipd(mmp_io.fixture_path(name, f"IPD_{k}_{ff}{cut}.ref"))
     assert np.allclose(o.ipd.reshape(-1, 3), ref, rtol=1e-6, atol=1e-5)
-    if not cut:
-        eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_{k}_{ff}.ref"))["EP"]
+    ene = mmp_io.fixture_path(name, f"ENE_{k}_{ff}{cut}.ref")
+    if os.path.exists(ene):
+        eref = mmp_io.read_ene(ene)["EP"]
         assert abs(ep - eref) <= 1e-6 * abs(eref) + 1e-6
 
 
-@pytest.mark.parametrize("name,ff", [("N-methylacetamide", "AMOEBA"), ("N-methylacetamide", "WANG_AL"),
-                                     ("1crn", "AMOEBA"), ("1crn", "WANG_AL"),
-                                     ("1ubq", "AMOEBA"), ("1ubq", "WANG_AL")])
+@pytest.mark.parametrize("name,ff,cut", [("N-methylacetamide", "AMOEBA", ""), ("N-methylacetamide", "WANG_AL", ""),
+                                         ("1crn", "AMOEBA", ""), ("1crn", "WANG_AL", ""),
+                                         ("1ubq", "AMOEBA", ""), ("1ubq", "WANG_AL", ""),
+                                         ("1ao6", "WANG_AL", "_CUT10")])
 @pytest.mark.parametrize("k", [0, 1])
-def test_reference_dipoles_solve_oracle_system(name, ff, k):
+def test_reference_dipoles_solve_oracle_system(name, ff, cut, k):
     """The reference's converged dipoles must satisfy the oracle's linear system to
     the reference's own convergence criterion sqrt(r.alpha r / n) < 1e-8
     (mod_solvers.F90:188-199), and with the oracle's E_M2D they must give the
     reference's polarization energy to ~1e-12.  This pins T, both RHS sets and
     the energy formula far below the fixture tolerances.  1UBQ ran with FMM
-    (l=18) in the reference; the direct oracle still reproduces it."""
-    o = _system(name, ff)
+    (l=18) and 1AO6 with the default automatic ones (order 8) in the reference;
+    the direct oracle still reproduces them."""
+    o = _system(name, ff, cut)
     E = o.m2d()
     ext = _ext(o, name, k)
-    ipd_ref = mmp_io.read_ipd(mmp_io.fixture_path(name, f"IPD_{k}_{ff}.ref")).reshape(o.n_ipd, o.npol, 3)
+    ipd_ref = mmp_io.read_ipd(mmp_io.fixture_path(name, f"IPD_{k}_{ff}{cut}.ref")).reshape(o.n_ipd, o.npol, 3)
     alpha = o.prep["pol"][o.polar_mm][:, None]
     for s in range(o.n_ipd):
         r = E[s] + ext - o.tmatvec(ipd_ref[s])
         assert np.sqrt((r * r * alpha).sum() / r.size) < 1e-8
-    eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_{k}_{ff}.ref"))["EP"]
+    eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_{k}_{ff}{cut}.ref"))["EP"]
     assert abs(o.energy(ipd_ref) - eref) <= 1e-11 * abs(eref)
 
 
@@ -395,3 +402,43 @@ def test_x2e_coincident_point_is_fatal():
     o = oa.OracleSystem(mmp_io.prepare_mmp(mmp_io.fixture_path("N-methylacetamide", "input_AMOEBA.mmp")))
     with pytest.raises(RuntimeError, match="same point"):
         o.x2e(np.asarray(o.prep["cmm"]).reshape(-1, 3)[:1], 1)
+
+
+@pytest.mark.parametrize("name,ff,cut", [("N-methylacetamide", "AMOEBA", ""), ("N-methylacetamide", "WANG_AL", ""),
+                                         ("1crn", "AMOEBA", ""), ("1crn", "WANG_AL", ""),
+                                         ("1ubq", "AMOEBA", ""), ("1ubq", "WANG_AL", ""),
+                                         ("1ao6", "WANG_AL", "_CUT10")])
+def test_fixed_multipole_energy_matches_reference(name, ff, cut):
+    """energy_MM_MM (mod_electrostatics.F90:899-941) against the 'EM' line of the
+    reference's ENE_0_*.ref (printed to 13 significant digits; the reference's
+    energy ctest tolerance is 1e-6)."""
+    o = _system(name, ff, cut)
+    eref = mmp_io.read_ene(mmp_io.fixture_path(name, f"ENE_0_{ff}{cut}.ref"))["EM"]
+    assert abs(o.energy_mm() - eref) <= 1e-11 * abs(eref)
+
+
+def test_fixed_charge_energy_closed_form():
+    """Charges only (AMBER rules): E = sum over unordered pairs of s_ij q_i q_j / r_ij
+    with s_ij the M-M scale of the pair by ring bond distance (Wang AL: 0 for
+    1-2 and 1-3 neighbours, 1 beyond), written here from the geometry alone."""
+    rng = np.random.default_rng(4)
+    n = 12
+    cmm = rng.uniform(-6.0, 6.0, size=(n, 3))
+    adj = [[(i + 1) % n] + [(i - 1) % n] for i in range(n)]  # a ring
+    ri = np.zeros(n + 1, dtype=np.int32)
+    for i in range(n):
+        ri[i + 1] = ri[i] + len(adj[i])
+    ci = np.array([j for r in adj for j in sorted(r)], dtype=np.int32)
+    q = rng.normal(0.0, 0.4, size=(n, 1))
+    prep = dict(nmm=n, amoeba=False, cmm=cmm, q=q, pol=np.full(n, 1.2), adj_ri=ri, adj_ci=ci,
+                td=np.full(n, 0.39), grp=np.zeros(n, dtype=np.int32), scales=mmp_io.SCALES_WANG_AL.copy(),
+                thole_scale=mmp_io.THOLE_SCALE_WANG_AL)
+    o = oa.OracleSystem(prep)
+    msc = mmp_io.SCALES_WANG_AL[0]
+    want = 0.0
+    for i in range(n):
+        for j in range(i + 1, n):
+            d = min(abs(i - j), n - abs(i - j))  # ring bond distance
+            sij = msc[d - 1] if d <= 4 else 1.0
+            want += sij * q[i, 0] * q[j, 0] / np.linalg.norm(cmm[i] - cmm[j])
+    assert abs(o.energy_mm() - want) <= 1e-13 * abs(want)
