@@ -32,9 +32,11 @@ N_SITES, RADIUS, SEED = 15000, 33.0, 2
 PROF_EVERY = 4
 
 
-def pair_flops(n, m):
-    """Ordered-pair on-the-fly matvec: N(N-1)(29 + 18m) FLOP per launch."""
-    return n * (n - 1) * (29 + 18 * m)
+def sym_pair_flops(n, m):
+    """Pair-once on-the-fly matvec (each unordered pair evaluated once, both
+    directions accumulated): N(N-1)/2 (29 + 36m) FLOP per launch -- 29 for the
+    geometry and damping of the pair, 18 per column and direction."""
+    return n * (n - 1) // 2 * (29 + 36 * m)
 
 
 def grad_flops(npol, nmm):
@@ -63,7 +65,8 @@ def profile_traffic(kernel):
 
 
 def run_incore(spec, repeat):
-    """Packed in-core FP32 T (two columns) streamed from HBM: mean device time per launch."""
+    """Packed in-core FP32 T (two columns) streamed from HBM: mean device time per
+    launch of the whole matvec and of the stream kernel alone."""
     from openmmpol_amd import synth
     s = synth.make_system(spec)
     x = np.random.default_rng(1).normal(size=(2, s.pol_atoms, 3))
@@ -200,7 +203,7 @@ def run_multi(tol, steps, bufs):
     if os.path.exists(gpath):
         with open(gpath) as f:
             g = json.load(f)
-        err_e = max(abs(ep[k] - r["epol"]) / abs(r["epol"]) for k, r in enumerate(g["rhs"]))
+        err_e = max(abs(ep[r["field"]] - r["epol"]) / abs(r["epol"]) for r in g["rhs"])
         npy = os.path.join(ROOT, "tests", "golden", "multi15000_cavity_ipd0.npy")
         if os.path.exists(npy):
             err_mu = float(np.abs(ipd0 - np.load(npy)).max())
@@ -223,19 +226,8 @@ def run_serving(tol, nreq, seed, mean_gap_ms, max_active, bufs):
     500-40,000 sites; the continuous-batching server against the same requests
     run one after another through the single-field entry in plain FP64."""
     from openmmpol_amd import Server, synth
-    rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(500), np.log(40000), size=nreq)).astype(int)
-    nsteps = rng.integers(10, 21, size=nreq)
-    gaps = rng.exponential(mean_gap_ms * 1e-3, size=nreq)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    specs, charges = [], []
-    for k in range(nreq):
-        radius = (3.0 * sizes[k] / (4.0 * np.pi * 0.1)) ** (1.0 / 3.0) + 3.0
-        specs.append(synth.sphere_system(int(sizes[k]), radius, seed=1000 + k, cavity=6.0))
-        base = synth.qm_charge_sets(1, 2000 + k)[0]
-        steps = np.repeat(base[None], nsteps[k], axis=0)
-        steps[:, :, :3] += rng.normal(0.0, 0.02 * 1.8897261245650, size=(nsteps[k], base.shape[0], 3))
-        charges.append(steps)
+    sizes, nsteps, arrivals, charges, make_spec = synth.serving_requests(nreq, seed, mean_gap_ms)
+    specs = [make_spec(k) for k in range(nreq)]
     total = int(nsteps.sum())
     # ---- server
     systems = [synth.make_system(sp) for sp in specs]
@@ -285,24 +277,36 @@ def run_serving(tol, nreq, seed, mean_gap_ms, max_active, bufs):
             seq_it.append(x.last_iterations[:2])
         e_seq.append(np.array(ee))
     t_seq = time.perf_counter() - t
-    # accuracy of both against tight solves (mixed solver, tolerance 1e-12), untimed
-    e_tight = []
-    for k in range(nreq):
-        x = systems[k]
-        x.set_precision("mixed")
-        x.set_solver_tolerance(1e-12, 1000)
-        x.reset_guess(False)
-        d_f, npol = fields[k]
-        ee = []
-        for j in range(nsteps[k]):
-            x.set_external_field_device(d_f + j * 3 * npol * 8)
-            ee.append(x.get_polelec_energy())
-        e_tight.append(np.array(ee))
+    for x in systems:
         x.terminate()
-
-    def max_rel(ea):
-        return max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(ea, e_tight))
+    # accuracy: the first and last solve of four requests (the two largest
+    # among them) against the oracle's tight solves of the same fields
+    # (tests/golden/make_serving_golden.py), and every solve of the server
+    # against the same solve of the sequential plain FP64 CG
+    gpath = os.path.join(ROOT, "tests", "golden", f"serving{nreq}_seed{seed}.json")
+    err_srv = err_seq = None
+    golden_cases = 0
+    if os.path.exists(gpath):
+        with open(gpath) as f:
+            g = json.load(f)
+        if g["nreq"] == nreq and g["seed"] == seed:
+            err_srv = max(abs(e_srv[c["request"]][c["step"]] - c["epol"]) / abs(c["epol"]) for c in g["cases"])
+            err_seq = max(abs(e_seq[c["request"]][c["step"]] - c["epol"]) / abs(c["epol"]) for c in g["cases"])
+            golden_cases = len(g["cases"])
+    diff_srv_seq = max(float(np.max(np.abs(a - b) / np.abs(b))) for a, b in zip(e_srv, e_seq))
     lat = np.array(lat)
+    seq_lat = np.array(seq_lat)
+    req_size = np.concatenate([np.full(nsteps[k], sizes[k]) for k in range(nreq)])
+    by_size = {}
+    for lo, hi in ((500, 2000), (2000, 10000), (10000, 40001)):
+        sel = (req_size >= lo) & (req_size < hi)
+        if sel.any():
+            by_size[f"{lo}-{min(hi, 40000)}"] = {
+                "solves": int(sel.sum()),
+                "server_p50_ms": float(np.percentile(lat[sel], 50)),
+                "server_p99_ms": float(np.percentile(lat[sel], 99)),
+                "sequential_fp64_p50_ms": float(np.percentile(seq_lat[sel], 50)),
+                "sequential_fp64_p99_ms": float(np.percentile(seq_lat[sel], 99))}
     return {
         "workload": (f"{nreq} requests, sizes log-uniform 500-40000 sites (seed {seed}), 10-20 solves each "
                      f"({total} solves), 30-charge QM field perturbed N(0, 0.02 A) per solve, Poisson arrivals "
@@ -312,9 +316,12 @@ def run_serving(tol, nreq, seed, mean_gap_ms, max_active, bufs):
         "sequential_fp64_solves_per_s": total / t_seq,
         "sequential_fp64_p50_ms": float(np.percentile(seq_lat, 50)),
         "sequential_fp64_p99_ms": float(np.percentile(seq_lat, 99)),
+        "latency_by_request_size": by_size,
         "speedup_vs_sequential_fp64": t_seq / makespan,
-        "epol_max_rel_err_server": max_rel(e_srv),
-        "epol_max_rel_err_sequential_fp64": max_rel(e_seq),
+        "epol_max_rel_err_server_vs_oracle": err_srv,
+        "epol_max_rel_err_sequential_fp64_vs_oracle": err_seq,
+        "oracle_golden_solves": golden_cases,
+        "epol_max_rel_diff_server_vs_sequential_fp64": diff_srv_seq,
         "iterations_server_mean": {"outer_fp64": float(its[:, 0].mean()), "inner_fp32": float(its[:, 1].mean())},
         "iterations_sequential_fp64_mean": float(np.mean(seq_it)),
         "server_stats": stats,
@@ -509,31 +516,24 @@ def run_fmm(tol):
             "grad_rel_diff_vs_direct": float(np.abs(g_fmm - g_dir).max() / np.abs(g_dir).max())}
 
 
-def cpu_baseline(spec, fp64_iters):
+def cpu_baseline(spec, tol):
     """Oracle (FP64 C/OpenMP restatement of the reference) on the host cores:
-    static field once + 8 T matvecs of the same system, extrapolated to one
-    full D+P CG solve with the FP64 iteration count."""
+    one complete timed solve of the headline step -- static field, D and P
+    CG solves from a cold start to the same tolerance with the reference's
+    stopping rule, energy."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as oa
     from openmmpol_amd import synth
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     o = oa.OracleSystem(synth.oracle_prep(spec))
     t = time.perf_counter()
-    o.m2d()
-    t_rhs = time.perf_counter() - t
-    x = np.random.default_rng(0).normal(size=(o.npol, 3))
-    nmv = 8
-    t = time.perf_counter()
-    for _ in range(nmv):
-        x = 1e-3 * o.tmatvec(x)
-    t_mv = (time.perf_counter() - t) / nmv
-    n_mv = int(fp64_iters[0]) + int(fp64_iters[1]) + 2
-    t_solve = t_rhs + n_mv * t_mv
+    o.set_external_field(np.zeros((o.npol, 3)), tol=tol)
+    e = o.get_polelec_energy()
+    t_solve = time.perf_counter() - t
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": int(oa.lib().oracle_num_threads()),
-            "kind": "port",
-            "sample": (f"oracle FP64 on {cores} host cores: static field once ({t_rhs:.2f} s) + {nmv} T matvecs "
-                       f"({t_mv * 1e3:.0f} ms each) of the 15k-site system, extrapolated to one D+P CG solve of "
-                       f"{n_mv} matvecs")}
+            "kind": "port", "epol": e, "iterations": {"D": int(o.iters[0]), "P": int(o.iters[1])},
+            "sample": (f"one complete oracle FP64 solve of the 15k-site step (static field, D and P CG from a cold "
+                       f"start to tol {tol:g}, energy) on {cores} host cores, timed: {t_solve:.2f} s")}
 
 
 def main():
@@ -551,6 +551,9 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=16)
     ap.add_argument("--serving-gap-ms", type=float, default=10.0)
     ap.add_argument("--serving-slots", type=int, default=16)
+    ap.add_argument("--incore-sites", default="15000",
+                    help="comma-separated sphere sizes of the packed-T stream leg (density as the 15k sphere); "
+                         "the bench line reports the first")
     ap.add_argument("--legs", default="headline,fp64,multi,serving,incore,large,grad,fmm,cpu",
                     help="comma-separated subset to run (profiling one leg); the default runs all")
     args = ap.parse_args()
@@ -612,9 +615,20 @@ def main():
               f"FMM gradient {fmm['grad_fmm_ms']:.2f} ms vs direct {fmm['grad_direct_ms']:.2f} ms",
               file=sys.stderr, flush=True)
     incore = None
+    incore_extra = []
     if "incore" in legs:
-        incore = run_incore(spec, 20)
-        print(f"[bench] incore32 matvec: {incore[1]:.3f} ms per stream launch", file=sys.stderr, flush=True)
+        for k, ns in enumerate(int(v) for v in args.incore_sites.split(",")):
+            sp = spec if ns == N_SITES else synth.sphere_system(ns, RADIUS * (ns / N_SITES) ** (1.0 / 3.0),
+                                                                seed=SEED)
+            r = run_incore(sp, 20)
+            print(f"[bench] incore32 matvec at {ns} sites: {r[1]:.3f} ms per stream launch", file=sys.stderr,
+                  flush=True)
+            if k == 0:
+                incore = r
+            else:
+                b = incore_bytes(ns, 2, 4)
+                incore_extra.append({"n_sites": ns, "avg_ms_stream_kernel": r[1], "avg_ms_with_finalize": r[0],
+                                     "bytes_per_launch": b, "achieved_GBs": b / (r[1] * 1e-3) / 1e9})
 
     golden = load_golden()
     ref_e = golden["cases"]["zero"]["epol"] if golden else None
@@ -631,12 +645,16 @@ def main():
                          ("fmm_250k", fmm)):
             if val is not None:
                 out[key] = val
+        if incore is not None:
+            out["incore_fp32_stream"] = [{"avg_ms_stream_kernel": incore[1], "avg_ms_with_finalize": incore[0]}] + \
+                incore_extra
         print(json.dumps(out))
         return
     value = args.steps / mixed["elapsed"]
     calls, ms = mixed["stats"]["otf32"]
     avg_ms = ms / max(calls, 1)
-    flops = pair_flops(n, 2)
+    # at 15,000 sites both pair kernels are the pair-once sweep k_tmv_sym
+    flops = sym_pair_flops(n, 2)
     achieved = flops / (avg_ms * 1e-3) / 1e12 if calls else None
     out = {
         "metric": METRIC,
@@ -661,9 +679,12 @@ def main():
         "epol_oracle_golden": ref_e,
         "epol_rel_err_mixed": abs(mixed["epol"] - ref_e) / abs(ref_e) if ref_e else None,
         "mu_maxabs_err_mixed": mu_err(mixed["ipd"]),
-        "roofline": {"bound": "mfma", "compute": "vector ALU: the FP32 pair kernel is compute-bound, priced "
-                                                 "against the FP32 compute peak; algorithmic FLOP count every "
-                                                 "ordered pair, as the reference evaluates them",
+        "roofline": {"bound": "mfma", "engine": "valu",
+                     "compute": "compute-bound (the 'mfma' category of the two this line allows), executed on "
+                                "the vector ALUs, not the matrix cores: rsqrt, exp and 3x3 tensor products per "
+                                "pair; priced against the published FP32 peak; FLOP = N(N-1)/2 (29 + 36m), the "
+                                "pair-once count (each unordered pair once, m = 2 columns)",
+                     "flop_formula": "N(N-1)/2*(29+36m)",
                      "kernel": "k_tmv_sym<float,2,AMOEBA> (on-the-fly FP32 pair kernel, each unordered pair "
                                "evaluated once)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -690,7 +711,7 @@ def main():
             "kernel": "k_tpack_mv<float,2> (packed symmetric T, 64x64 site tiles)", "bound": "hbm",
             "avg_ms_stream_kernel": incore[1], "avg_ms_with_finalize": incore[0], "bytes_per_launch": b,
             "achieved": b / (incore[1] * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": b / (incore[1] * 1e-3) / 1e9 / PEAK_HBM_GBS}
+            "frac": b / (incore[1] * 1e-3) / 1e9 / PEAK_HBM_GBS, "other_sizes": incore_extra}
     if fp64 is not None:
         v64 = len(fp64["lat"]) / fp64["elapsed"]
         c64, ms64 = fp64["stats"]["otf64"]
@@ -705,13 +726,18 @@ def main():
             "epol_rel_err_fp64": abs(fp64["epol"] - ref_e) / abs(ref_e) if ref_e else None,
             "mu_maxabs_err_fp64": mu_err(fp64["ipd"]),
             "mu_maxabs_mixed_vs_fp64": float(np.abs(mixed["ipd"] - fp64["ipd"]).max()),
-            "fp64_pair_kernel": {"launches_timed": c64, "timed_every": PROF_EVERY, "avg_ms": ms64 / max(c64, 1),
-                                 "achieved_tflops": pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12
-                                 if c64 else None, "peak": PEAK_FP64_TFLOPS},
+            "fp64_pair_kernel": {"kernel": "k_tmv_sym<double,1,AMOEBA> (pair-once, one column per solve of "
+                                           "the reference's D-then-P CG)",
+                                 "launches_timed": c64, "timed_every": PROF_EVERY, "avg_ms": ms64 / max(c64, 1),
+                                 "flop_per_launch": sym_pair_flops(n, 1), "flop_formula": "N(N-1)/2*(29+36m), m=1",
+                                 "achieved_tflops": sym_pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12
+                                 if c64 else None, "peak": PEAK_FP64_TFLOPS,
+                                 "frac": sym_pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
+                                 if c64 else None},
         })
         if "cpu" in legs:
             print("[bench] cpu baseline", file=sys.stderr, flush=True)
-            out["cpu_baseline"] = cpu_baseline(spec, fp64["iters"])
+            out["cpu_baseline"] = cpu_baseline(spec, args.tol)
     print(json.dumps(out))
 
 

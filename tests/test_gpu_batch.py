@@ -234,3 +234,37 @@ def test_server_pair_once_sizes_alone_and_in_mix(tight_oracle):
         ipd, ew = _exact(o, ext)
         assert abs(mix[0][0][step] - ew) <= 1e-9 * abs(ew), step
     assert np.abs(mix[0][1] - ipd).max() < 1e-7
+
+
+def test_server_serving_workload_matches_oracle_golden():
+    """The largest request of the serving workload (39,855 sites) together with
+    a 633-site one, through the server, against the oracle's tight solves of
+    their first and last fields (tests/golden/make_serving_golden.py)."""
+    import json
+    import os
+    from openmmpol_amd import Server, synth
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "serving16_seed5.json")) as f:
+        g = json.load(f)
+    sizes, nsteps, _, charges, make_spec = synth.serving_requests(g["nreq"], g["seed"], g["mean_gap_ms"])
+    which = (9, 4)
+    systems = {k: synth.make_system(make_spec(k)) for k in which}
+    for s in systems.values():
+        s.set_solver_tolerance(3e-10, 500)
+    srv = Server(hbm_budget=200e9, max_active=16)
+    ids = {k: srv.submit(systems[k], charges[k]) for k in which}
+    srv.wait()
+    cases = [c for c in g["cases"] if c["request"] in which]
+    assert {c["request"] for c in cases} == set(which)
+    for k in which:
+        st, e, _, _ = srv.result(ids[k])
+        assert st == 0
+        ipd = systems[k].ipd
+        for c in (c for c in cases if c["request"] == k):
+            print(f"request {k} ({sizes[k]} sites) step {c['step']}: E_pol {e[c['step']]:.12e} "
+                  f"oracle {c['epol']:.12e}", flush=True)
+            assert abs(e[c["step"]] - c["epol"]) <= 1e-9 * abs(c["epol"]), (k, c["step"])
+            if c["step"] == nsteps[k] - 1:
+                assert np.abs(ipd[:, c["sample"], :] - np.array(c["ipd_sample"])).max() <= 1e-7, k
+    srv.close()
+    for s in systems.values():
+        s.terminate()

@@ -151,7 +151,8 @@ def test_mixed_solves_enqueued_ahead(sphere):
 def test_multi_rhs_64_fields_matches_golden():
     """64 QM-region fields on the 15,000-site cavity system in one batched call
     (matrix-core FP32 block matvec, FP64 refinement per column) against the
-    oracle's tight solves of the first four (tests/golden/make_multi_golden.py)."""
+    oracle's tight solves of 19 of them spread over the batch, the last column
+    group included (tests/golden/make_multi_golden.py)."""
     from openmmpol_amd import synth
     with open(os.path.join(GOLDEN, "multi15000_cavity.json")) as f:
         g = json.load(f)
@@ -163,10 +164,11 @@ def test_multi_rhs_64_fields_matches_golden():
     cpol = spec["cmm"][np.asarray(s.polar_mm)]
     ext = np.stack([synth.point_charge_field(cpol, q[:, :3], q[:, 3]) for q in charges])
     ipd, ep = s.set_external_fields_multi(ext)
-    for k, r in enumerate(g["rhs"]):
+    fields = [r["field"] for r in g["rhs"]]
+    assert len(fields) >= 16 and fields[-1] == g["nrhs"] - 1
+    for r in g["rhs"]:
+        k = r["field"]
         assert abs(ep[k] - r["epol"]) <= 1e-9 * abs(r["epol"]), k
-        assert np.abs(ipd[k][:, g["sample"], :] - np.array(r["ipd_sample"])).max() < 1e-7, k
-    assert np.abs(ipd[0] - ref0).max() < 1e-7
-    # every other column is a converged solve of its own field: energies close to RHS 0's
-    assert np.all(np.abs(ep - ep[0]) < 1e-3 * abs(ep[0]))
+        assert np.abs(ipd[k][:, g["sample"], :] - np.array(r["ipd_sample"])).max() <= 1e-7, k
+    assert np.abs(ipd[0] - ref0).max() <= 1e-7
     s.terminate()
