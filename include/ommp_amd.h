@@ -80,9 +80,23 @@ void ommp_amd_set_external_field_device(OMMP_SYSTEM_PRT s, const double *d_ext, 
 const double *ommp_amd_device_ipd(OMMP_SYSTEM_PRT s);
 /* Stream used by the library for this system (a hipStream_t; NULL = default). */
 void ommp_amd_set_stream(OMMP_SYSTEM_PRT s, void *hip_stream);
-/* Drop the warm-start guess and the cached static field, as the reference does
-   after a coordinate update (mod_mmpol.F90:356-361), without changing coordinates. */
+/* Drop the warm-start guess and, with also_static_field, the cached static
+   field and the near-field preconditioner of the mixed solver, as the reference
+   does after a coordinate update (mod_mmpol.F90:356-361), without changing
+   coordinates (the packed in-core T stays). */
 void ommp_amd_reset_guess(OMMP_SYSTEM_PRT s, bool also_static_field);
+/* Preconditioner of the inner FP32 solves of the mixed solver (the plain FP64
+   solvers always keep the reference's z = alpha r, mod_polarization.F90:535-557):
+   0 = alpha (Jacobi); 1 = near-field (default): alpha plus the first-order
+   Neumann term of the pairs closer than 3 Angstrom, z_i = 2 alpha_i r_i -
+   alpha_i sum_j T_ij alpha_j r_j, rebuilt on the device for every geometry.
+   Convergence is decided by the FP64 residual either way. */
+void ommp_amd_set_inner_preconditioner(OMMP_SYSTEM_PRT s, int32_t kind);
+/* z = M^-1 r of that near-field preconditioner for ncols (1 or 2) host vectors
+   r, z [ncols][pol_atoms][3], with the device kernel the inner solves use.
+   Returns 1, or 0 when it is switched off for this geometry (more than 64
+   neighbours per site on average) and z = alpha r. */
+int32_t ommp_amd_inner_precond_apply(OMMP_SYSTEM_PRT s, int32_t ncols, const double *r, double *z);
 
 /* Host-side introspection of the prepared state (no device work):
    which = 0 list_S_P_P, 1 list_S_P_D, 2 list_P_P (rows = source, reference layout),

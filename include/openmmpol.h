@@ -56,6 +56,8 @@ double ommp_get_polelec_energy(OMMP_SYSTEM_PRT s);
    (Hartree), m-scaled by bond distance; cached until the coordinates change.
    Computed by the exact direct sums also when fast multipoles are enabled. */
 double ommp_get_fixedelec_energy(OMMP_SYSTEM_PRT s);
+/* :63; mod_interface.F90:388-398: ommp_get_fixedelec_energy + ommp_get_polelec_energy. */
+double ommp_get_full_ele_energy(OMMP_SYSTEM_PRT s);
 /* :40; mod_c_interface.F90:289-305 -> mod_interface.F90:184-223.  ext is
    [pol_atoms][3]; the MM field is added and both D and P sets are solved with
    the default matrix-vector method (the matv argument is ignored, as in the

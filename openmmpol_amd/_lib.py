@@ -30,6 +30,7 @@ SIGNATURES = [
     ("ommp_time_push", None, []),
     ("ommp_get_polelec_energy", ctypes.c_double, [c_vp]),
     ("ommp_get_fixedelec_energy", ctypes.c_double, [c_vp]),
+    ("ommp_get_full_ele_energy", ctypes.c_double, [c_vp]),
     ("ommp_set_external_field", None, [c_vp, c_dp, ctypes.c_int32, ctypes.c_int32]),
     ("ommp_set_external_field_nomm", None, [c_vp, c_dp, ctypes.c_int32, ctypes.c_int32]),
     ("ommp_ff_is_amoeba", ctypes.c_bool, [c_vp]),
@@ -63,6 +64,8 @@ SIGNATURES = [
     ("ommp_amd_device_ipd", c_vp, [c_vp]),
     ("ommp_amd_set_stream", None, [c_vp, c_vp]),
     ("ommp_amd_reset_guess", None, [c_vp, ctypes.c_bool]),
+    ("ommp_amd_set_inner_preconditioner", None, [c_vp, ctypes.c_int32]),
+    ("ommp_amd_inner_precond_apply", ctypes.c_int32, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_amd_get_list", ctypes.c_int32, [c_vp, ctypes.c_int32, c_ip, c_ip, c_dp]),
     ("ommp_amd_get_thole", c_dp, [c_vp]),
     ("ommp_amd_get_pol", c_dp, [c_vp]),

@@ -50,6 +50,11 @@ double ommp_get_fixedelec_energy(OMMP_SYSTEM_PRT s) {
     return e;
 }
 
+// mod_interface.F90:388-398: fixed multipoles plus polarization
+double ommp_get_full_ele_energy(OMMP_SYSTEM_PRT s) {
+    return ommp_get_fixedelec_energy(s) + ommp_get_polelec_energy(s);
+}
+
 double ommp_get_polelec_energy(OMMP_SYSTEM_PRT s) {
     ommp_time_push();
     double e = ommp::polelec_energy(*sys(s));
@@ -208,7 +213,19 @@ void ommp_amd_reset_guess(OMMP_SYSTEM_PRT s, bool also_static_field) {
     System &p = *sys(s);
     p.ipd_use_guess = false;
     p.ipd_done = false;
-    if (also_static_field) p.m2d_done = false;
+    if (also_static_field) {
+        p.m2d_done = false;
+        ommp::invalidate_precond(p);
+    }
+}
+
+int32_t ommp_amd_inner_precond_apply(OMMP_SYSTEM_PRT s, int32_t ncols, const double *r, double *z) {
+    return ommp::inner_precond_apply_host(*sys(s), ncols, r, z);
+}
+
+void ommp_amd_set_inner_preconditioner(OMMP_SYSTEM_PRT s, int32_t kind) {
+    if (kind != 0 && kind != 1) ommp::fatal("Unknown preconditioner for the inner solves");
+    sys(s)->inner_precond = kind;
 }
 
 int32_t ommp_amd_get_list(OMMP_SYSTEM_PRT s, int32_t which, int32_t *ri, int32_t *ci, double *val) {

@@ -158,6 +158,39 @@ void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double
                     double *r, double *z, double *p, const double *alpha, double *red, CgState *cs,
                     bool zero_guess, const double *tol_eps /* [tol0, tol1, eps0, eps1] */,
                     const CgState *pre = nullptr /* columns with pre[m].done start done */);
+// Second half of launch_cg_init alone: CG state from nred per-block r.z partials.
+void launch_cg_init2(hipStream_t st, int M, const double *red, int nred, CgState *cs, const double *tol,
+                     const CgState *pre);
+
+// Near-field preconditioner of the inner FP32 PCG (precond.hip): neighbour
+// lists within 3 Angstrom by target, B_ij = alpha_i alpha_j s_ij (k3 d d^T - k2 I)
+// as 6 floats (xx, xy, xz, yy, yz, zz) per entry; off_flag = 1 switches it off
+// (list over capacity, or a breakdown seen by the direction kernel).
+struct NbDev {
+    int *row_ptr = nullptr, *ci = nullptr;
+    float *b = nullptr;
+    int *off_flag = nullptr;
+    int *cnt = nullptr, *off = nullptr;  // per (split, target) counts and write offsets
+    long cap = 0;
+    int nsplit = 1;
+    bool valid = false;
+};
+constexpr int kNbCapPerSite = 64;  // capacity: npol * kNbCapPerSite entries
+int nb_split(int npol);
+void launch_nb_build(hipStream_t st, bool amoeba, int npol, const float4 *ps32, const float *pa32,
+                     const double *alpha, const CsrDev &pp, int nsplit, int *cnt, int *off, int *row_ptr, long cap,
+                     int *nb_ci, float *nb_b, int *nb_off);
+// Preconditioned CG pieces (columns m < M, vectors [m * 3 npol + k]); the
+// per-block partials have site_blocks(npol) entries per column.
+void launch_cg_init_nb(hipStream_t st, int npol, int M, const double *b, double *x, double *r, double *z, double *p,
+                       const double *alpha, const NbDev &nb, double *red, CgState *cs, const double *tol,
+                       const CgState *pre);
+void launch_cg_update_nb(hipStream_t st, int npol, int M, int it, const double *h, double *x, const double *rold,
+                         double *rnew, double *z, const double *p, const double *alpha, const double *red_hp,
+                         int nred_hp, double *red_rz, double *red_ra, CgState *cs, const NbDev &nb);
+void launch_cg_direction_nb(hipStream_t st, int npol, int M, int it, const double *z, double *p,
+                            const double *red_rz, const double *red_ra, CgState *cs, const NbDev &nb);
+
 // Mixed-precision refinement step decided on the device (k_refine_decide).
 void launch_refine_decide(hipStream_t st, int M, const double *rms, CgState *out, double tol, double eta,
                           double frac, double *inner_tol, int outer);

@@ -83,6 +83,10 @@ struct DeviceState {
     double *lu_work = nullptr;
     // DIIS
     double *diis_x = nullptr, *diis_e = nullptr, *diis_coef = nullptr;
+    // near-field preconditioner of the inner FP32 solves (on first use), the
+    // second residual buffer and the r.(alpha r) partials its CG steps need
+    NbDev nb;
+    double *vr3 = nullptr, *red_d = nullptr;
     // fixed-multipole energy (on first use)
     double *mm_part = nullptr, *mm_red = nullptr, *mm_out = nullptr;
     int *mm_bad = nullptr;
@@ -335,6 +339,7 @@ void upload_geometry(System &s, DeviceState &d) {
     upload(s.q.data(), d.mq, sizeof(double) * s.ld_cart * s.nmm);
     d.tpack_valid = false;
     d.lu_valid = false;
+    d.nb.valid = false;
     s.dev_geometry_dirty = false;
 }
 
@@ -573,6 +578,28 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
     }
 }
 
+// Neighbour lists and tensors of the near-field preconditioner for the current
+// geometry (built on the device; a few microseconds at 15,000 sites).
+void ensure_nb(System &s, DeviceState &d) {
+    if (d.nb.valid) return;
+    NbDev &nb = d.nb;
+    if (!nb.row_ptr) {
+        nb.nsplit = nb_split(s.npol);
+        nb.cap = (long)s.npol * kNbCapPerSite;
+        nb.cnt = dalloc<int>(d, (size_t)nb.nsplit * s.npol);
+        nb.off = dalloc<int>(d, (size_t)nb.nsplit * s.npol);
+        nb.row_ptr = dalloc<int>(d, s.npol + 1);
+        nb.ci = dalloc<int>(d, nb.cap);
+        nb.b = dalloc<float>(d, 6 * (size_t)nb.cap);
+        nb.off_flag = dalloc<int>(d, 1);
+        d.vr3 = dalloc<double>(d, 2 * d.n);
+        d.red_d = dalloc<double>(d, 2 * (size_t)site_blocks(s.npol) + 8);
+    }
+    launch_nb_build(d.stream, s.amoeba, s.npol, d.ps32, d.pa32, d.alpha, d.pp, nb.nsplit, nb.cnt, nb.off,
+                    nb.row_ptr, nb.cap, nb.ci, nb.b, nb.off_flag);
+    nb.valid = true;
+}
+
 // conjugate_gradient_solver (mod_solvers.F90:86-216) on M columns that share
 // the matvec.  Vectors are column-strided by n.  Returns per-column status.
 struct CgResult {
@@ -582,17 +609,25 @@ struct CgResult {
     double beta[2] = {0, 0};
 };
 
+// With nb (inner solves of the mixed solver, zero guess only) the iterations
+// use the near-field preconditioner, r alternating between vr and vr3.
 CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, double *x, const double *tol,
-                  int maxit, bool zero_guess, double small_step_eps, int slot) {
+                  int maxit, bool zero_guess, double small_step_eps, int slot, bool nb = false) {
     const long n = d.n;
     CgResult res;
     double te[4] = {tol[0], M > 1 ? tol[1] : tol[0], small_step_eps, small_step_eps};
     HCHECK(hipMemcpyAsync(d.tol_dev, te, sizeof(te), hipMemcpyHostToDevice, d.stream));
+    if (nb && !zero_guess) fatal("Internal error: the near-field preconditioned solve starts from zero");
     if (!zero_guess) {
         launch_norm2_guess(d.stream, n, M, x, d.alpha, d.red_a);
         matvec(s, d, kind, M, x, d.vh, true, nullptr, nullptr);
     }
-    launch_cg_init(d.stream, n, M, b, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_a, d.cs, zero_guess, d.tol_dev);
+    double *rb[2] = {d.vr, nb ? d.vr3 : d.vr};
+    if (nb)
+        launch_cg_init_nb(d.stream, s.npol, M, b, x, rb[0], d.vz, d.vp, d.alpha, d.nb, d.red_a, d.cs, d.tol_dev,
+                          nullptr);
+    else
+        launch_cg_init(d.stream, n, M, b, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_a, d.cs, zero_guess, d.tol_dev);
     const int nred_hp = site_blocks(s.npol);
     int it = 1;
     // first batch: as many iterations as this call site needed last time
@@ -603,8 +638,15 @@ CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, d
         const int end = std::min(maxit, it + chunk - 1);
         for (; it <= end; it++) {
             matvec(s, d, kind, M, d.vp, d.vh, true, d.red_b, d.cs);
-            launch_cg_update(d.stream, n, M, it, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_b, nred_hp, d.red_c, d.cs);
-            launch_cg_direction(d.stream, n, M, it, d.vz, d.vp, d.red_c, d.cs, nullptr);
+            if (nb) {
+                launch_cg_update_nb(d.stream, s.npol, M, it, d.vh, x, rb[(it - 1) & 1], rb[it & 1], d.vz, d.vp,
+                                    d.alpha, d.red_b, nred_hp, d.red_c, d.red_d, d.cs, d.nb);
+                launch_cg_direction_nb(d.stream, s.npol, M, it, d.vz, d.vp, d.red_c, d.red_d, d.cs, d.nb);
+            } else {
+                launch_cg_update(d.stream, n, M, it, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_b, nred_hp, d.red_c,
+                                 d.cs);
+                launch_cg_direction(d.stream, n, M, it, d.vz, d.vp, d.red_c, d.cs, nullptr);
+            }
         }
         HCHECK(hipMemcpyAsync(d.cs_host, d.cs, sizeof(CgState) * M, hipMemcpyDeviceToHost, d.stream));
         HCHECK(hipStreamSynchronize(d.stream));
@@ -760,7 +802,7 @@ struct MixedResult {
 constexpr int kSpecMaxOuter = 6;
 
 int mixed_solve_ahead(System &s, DeviceState &d, int M, const double *b, double *x, bool zero_guess, double tol,
-                      int kind32, MixedResult &res) {
+                      int kind32, bool nb, MixedResult &res) {
     const int K = s.pred_outer;
     if (K < 2 || K > kSpecMaxOuter) return -1;
     for (int o = 0; o < K - 1; o++)
@@ -785,13 +827,24 @@ int mixed_solve_ahead(System &s, DeviceState &d, int M, const double *b, double 
         launch_refine_decide(d.stream, M, d.scal, out, tol, refine_eta(), refine_frac(), d.tol_dev, o);
         if (o == K - 1) break;
         CgState *cs = d.cs_spec + 2 * (o + 1);
-        launch_cg_init(d.stream, n, M, d.vr2, d.vh, d.vd, d.vr, d.vz, d.vp, d.alpha, d.red_a, cs, true, d.tol_dev,
-                       out);
+        double *rb[2] = {d.vr, nb ? d.vr3 : d.vr};
+        if (nb)
+            launch_cg_init_nb(d.stream, s.npol, M, d.vr2, d.vd, rb[0], d.vz, d.vp, d.alpha, d.nb, d.red_a, cs,
+                              d.tol_dev, out);
+        else
+            launch_cg_init(d.stream, n, M, d.vr2, d.vh, d.vd, d.vr, d.vz, d.vp, d.alpha, d.red_a, cs, true,
+                           d.tol_dev, out);
         for (int it = 1; it <= s.pred_iters[2 + o]; it++) {
             matvec(s, d, kind32, M, d.vp, d.vh, true, d.red_b, cs);
-            launch_cg_update(d.stream, n, M, it, d.vh, d.vd, d.vr, d.vz, d.vp, d.alpha, d.red_b, nred_hp, d.red_c,
-                             cs);
-            launch_cg_direction(d.stream, n, M, it, d.vz, d.vp, d.red_c, cs, nullptr);
+            if (nb) {
+                launch_cg_update_nb(d.stream, s.npol, M, it, d.vh, d.vd, rb[(it - 1) & 1], rb[it & 1], d.vz, d.vp,
+                                    d.alpha, d.red_b, nred_hp, d.red_c, d.red_d, cs, d.nb);
+                launch_cg_direction_nb(d.stream, s.npol, M, it, d.vz, d.vp, d.red_c, d.red_d, cs, d.nb);
+            } else {
+                launch_cg_update(d.stream, n, M, it, d.vh, d.vd, d.vr, d.vz, d.vp, d.alpha, d.red_b, nred_hp, d.red_c,
+                                 cs);
+                launch_cg_direction(d.stream, n, M, it, d.vz, d.vp, d.red_c, cs, nullptr);
+            }
         }
         launch_axpy(d.stream, n, M, 1.0, d.vd, x);
         xzero = false;
@@ -827,7 +880,11 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
     // Inner target per refinement step: max(refine_eta() * rms, refine_frac() * tol).
     const double eta_min = refine_eta(), frac = refine_frac();
     const int K = s.pred_outer;
-    const int ahead = mixed_solve_ahead(s, d, M, b, x, zero_guess, tol, kind32, res);
+    // near-field preconditioner for the inner solves (any SPD one will do: the
+    // FP64 residual decides convergence)
+    const bool nb = s.inner_precond == 1 && s.npol > 0;
+    if (nb) ensure_nb(s, d);
+    const int ahead = mixed_solve_ahead(s, d, M, b, x, zero_guess, tol, kind32, nb, res);
     if (ahead > 0) {
         s.pred_outer = res.outer;
         return res;
@@ -860,7 +917,7 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
             return res;
         }
         // inner FP32 PCG on T delta = r from delta = 0
-        CgResult cr = cg_solve(s, d, kind32, M, d.vr2, d.vd, tin, maxit, true, 0.0, 2 + std::min(outer, 13));
+        CgResult cr = cg_solve(s, d, kind32, M, d.vr2, d.vd, tin, maxit, true, 0.0, 2 + std::min(outer, 13), nb);
         res.inner += std::max(cr.iters[0], M > 1 ? cr.iters[1] : 0);
         launch_axpy(d.stream, n, M, 1.0, d.vd, x);
         xzero = false;
@@ -905,6 +962,10 @@ void invalidate_geometry(System &s) {
         s.dev->lu_valid = false;
         s.dev->fmm_valid = false;
     }
+}
+
+void invalidate_precond(System &s) {
+    if (s.dev) s.dev->nb.valid = false;
 }
 
 void fmm_settings_changed(System &s) {
@@ -1457,6 +1518,27 @@ double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, 
     float ms = 0;
     (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);
     return ms / repeat;
+}
+
+// z = M^-1 r of the inner solves' near-field preconditioner for ncols host
+// vectors (the kernel the inner solves start with); returns 1 when it is in
+// use, 0 when it is switched off for this geometry (then z = alpha r).
+int inner_precond_apply_host(System &s, int ncols, const double *r, double *z) {
+    if (ncols < 1 || ncols > 2) fatal("The preconditioner applies to 1 or 2 columns");
+    if (s.npol == 0) return 0;
+    DeviceState &d = ensure_device(s);
+    ensure_nb(s, d);
+    const long n = d.n;
+    const double te[4] = {0.0, 0.0, 0.0, 0.0};
+    HCHECK(hipMemcpyAsync(d.tol_dev, te, sizeof(te), hipMemcpyHostToDevice, d.stream));
+    HCHECK(hipMemcpyAsync(d.vr2, r, sizeof(double) * ncols * n, hipMemcpyHostToDevice, d.stream));
+    launch_cg_init_nb(d.stream, s.npol, ncols, d.vr2, d.vd, d.vr, d.vz, d.vp, d.alpha, d.nb, d.red_a, d.cs,
+                      d.tol_dev, nullptr);
+    int off = 0;
+    HCHECK(hipMemcpyAsync(z, d.vz, sizeof(double) * ncols * n, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipMemcpyAsync(&off, d.nb.off_flag, sizeof(int), hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    return off ? 0 : 1;
 }
 
 void profile_enable(System &s, int every) {

@@ -973,6 +973,12 @@ void launch_cg_init(hipStream_t st, long n, int M, const double *b, const double
     HIP_CHECK(hipGetLastError());
 }
 
+void launch_cg_init2(hipStream_t st, int M, const double *red, int nred, CgState *cs, const double *tol,
+                     const CgState *pre) {
+    hipLaunchKernelGGL(k_cg_init2, dim3(M), dim3(kBlock), 0, st, red, nred, cs, tol, pre);
+    HIP_CHECK(hipGetLastError());
+}
+
 // One refinement step's decision on the device: the FP64 residual rms of each
 // column (rms[m]) marks the column converged (out[m].done) when below tol; a
 // converged column stays converged (later residuals of it are not computed).

@@ -78,6 +78,7 @@ struct System {
     double tol = 1e-8;
     int max_iter = 200;
     int precision = 0;             // OMMP_AMD_PREC_*
+    int inner_precond = 1;         // inner FP32 solves: 0 alpha (Jacobi), 1 near-field (default)
     int last_iters[4] = {0, 0, 0, 0};
     // iterations each CG call site took last time: the first batch of
     // iterations is enqueued without a host round trip (slot: 0/1 FP64 D/P,
@@ -138,9 +139,11 @@ void refresh_host_ipd(System &s);
 void set_external_field(System &s, const double *ext, bool ext_on_device, int solver, int matv,
                         bool nomm);
 void invalidate_geometry(System &s);
+void invalidate_precond(System &s);  // the near-field preconditioner is rebuilt before the next mixed solve
 const double *device_ipd(System &s);
 void refresh_host_m2d(System &s);
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat);
+int inner_precond_apply_host(System &s, int ncols, const double *r, double *z);
 void profile_enable(System &s, int every);  // 0 off, n > 0: every n-th launch per class
 void profile_stats(System &s, double *out, bool reset);
 void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *ipd_out, bool out_dev,

@@ -14,7 +14,6 @@
 
 namespace ommp {
 
-
 constexpr double kEps2 = kEpsRpDev * kEpsRpDev;
 
 // Multipole sources: ms[i] = (x, y, z, .), mq[i][LD] = q, mu, quadrupole / 3

@@ -77,6 +77,11 @@ module ommp_interface
             type(c_ptr), value :: s
             real(c_double) :: e
         end function
+        function c_ommp_get_full_ele_energy(s) bind(c, name='ommp_get_full_ele_energy') result(e)
+            import :: c_ptr, c_double
+            type(c_ptr), value :: s
+            real(c_double) :: e
+        end function
         function c_ommp_get_pol_atoms(s) bind(c, name='ommp_get_pol_atoms') result(n)
             import :: c_ptr, c_int32_t
             type(c_ptr), value :: s
@@ -168,7 +173,7 @@ module ommp_interface
     public :: ommp_init_mmp, ommp_terminate, ommp_set_verbose
     public :: ommp_set_default_solver, ommp_set_default_matv
     public :: ommp_set_external_field, ommp_set_external_field_nomm
-    public :: ommp_get_polelec_energy, ommp_get_fixedelec_energy
+    public :: ommp_get_polelec_energy, ommp_get_fixedelec_energy, ommp_get_full_ele_energy
     public :: ommp_get_pol_atoms, ommp_get_mm_atoms, ommp_get_n_ipd
     public :: ommp_ipd, ommp_polar_mm, ommp_update_coordinates
 
@@ -242,6 +247,13 @@ contains
         type(ommp_system), intent(in) :: s
         real(ommp_real) :: e
         e = c_ommp_get_fixedelec_energy(s%h)
+    end function
+
+    ! mod_interface.F90:388-398
+    function ommp_get_full_ele_energy(s) result(e)
+        type(ommp_system), intent(in) :: s
+        real(ommp_real) :: e
+        e = c_ommp_get_full_ele_energy(s%h)
     end function
 
     function ommp_get_pol_atoms(s) result(n)

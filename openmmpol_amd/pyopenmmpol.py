@@ -163,6 +163,10 @@ class OMMPSystem:
         """Interaction energy of the static multipoles (pyommp_interface.cpp:892-894)."""
         return lib().ommp_get_fixedelec_energy(self.handler)
 
+    def get_full_ele_energy(self):
+        """Fixed multipoles plus polarization (mod_interface.F90:388-398)."""
+        return lib().ommp_get_full_ele_energy(self.handler)
+
     # potentials and fields at external points (pyommp_interface.cpp:362-480):
     # cext [n, 3] in bohr; new zeroed arrays are returned
     def _x2e(self, fn, cext, field):
@@ -271,6 +275,19 @@ class OMMPSystem:
 
     def set_solver_tolerance(self, tol, max_iter=200):
         lib().ommp_amd_set_solver_tolerance(self.handler, float(tol), int(max_iter))
+
+    def set_inner_preconditioner(self, kind):
+        """Inner FP32 solves of the mixed solver: "alpha" or "near" (default)."""
+        lib().ommp_amd_set_inner_preconditioner(self.handler, {"alpha": 0, "near": 1}[kind])
+
+    def inner_precond_apply(self, r):
+        """z = M^-1 r of the near-field preconditioner (r: [ncols, pol_atoms, 3]);
+        returns (z, in_use)."""
+        x = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, self.pol_atoms, 3)
+        z = np.zeros_like(x)
+        on = lib().ommp_amd_inner_precond_apply(self.handler, x.shape[0], x.ctypes.data_as(c_dp),
+                                                z.ctypes.data_as(c_dp))
+        return z, bool(on)
 
     def set_precision(self, mode):
         lib().ommp_amd_set_precision(self.handler, {"fp64": PREC_FP64, "mixed": PREC_MIXED}[mode])
