@@ -137,6 +137,21 @@ void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset);
    out[1] solver, out[2] energy/copy-back.  out must hold 3 entries. */
 void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory -- code under HIP graph capture, tight serving loops, harnesses that
+   forbid allocation: ommp_field_mmpol2ext (reference include/openmmpol.h:46,
+   mod_interface.F90:297-311) with no allocation of any kind.  The points
+   cext[n][3], the per-split partial sums and the result are staged in a static
+   device array of 34.5 MB that each GPU the library runs on holds until the
+   process ends; the field of the multipoles and of the induced dipoles at the
+   n points is ADDED to e[n][3], with the same kernels, split counts and bits
+   as ommp_field_mmpol2ext.  Limits: n <= 8192 points per call (larger n is a
+   fatal error), and the system's device state must exist already (after a
+   solve, or any other call that ran on the GPU; fatal otherwise).  Calls that
+   overlap on one GPU (several host threads or streams) share that GPU's array
+   and race; each call returns after its own work has finished. */
+void ommp_amd_field_mmpol2ext_noalloc(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e);
+
 /* ---------------------------------------------------------------- batched */
 /* Several external fields of one system solved together: nrhs right-hand sides
    E_M2D + ext[k], each with its D and P sets (AMOEBA) or its single set, by the

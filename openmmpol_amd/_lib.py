@@ -78,6 +78,7 @@ SIGNATURES = [
     ("ommp_potential_mm2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_potential_pol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_field_mmpol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
+    ("ommp_amd_field_mmpol2ext_noalloc", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_field_mm2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_field_pol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_polelec_geomgrad", None, [c_vp, c_dp]),

@@ -281,6 +281,10 @@ void ommp_potential_pol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, do
 void ommp_field_mmpol2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e) {
     ommp::x2e_host(*sys(s), n, cext, true, true, nullptr, e);
 }
+
+void ommp_amd_field_mmpol2ext_noalloc(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e) {
+    ommp::x2e_field_noalloc_host(*sys(s), n, cext, e);
+}
 void ommp_field_mm2ext(OMMP_SYSTEM_PRT s, int32_t n, const double *cext, double *e) {
     ommp::x2e_host(*sys(s), n, cext, true, false, nullptr, e);
 }

@@ -124,6 +124,15 @@ void launch_x2e(hipStream_t st, int n, const double *cext, bool mm, bool pol, bo
                 const double4 *ms, const double *mq, int npol, const double4 *ps, const double *ipd, double4 *part,
                 int smm, int spol, double4 *out, int *bad);
 
+// The same for the field only, with no allocation: the points cext[n][3] and
+// the field e[n][3] (host, e += result) are staged in a static per-device
+// scratch array (n <= x2e_noalloc_max_points()).  Returns the coincident-point flag.
+int x2e_noalloc_max_points();
+size_t x2e_noalloc_scratch_bytes();
+int x2e_field_noalloc(hipStream_t st, int device, int n, const double *cext, bool mm, bool pol, bool amoeba, int ld,
+                      int nmm, const double4 *ms, const double *mq, int npol, const double4 *ps, const double *ipd,
+                      double *e);
+
 // Geometric gradient of the polarization energy.  part_pol holds
 // (split_s + split_p) * 3 * npol doubles, part_mm split_m * 12 * nmm (AMOEBA) or
 // split_m * 3 * nmm; mp: correction list per MM target (polarizable sources,

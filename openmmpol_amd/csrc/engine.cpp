@@ -1345,6 +1345,28 @@ void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V
     }
 }
 
+// ommp_field_mmpol2ext without any device allocation: the system's device
+// state must already exist (a solve or another device call made it); the
+// points, the partial sums and the result live in a static per-device scratch
+// array (x2e_kernels.hip), so at most x2e_noalloc_max_points() points per call.
+void x2e_field_noalloc_host(System &s, int n, const double *cext, double *E) {
+    if (n <= 0) return;
+    if (n > x2e_noalloc_max_points())
+        fatal("ommp_amd_field_mmpol2ext_noalloc takes at most " + std::to_string(x2e_noalloc_max_points()) +
+              " external points per call (" + std::to_string(n) + " requested)");
+    const bool pol = s.npol > 0;
+    if (pol && !s.ipd_done) fatal("IPD should be computed before computing D2E field.");
+    if (!s.dev)
+        fatal("ommp_amd_field_mmpol2ext_noalloc needs the system's device state: call it after a solve (or "
+              "another device call) on this system");
+    DeviceState &d = ensure_device(s);  // exists: at most re-uploads coordinates, no allocation
+    const int bad = x2e_field_noalloc(d.stream, d.device, n, cext, true, pol, s.amoeba, s.ld_cart, s.nmm, d.ms64,
+                                      d.mq, s.npol, d.ps64, d.ipd, E);
+    if (bad)
+        fatal("Requesting Coulomb kernel for two atoms placed in the same point, this could be an internal bug "
+              "or a problem in your input file, please check.");
+}
+
 namespace {
 
 // Index lists and buffers of the gradient, built on first use.  The MM-target

@@ -150,6 +150,7 @@ void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *i
                  double *epol);
 void charges_field_host(System &s, int nq, const double *charges, double *out);
 void x2e_host(System &s, int n, const double *cext, bool mm, bool pol, double *V, double *E);
+void x2e_field_noalloc_host(System &s, int n, const double *cext, double *E);
 void fmm_settings_changed(System &s);  // drops the tree, the static field and the dipoles
 void polelec_geomgrad_host(System &s, double *grd);
 void rotation_geomgrad_host(System &s, const double *E, const double *Egrd, double *grd);

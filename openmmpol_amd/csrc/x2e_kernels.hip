@@ -9,6 +9,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 
 #include "device.h"
 
@@ -186,6 +188,103 @@ void launch_x2e(hipStream_t st, int n, const double *cext, bool mm, bool pol, bo
     }
     hipLaunchKernelGGL(k_x2e_finalize, dim3(pb), dim3(kBlock), 0, st, n, off, part, out);
     HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------ no-allocation variant
+// ommp_amd_field_mmpol2ext_noalloc keeps everything beyond its inputs and
+// outputs in one file-scope device array per GPU, `scratch`: the external
+// points and the caller's field (copied in), the per-split partial sums, the
+// summed result and the coincident-point flag.  Split counts are those of the
+// allocating entry (x2e_splits), so both give the same bits.  For n points
+// each source kind takes at most ceil(2048 / ceil(n / 256)) splits, so the
+// partial sums need at most 2 (2048 + ceil(n / 256) - 1) 256 double4.
+// Every call on a GPU uses that GPU's array: calls that overlap on one GPU
+// (other host threads, other streams) race; each call waits for its own work.
+constexpr int kX2eNoAllocMaxPoints = 8192;
+constexpr long kX2eNoAllocMaxParts = 2L * (2048 + (kX2eNoAllocMaxPoints + kBlock - 1) / kBlock - 1) * kBlock;
+constexpr size_t kScrPart = 0;
+constexpr size_t kScrOut = kScrPart + sizeof(double4) * kX2eNoAllocMaxParts;
+constexpr size_t kScrC = kScrOut + sizeof(double4) * kX2eNoAllocMaxPoints;
+constexpr size_t kScrE = kScrC + sizeof(double) * 3 * kX2eNoAllocMaxPoints;
+constexpr size_t kScrBad = kScrE + sizeof(double) * 3 * kX2eNoAllocMaxPoints;
+constexpr size_t kScrBytes = kScrBad + 256;
+static_assert(kScrOut % 256 == 0 && kScrC % 256 == 0 && kScrE % 256 == 0 && kScrBad % 256 == 0,
+              "scratch sub-buffers stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+int x2e_noalloc_max_points() { return kX2eNoAllocMaxPoints; }
+size_t x2e_noalloc_scratch_bytes() { return kScrBytes; }
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (each GPU holds its own copy),
+// looked up once per device with that device current.
+unsigned char *scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) fatal("Device index out of range for the no-allocation entry point");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        HIP_CHECK(hipGetDevice(&cur));
+        if (cur != device) HIP_CHECK(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) HIP_CHECK(hipSetDevice(cur));
+        HIP_CHECK(e);
+        if (!p) fatal("No device address for the scratch array of the no-allocation entry point");
+        g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+__global__ void k_zero_flag(int *__restrict__ flag) {
+    if (threadIdx.x == 0) *flag = 0;
+}
+
+// e[j] += field part of out[j] (out = V, Ex, Ey, Ez)
+__global__ __launch_bounds__(kBlock) void k_add_x2e_field(int n, const double4 *__restrict__ out,
+                                                          double *__restrict__ e) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const double4 o = out[j];
+    e[3L * j] += o.y;
+    e[3L * j + 1] += o.z;
+    e[3L * j + 2] += o.w;
+}
+
+}  // namespace
+
+int x2e_field_noalloc(hipStream_t st, int device, int n, const double *cext, bool mm, bool pol, bool amoeba, int ld,
+                      int nmm, const double4 *ms, const double *mq, int npol, const double4 *ps, const double *ipd,
+                      double *e) {
+    if (n <= 0) return 0;
+    if (n > kX2eNoAllocMaxPoints)
+        fatal("ommp_amd_field_mmpol2ext_noalloc takes at most " + std::to_string(kX2eNoAllocMaxPoints) +
+              " external points per call (" + std::to_string(n) + " requested)");
+    const int smm = mm ? x2e_splits(n, nmm) : 0, spol = pol ? x2e_splits(n, npol) : 0;
+    if ((long)(smm + spol) * n > kX2eNoAllocMaxParts) fatal("Scratch of the no-allocation entry point too small");
+    unsigned char *base = scratch_on(device);
+    double4 *part = reinterpret_cast<double4 *>(base + kScrPart);
+    double4 *out = reinterpret_cast<double4 *>(base + kScrOut);
+    double *c = reinterpret_cast<double *>(base + kScrC);
+    double *es = reinterpret_cast<double *>(base + kScrE);
+    int *bad = reinterpret_cast<int *>(base + kScrBad);
+    HIP_CHECK(hipMemcpyAsync(c, cext, sizeof(double) * 3L * n, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(es, e, sizeof(double) * 3L * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(64), 0, st, bad);
+    HIP_CHECK(hipGetLastError());
+    launch_x2e(st, n, c, mm, pol, amoeba, ld, nmm, ms, mq, npol, ps, ipd, part, smm, spol, out, bad);
+    hipLaunchKernelGGL(k_add_x2e_field, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, out, es);
+    HIP_CHECK(hipGetLastError());
+    int bad_h = 0;
+    HIP_CHECK(hipMemcpyAsync(e, es, sizeof(double) * 3L * n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipMemcpyAsync(&bad_h, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return bad_h;
 }
 
 }  // namespace ommp
