@@ -172,25 +172,24 @@ void launch_cg_init2(hipStream_t st, int M, const double *red, int nred, CgState
                      const CgState *pre);
 
 // Near-field preconditioner of the inner FP32 PCG (precond.hip): neighbour
-// lists within 3 Angstrom by target, B_ij = alpha_i alpha_j s_ij (k3 d d^T - k2 I)
-// as 6 floats (xx, xy, xz, yy, yz, zz) per entry; off_flag = 1 switches it off
-// (list over capacity, or a breakdown seen by the direction kernel).
+// lists within kNbCut by target (row_ptr, ci; rowi = target of each entry),
+// B_ij = alpha_i alpha_j s_ij (k3 d d^T - k2 I) as 6 floats (xx, xy, xz, yy, yz,
+// zz) per entry; off_flag = 1 switches it off (set by the direction kernel on a
+// breakdown, cleared when B is recomputed).
+constexpr double kNbCut = 3.0 * 1.8897261245650;  // 3 Angstrom in bohr
 struct NbDev {
-    int *row_ptr = nullptr, *ci = nullptr;
+    int *row_ptr = nullptr, *ci = nullptr, *rowi = nullptr;
     float *b = nullptr;
     int *off_flag = nullptr;
-    int *cnt = nullptr, *off = nullptr;  // per (split, target) counts and write offsets
-    long cap = 0;
-    int nsplit = 1;
-    bool valid = false;
+    long nnz = 0, cap = 0;
+    bool list_valid = false;  // lists of the current coordinates uploaded
+    bool valid = false;       // B computed
 };
-constexpr int kNbCapPerSite = 64;  // capacity: npol * kNbCapPerSite entries
-int nb_split(int npol);
-void launch_nb_build(hipStream_t st, bool amoeba, int npol, const float4 *ps32, const float *pa32,
-                     const double *alpha, const CsrDev &pp, int nsplit, int *cnt, int *off, int *row_ptr, long cap,
-                     int *nb_ci, float *nb_b, int *nb_off);
+int nb_blocks(int npol);  // per-column partial sums of the preconditioned CG kernels
+void launch_nb_tensors(hipStream_t st, bool amoeba, const NbDev &nb, const float4 *ps32, const float *pa32,
+                       const double *alpha, const CsrDev &pp);
 // Preconditioned CG pieces (columns m < M, vectors [m * 3 npol + k]); the
-// per-block partials have site_blocks(npol) entries per column.
+// per-block partials have nb_blocks(npol) entries per column.
 void launch_cg_init_nb(hipStream_t st, int npol, int M, const double *b, double *x, double *r, double *z, double *p,
                        const double *alpha, const NbDev &nb, double *red, CgState *cs, const double *tol,
                        const CgState *pre);

@@ -339,6 +339,7 @@ void upload_geometry(System &s, DeviceState &d) {
     upload(s.q.data(), d.mq, sizeof(double) * s.ld_cart * s.nmm);
     d.tpack_valid = false;
     d.lu_valid = false;
+    d.nb.list_valid = false;
     d.nb.valid = false;
     s.dev_geometry_dirty = false;
 }
@@ -578,26 +579,114 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
     }
 }
 
-// Neighbour lists and tensors of the near-field preconditioner for the current
-// geometry (built on the device; a few microseconds at 15,000 sites).
-void ensure_nb(System &s, DeviceState &d) {
-    if (d.nb.valid) return;
-    NbDev &nb = d.nb;
-    if (!nb.row_ptr) {
-        nb.nsplit = nb_split(s.npol);
-        nb.cap = (long)s.npol * kNbCapPerSite;
-        nb.cnt = dalloc<int>(d, (size_t)nb.nsplit * s.npol);
-        nb.off = dalloc<int>(d, (size_t)nb.nsplit * s.npol);
-        nb.row_ptr = dalloc<int>(d, s.npol + 1);
-        nb.ci = dalloc<int>(d, nb.cap);
-        nb.b = dalloc<float>(d, 6 * (size_t)nb.cap);
-        nb.off_flag = dalloc<int>(d, 1);
-        d.vr3 = dalloc<double>(d, 2 * d.n);
-        d.red_d = dalloc<double>(d, 2 * (size_t)site_blocks(s.npol) + 8);
+// Neighbour lists of the near-field preconditioner: for every polarizable site
+// the sites closer than kNbCut, in site order.  A function of the coordinates
+// alone, built on the host with a grid of cells at least kNbCut wide (only the
+// 27 cells around a site can hold its neighbours) when the coordinates reach
+// the device; distances in FP64, the same for (i, j) and (j, i), so the lists
+// are symmetric.
+void build_nb_lists(const System &s, std::vector<int> &row_ptr, std::vector<int> &ci) {
+    const int n = s.npol;
+    row_ptr.assign(n + 1, 0);
+    ci.clear();
+    if (n == 0) return;
+    double lo[3], hi[3];
+    for (int c = 0; c < 3; c++) lo[c] = hi[c] = s.cpol[c];
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], s.cpol[3 * i + c]);
+            hi[c] = std::max(hi[c], s.cpol[3 * i + c]);
+        }
+    // cells at least kNbCut wide, at most ~8 per site
+    double h = kNbCut;
+    long dim[3];
+    for (;;) {
+        for (int c = 0; c < 3; c++) dim[c] = (long)std::floor((hi[c] - lo[c]) / h) + 1;
+        if (dim[0] * dim[1] * dim[2] <= 8L * n + 64) break;
+        h *= 1.5;
     }
-    launch_nb_build(d.stream, s.amoeba, s.npol, d.ps32, d.pa32, d.alpha, d.pp, nb.nsplit, nb.cnt, nb.off,
-                    nb.row_ptr, nb.cap, nb.ci, nb.b, nb.off_flag);
-    nb.valid = true;
+    auto cell_of = [&](int i, long *g) {
+        for (int c = 0; c < 3; c++)
+            g[c] = std::min(dim[c] - 1, (long)std::floor((s.cpol[3 * i + c] - lo[c]) / h));
+    };
+    const long ncell = dim[0] * dim[1] * dim[2];
+    std::vector<int> start(ncell + 1, 0), order(n), cell(n);
+    for (int i = 0; i < n; i++) {
+        long g[3];
+        cell_of(i, g);
+        cell[i] = (int)((g[2] * dim[1] + g[1]) * dim[0] + g[0]);
+        start[cell[i] + 1]++;
+    }
+    for (long k = 0; k < ncell; k++) start[k + 1] += start[k];
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; i++) order[fill[cell[i]]++] = i;
+    }
+    const double rc2 = kNbCut * kNbCut;
+    std::vector<int> row;
+    ci.reserve((size_t)n * 12);
+    for (int i = 0; i < n; i++) {
+        long g[3];
+        cell_of(i, g);
+        row.clear();
+        for (long z = std::max(0L, g[2] - 1); z <= std::min(dim[2] - 1, g[2] + 1); z++)
+            for (long y = std::max(0L, g[1] - 1); y <= std::min(dim[1] - 1, g[1] + 1); y++)
+                for (long x = std::max(0L, g[0] - 1); x <= std::min(dim[0] - 1, g[0] + 1); x++) {
+                    const long k = (z * dim[1] + y) * dim[0] + x;
+                    for (int q = start[k]; q < start[k + 1]; q++) {
+                        const int j = order[q];
+                        if (j == i) continue;
+                        const double dx = s.cpol[3 * i] - s.cpol[3 * j], dy = s.cpol[3 * i + 1] - s.cpol[3 * j + 1],
+                                     dz = s.cpol[3 * i + 2] - s.cpol[3 * j + 2];
+                        if (dx * dx + dy * dy + dz * dz < rc2) row.push_back(j);
+                    }
+                }
+        std::sort(row.begin(), row.end());
+        ci.insert(ci.end(), row.begin(), row.end());
+        row_ptr[i + 1] = (int)ci.size();
+    }
+}
+
+// The preconditioner of the current geometry on the device: lists uploaded
+// once per coordinates, tensors B recomputed before the first mixed solve of a
+// geometry (and after a reset of the static field), which also clears a
+// breakdown switch-off.
+void ensure_nb(System &s, DeviceState &d) {
+    NbDev &nb = d.nb;
+    if (!nb.list_valid) {
+        std::vector<int> rp, ci;
+        build_nb_lists(s, rp, ci);
+        const long nnz = (long)ci.size();
+        if (!nb.row_ptr) {
+            nb.row_ptr = dalloc<int>(d, s.npol + 1);
+            nb.off_flag = dalloc<int>(d, 1);
+            d.vr3 = dalloc<double>(d, 2 * d.n);
+            d.red_d = dalloc<double>(d, 2 * (size_t)nb_blocks(s.npol) + 8);
+        }
+        if (nnz > nb.cap) {
+            dfree(d, nb.ci);
+            dfree(d, nb.rowi);
+            dfree(d, nb.b);
+            nb.cap = std::max(1L, nnz + nnz / 4);
+            nb.ci = dalloc<int>(d, nb.cap);
+            nb.rowi = dalloc<int>(d, nb.cap);
+            nb.b = dalloc<float>(d, 6 * (size_t)nb.cap);
+        }
+        std::vector<int> rowi(nnz);
+        for (int i = 0; i < s.npol; i++)
+            for (int k = rp[i]; k < rp[i + 1]; k++) rowi[k] = i;
+        upload(rp.data(), nb.row_ptr, sizeof(int) * (s.npol + 1));
+        upload(ci.data(), nb.ci, sizeof(int) * nnz);
+        upload(rowi.data(), nb.rowi, sizeof(int) * nnz);
+        nb.nnz = nnz;
+        nb.list_valid = true;
+        nb.valid = false;
+    }
+    if (!nb.valid) {
+        HCHECK(hipMemsetAsync(nb.off_flag, 0, sizeof(int), d.stream));
+        launch_nb_tensors(d.stream, s.amoeba, nb, d.ps32, d.pa32, d.alpha, d.pp);
+        nb.valid = true;
+    }
 }
 
 // conjugate_gradient_solver (mod_solvers.F90:86-216) on M columns that share
