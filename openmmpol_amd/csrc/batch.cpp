@@ -54,7 +54,8 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
     L.ered = align256(sizeof(double) * nrhs * L.vblocks);
     L.eout = align256(sizeof(double) * nrhs);
     L.charges = align256(sizeof(double4) * kMaxCharges);
-    L.total = 7 * L.vec + L.pf + L.part + L.red_hp + L.red_rz + L.ext + L.ered + L.eout + L.charges;
+    // 8 vectors (b, x, r, z, p, h, d, r2), two kinds of r partials
+    L.total = 8 * L.vec + L.pf + L.part + L.red_hp + 2 * L.red_rz + L.ext + L.ered + L.eout + L.charges;
     return L;
 }
 
@@ -185,7 +186,10 @@ int BatchEngine::add(const DeviceView &v, int nrhs, int extra) {
     d.pp_ri = v.pp.ri;
     d.pp_ci = v.pp.ci;
     d.pp_f = v.pp.f;
-    double **vecs[] = {&d.b, &d.x, &d.r, &d.z, &d.p, &d.h, &d.d};
+    d.nb_row_ptr = v.nb_row_ptr;
+    d.nb_ci = v.nb_ci;
+    d.nb_b = v.nb_b;
+    double **vecs[] = {&d.b, &d.x, &d.r, &d.z, &d.p, &d.h, &d.d, &d.r2};
     size_t off = 0;
     for (auto pv : vecs) {
         *pv = reinterpret_cast<double *>(mem + off);
@@ -198,6 +202,8 @@ int BatchEngine::add(const DeviceView &v, int nrhs, int extra) {
     d.red_hp = reinterpret_cast<double *>(mem + off);
     off += L.red_hp;
     d.red_rz = reinterpret_cast<double *>(mem + off);
+    off += L.red_rz;
+    d.red_ra = reinterpret_cast<double *>(mem + off);
     off += L.red_rz;
     S.ext = reinterpret_cast<double *>(mem + off);
     off += L.ext;

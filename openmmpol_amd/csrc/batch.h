@@ -54,7 +54,7 @@ struct BCol {
     int maxit;       // inner iterations per refinement step
     int rinit;       // first residual: kResMatvec, kResXZero or kResKnown
     int oneshot;     // stop after the first inner solve, x = its correction (no FP64 check)
-    int pad;
+    int nb_off;      // 1: this column's inner solves use alpha alone (set on a breakdown)
 };
 
 // One system in a batch; device pointers only.
@@ -80,7 +80,14 @@ struct BSeg {
     void *part;
     int sym32, sym64;
     double *red_hp;                      // [ncols][tblocks]
-    double *red_rz;                      // [ncols][vblocks]
+    double *red_rz;                      // [ncols][vblocks]: r.z
+    double *red_ra;                      // [ncols][vblocks]: r.(alpha r), the convergence measure
+    // near-field preconditioner of the inner solves (the system's lists and
+    // tensors, precond.hip); nb_row_ptr == nullptr: z = alpha r
+    const int *nb_row_ptr;
+    const int *nb_ci;
+    const float *nb_b;
+    double *r2;                          // [ncols][3 npol]: the inner residual alternates r, r2
 };
 
 // Work lists (built on the host when the batch composition changes):
@@ -168,6 +175,10 @@ struct DeviceView {
     const double *e_m2d;
     double *ipd;
     size_t bytes;    // device memory held by the system
+    // near-field preconditioner (nullptr: the system's inner solves use alpha)
+    const int *nb_row_ptr = nullptr;
+    const int *nb_ci = nullptr;
+    const float *nb_b = nullptr;
 };
 
 }  // namespace ommp

@@ -510,9 +510,32 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const BSeg *__restrict__ segs
 // ------------------------------------------------------------ vector kernels
 // Work item {segment, column, vector block, batch column}; element k of the
 // column belongs to site k / 3.
+//
+// With the near-field preconditioner (segment lists present, the column not
+// switched off): z = M^-1 r, M^-1 = 2 alpha - alpha T_near alpha (precond.hip),
+// element k summing row k % 3 of B_ij r_j over the neighbours j of its site,
+// the neighbours' r formed by the same expression as their own elements form
+// it.  Convergence is measured on r.(alpha r) as the reference does
+// (mod_solvers.F90:188-199), beta on r.z.  Without it z = alpha r and the two
+// partial sums coincide.
+constexpr double kBatchNbDiag = 2.0;
 
-// Outer step, part 1: r = b - T x (x = 0: r = b; known: r kept), z = alpha r,
-// r.z partials.
+// sum_j (B_ij r_j)_c over the neighbours of site i; r_j = rj(oj) with oj the
+// offset of site j's x component in this column
+template <typename RF>
+__device__ __forceinline__ double nb_row(const BSeg &S, int i, int c, long colbase, RF rj) {
+    double acc = 0.0;
+    const int e0 = c == 0 ? 0 : (c == 1 ? 1 : 2), e1 = c == 0 ? 1 : (c == 1 ? 3 : 4), e2 = c == 0 ? 2 : (c == 1 ? 4 : 5);
+    for (int q = S.nb_row_ptr[i]; q < S.nb_row_ptr[i + 1]; q++) {
+        const long oj = colbase + 3L * S.nb_ci[q];
+        const float *b = S.nb_b + 6L * q;
+        acc += (double)b[e0] * rj(oj) + (double)b[e1] * rj(oj + 1) + (double)b[e2] * rj(oj + 2);
+    }
+    return acc;
+}
+
+// Outer step, part 1: r = b - T x (x = 0: r = b; known: r kept), z = M^-1 r,
+// partials of r.z and r.(alpha r).
 __global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                    const int4 *__restrict__ work) {
     __shared__ double sh[4];
@@ -522,17 +545,29 @@ __global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs
     const BSeg &S = segs[w.x];
     const long n = 3L * S.npol;
     const long k = (long)w.z * kBlock + threadIdx.x;
-    const long o = (long)w.y * n + k;
-    double v = 0.0;
+    const long colbase = (long)w.y * n, o = colbase + k;
+    const int rinit = C.rinit;
+    const bool nb = S.nb_row_ptr != nullptr && !C.nb_off;
+    auto rule = [&](long q) {
+        return rinit == kResXZero ? S.b[q] : (rinit == kResKnown ? S.r[q] : S.b[q] - S.h[q]);
+    };
+    double vz = 0.0, va = 0.0;
     if (k < n) {
-        const double rr = C.rinit == kResXZero ? S.b[o] : (C.rinit == kResKnown ? S.r[o] : S.b[o] - S.h[o]);
-        const double zz = S.alpha[k / 3] * rr;
-        S.r[o] = rr;
+        const double rr = rule(o);
+        const int i = (int)(k / 3), c = (int)(k - 3L * i);
+        const double al = S.alpha[i];
+        double zz = al * rr;
+        if (nb) zz = kBatchNbDiag * zz + nb_row(S, i, c, colbase, rule);
+        // a known residual is already in r (and its neighbours read it)
+        if (rinit != kResKnown) S.r[o] = rr;
         S.z[o] = zz;
-        v = rr * zz;
+        vz = rr * zz;
+        va = rr * (al * rr);
     }
-    v = block_sum_all(v, sh);
-    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = v;
+    vz = block_sum_all(vz, sh);
+    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = vz;
+    va = block_sum_all(va, sh);
+    if (threadIdx.x == 0) S.red_ra[(long)w.y * S.vblocks + w.z] = va;
 }
 
 // Outer step, part 2 (state copy 0 -> 1): rms test; if not converged, start an
@@ -547,8 +582,9 @@ __global__ __launch_bounds__(kBlock) void k_bdecide(const BSeg *__restrict__ seg
     if (st.phase == kPhaseOuter) {
         const BSeg &S = segs[w.x];
         const long n = 3L * S.npol;
+        const double ra = reduce_parts(S.red_ra + (long)w.y * S.vblocks, S.vblocks, sh);
         const double rz = reduce_parts(S.red_rz + (long)w.y * S.vblocks, S.vblocks, sh);
-        const double rms = sqrt(rz / (double)n);
+        const double rms = sqrt(ra / (double)n);
         nst.rms = rms;
         if (rms < C.tol) {
             nst.phase = kPhaseDone;
@@ -573,7 +609,8 @@ __global__ __launch_bounds__(kBlock) void k_bdecide(const BSeg *__restrict__ seg
     if (w.z == 0 && threadIdx.x == 0) cols[w.w].s[1] = nst;
 }
 
-// Inner PCG, update: alpha = g / p.Tp; d += alpha p; r -= alpha h; z = alpha_i r.
+// Inner PCG, update: alpha = g / p.Tp; d += alpha p; r_new = r_old - alpha h
+// (r and r2 alternate with the inner iteration); z = M^-1 r_new.
 __global__ __launch_bounds__(kBlock) void k_bupdate(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                     const int4 *__restrict__ work) {
     __shared__ double sh[4];
@@ -585,23 +622,35 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(const BSeg *__restrict__ seg
     const double gama = reduce_parts(S.red_hp + (long)w.y * S.tblocks, S.tblocks, sh);
     if (gama == 0.0) return;  // the direction kernel ends this inner solve
     const double a = C.s[1].g / gama;
+    const bool odd = C.s[1].it & 1;
+    const double *rold = odd ? S.r2 : S.r;
+    double *rnew = odd ? S.r : S.r2;
+    const bool nb = S.nb_row_ptr != nullptr && !C.nb_off;
     const long k = (long)w.z * kBlock + threadIdx.x;
-    const long o = (long)w.y * n + k;
-    double v = 0.0;
+    const long colbase = (long)w.y * n, o = colbase + k;
+    double vz = 0.0, va = 0.0;
     if (k < n) {
         S.d[o] += a * S.p[o];
-        const double rr = S.r[o] - a * S.h[o];
-        const double zz = S.alpha[k / 3] * rr;
-        S.r[o] = rr;
+        const double rr = fma(-a, S.h[o], rold[o]);
+        const int i = (int)(k / 3), c = (int)(k - 3L * i);
+        const double al = S.alpha[i];
+        double zz = al * rr;
+        if (nb) zz = kBatchNbDiag * zz + nb_row(S, i, c, colbase, [&](long q) { return fma(-a, S.h[q], rold[q]); });
+        rnew[o] = rr;
         S.z[o] = zz;
-        v = rr * zz;
+        vz = rr * zz;
+        va = rr * (al * rr);
     }
-    v = block_sum_all(v, sh);
-    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = v;
+    vz = block_sum_all(vz, sh);
+    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = vz;
+    va = block_sum_all(va, sh);
+    if (threadIdx.x == 0) S.red_ra[(long)w.y * S.vblocks + w.z] = va;
 }
 
 // Inner PCG, direction (state copy 1 -> 0): inner convergence ends the
-// refinement step with x += d; otherwise p = z + beta p.
+// refinement step with x += d; otherwise p = z + beta p.  r.z <= 0 (the
+// preconditioner is not positive definite here) also ends it, and the
+// column's later inner solves use alpha alone.
 __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ segs, BCol *__restrict__ cols,
                                                        const int4 *__restrict__ work) {
     __shared__ double sh[4];
@@ -609,14 +658,16 @@ __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ 
     const BCol &C = cols[w.w];
     const BColSub st = C.s[1];
     BColSub nst = st;
-    bool ended = false;
+    bool ended = false, breakdown = false;
     if (st.phase == kPhaseInner) {
         const BSeg &S = segs[w.x];
         const long n = 3L * S.npol;
         const double gama = reduce_parts(S.red_hp + (long)w.y * S.tblocks, S.tblocks, sh);
         const double gnew = gama == 0.0 ? st.g : reduce_parts(S.red_rz + (long)w.y * S.vblocks, S.vblocks, sh);
-        const double rms_in = sqrt(gnew / (double)n);
-        ended = gama == 0.0 || rms_in < st.tin || st.it + 1 >= C.maxit;
+        const double ra = gama == 0.0 ? st.g : reduce_parts(S.red_ra + (long)w.y * S.vblocks, S.vblocks, sh);
+        const double rms_in = sqrt(ra / (double)n);
+        breakdown = gama != 0.0 && !(gnew > 0.0) && !(rms_in < st.tin);
+        ended = gama == 0.0 || rms_in < st.tin || st.it + 1 >= C.maxit || breakdown;
         const long k = (long)w.z * kBlock + threadIdx.x;
         const long o = (long)w.y * n + k;
         if (ended) {
@@ -634,6 +685,7 @@ __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ 
     if (w.z == 0 && threadIdx.x == 0) {
         cols[w.w].s[0] = nst;
         if (ended) cols[w.w].rinit = kResMatvec;
+        if (breakdown) cols[w.w].nb_off = 1;
     }
 }
 
@@ -741,6 +793,7 @@ __global__ void k_batch_start(BCol *cols, int c0, int nc, double tol, int maxit,
     C.maxit = maxit;
     C.rinit = rinit;
     C.oneshot = oneshot;
+    C.nb_off = 0;
 }
 
 void launch_batch_start(hipStream_t st, BCol *cols, int c0, int nc, double tol, int maxit, int rinit,

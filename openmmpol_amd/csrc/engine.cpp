@@ -1285,6 +1285,13 @@ DeviceView device_view(System &s) {
     v.pp = d.pp;
     v.e_m2d = d.e_m2d;
     v.ipd = d.ipd;
+    if (s.inner_precond == 1 && s.npol > 0) {
+        ensure_nb(s, d);
+        HCHECK(hipStreamSynchronize(d.stream));
+        v.nb_row_ptr = d.nb.row_ptr;
+        v.nb_ci = d.nb.ci;
+        v.nb_b = d.nb.b;
+    }
     v.bytes = 0;
     for (size_t b : d.alloc_bytes) v.bytes += b;
     return v;
