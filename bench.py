@@ -179,9 +179,9 @@ def run_multi(tol, steps, bufs):
         ep = s.set_external_fields_multi_device(MULTI_NRHS, d_ext, d_ipd)
     t_multi = (time.perf_counter() - t) / steps
     it_multi = s.last_iterations
-    ipd0 = np.zeros((2, n, 3))
-    bufs.hip.hipMemcpy(ipd0.ctypes.data_as(bufs.ct.c_void_p), bufs.ct.c_void_p(d_ipd),
-                       bufs.ct.c_size_t(ipd0.nbytes), 2)
+    ipd_all = np.zeros((MULTI_NRHS, 2, n, 3))
+    bufs.hip.hipMemcpy(ipd_all.ctypes.data_as(bufs.ct.c_void_p), bufs.ct.c_void_p(d_ipd),
+                       bufs.ct.c_size_t(ipd_all.nbytes), 2)
 
     def sequential(mode, count):
         s.set_precision(mode)
@@ -204,19 +204,22 @@ def run_multi(tol, steps, bufs):
         with open(gpath) as f:
             g = json.load(f)
         err_e = max(abs(ep[r["field"]] - r["epol"]) / abs(r["epol"]) for r in g["rhs"])
-        npy = os.path.join(ROOT, "tests", "golden", "multi15000_cavity_ipd0.npy")
-        if os.path.exists(npy):
-            err_mu = float(np.abs(ipd0 - np.load(npy)).max())
+        # dipoles at the golden's sample sites of every golden field (field 0 is
+        # its own guess; the others are solved by the batched kernels)
+        err_mu = max(float(np.abs(ipd_all[r["field"]][:, g["sample"], :] - np.array(r["ipd_sample"])).max())
+                     for r in g["rhs"])
     return {
         "workload": f"15000-site AMOEBA sphere with a 6 A cavity, {MULTI_NRHS} fields of a 30-charge QM region "
                     "displaced N(0, 0.1 A) per field; guess = dipoles of the undisplaced-field solve",
         "rhs_solves_per_s": MULTI_NRHS / t_multi, "ms_per_call": t_multi * 1e3,
-        "iterations": {"outer_fp64_max": it_multi[2], "inner_fp32_max": it_multi[3]},
+        "inner_matvec": "T tiles and p split as bf16 hi + lo, hi*hi + hi*lo + lo*hi on the matrix cores "
+                        "(v_mfma_f32_16x16x32_bf16), FP32 accumulation; FP64 refinement residuals",
+        "iterations": {"outer_fp64_max": it_multi[2], "inner_max": it_multi[3]},
         "sequential_mixed_solves_per_s": 1.0 / t_seq_mixed,
         "sequential_fp64_solves_per_s": 1.0 / t_seq_fp64,
         "speedup_vs_sequential_mixed": t_seq_mixed / t_multi * MULTI_NRHS,
         "speedup_vs_sequential_fp64": t_seq_fp64 / t_multi * MULTI_NRHS,
-        "epol_rel_err_vs_golden_max": err_e, "mu_maxabs_err_rhs0": err_mu,
+        "epol_rel_err_vs_golden_max": err_e, "mu_maxabs_err_vs_golden_samples": err_mu,
         "epol_max_rel_diff_vs_sequential_fp64": float(np.max(np.abs(ep[:16] - e_seq_fp64) / np.abs(e_seq_fp64))),
     }
 

@@ -19,7 +19,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct SegLayout {
     int nsplit, chunk, tblocks, vblocks, sym32, sym64;
-    size_t vec, pf, part, red_hp, red_rz, ext, ered, eout, charges, total;
+    size_t vec, xs, part, red_hp, red_rz, ext, ered, eout, charges, total;
 };
 
 SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int extra) {
@@ -37,8 +37,13 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
         L.nsplit = balanced_split(L.tblocks * groups, npol, num_cu);
     }
     L.chunk = (npol + L.nsplit - 1) / L.nsplit;
+    if (m32 == kBatchM32) {
+        // the matrix-core tiles read 32 sources at a time from 16-byte aligned rows
+        L.chunk = (L.chunk + kBxSrc - 1) / kBxSrc * kBxSrc;
+        L.nsplit = (npol + L.chunk - 1) / L.chunk;
+    }
     L.vec = align256(sizeof(double) * ncols * n);
-    L.pf = m32 == kBatchM32 ? align256(sizeof(float) * ncols * n) : 0;
+    L.xs = m32 == kBatchM32 ? align256(sizeof(unsigned short) * ncols * (size_t)bx_ld(npol)) : 0;
     // the pair-once sweep at the sizes where the single-system solver uses it
     // (the matrix-core FP32 matvec keeps its own tiling); many columns may take
     // up to 8 GiB of block-pair partials (15,000 sites x 128 columns: 2.8 GB,
@@ -55,7 +60,7 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
     L.eout = align256(sizeof(double) * nrhs);
     L.charges = align256(sizeof(double4) * kMaxCharges);
     // 8 vectors (b, x, r, z, p, h, d, r2), two kinds of r partials
-    L.total = 8 * L.vec + L.pf + L.part + L.red_hp + 2 * L.red_rz + L.ext + L.ered + L.eout + L.charges;
+    L.total = 8 * L.vec + 2 * L.xs + L.part + L.red_hp + 2 * L.red_rz + L.ext + L.ered + L.eout + L.charges;
     return L;
 }
 
@@ -195,8 +200,11 @@ int BatchEngine::add(const DeviceView &v, int nrhs, int extra) {
         *pv = reinterpret_cast<double *>(mem + off);
         off += L.vec;
     }
-    d.pf = L.pf ? reinterpret_cast<float *>(mem + off) : nullptr;
-    off += L.pf;
+    d.ldx = bx_ld(v.npol);
+    d.xh = L.xs ? reinterpret_cast<unsigned short *>(mem + off) : nullptr;
+    off += L.xs;
+    d.xl = L.xs ? reinterpret_cast<unsigned short *>(mem + off) : nullptr;
+    off += L.xs;
     d.part = mem + off;
     off += L.part;
     d.red_hp = reinterpret_cast<double *>(mem + off);

@@ -73,7 +73,11 @@ struct BSeg {
     const int *pp_ci;
     const double *pp_f;
     double *b, *x, *r, *z, *p, *h, *d;  // [ncols][3 npol]
-    float *pf;                           // [ncols][3 npol] FP32 copy of p (MFMA path)
+    // matrix-core path: p of the inner solves split as hi + lo, two bf16 values
+    // per element, [ncols][ldx], ldx = 3 * (npol rounded up to kBxSrc); the
+    // elements past 3 npol are zero
+    unsigned short *xh, *xl;
+    int ldx;
     // partial sums of the matvec: per-target sweep [nsplit][ncols][3 npol]; pair-once
     // sweep (sym32 / sym64 set for that precision, tmv_sym.h) [block pair][ncols][3][256]
     // target sides, then as many source sides; FP64-sized
@@ -115,10 +119,13 @@ struct BatchTables {
 };
 
 constexpr int kBatchM32 = 8;  // columns per FP32 finalize work item (multi-RHS)
-// Multi-RHS FP32 matvec on the matrix cores: work item = 32 target sites x 128
-// columns x one source split; T tiles of 32 x 16 sites built in LDS.
+// Multi-RHS inner matvec on the bf16 matrix cores: work item = 32 target sites x
+// 128 columns x one source split (a multiple of kBxSrc sites); T tiles of 32 x 32
+// sites built in LDS.
 constexpr int kMfmaSites = 32;
 constexpr int kMfmaCols = 128;
+constexpr int kBxSrc = 32;
+inline int bx_ld(int npol) { return 3 * ((npol + kBxSrc - 1) / kBxSrc) * kBxSrc; }
 constexpr int kBatchM64 = 4;  // columns per FP64 pair work item (multi-RHS)
 
 // Column-group width used for a segment of ncols columns.

@@ -185,42 +185,50 @@ __device__ __forceinline__ void bsym_body(const BSeg &S, const int4 w, const dou
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kSymBlock) void k_bsym(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                     const int4 *__restrict__ work, int use_p) {
-    constexpr int kPair = M < 2 ? M : 2;  // columns per sweep
+    // columns per sweep: FP64 sweeps take up to four (the pair's tensor, the
+    // costly part in FP64, is then built once for all of them), FP32 two
+    constexpr int kPair = sizeof(R) == 8 ? (M < 4 ? M : 4) : (M < 2 ? M : 2);
     __shared__ __align__(16) char lds[sym_lds_bytes<R, kPair>()];
     const int4 w = work[blockIdx.x];
     const BSeg &S = segs[w.x];
     const int c0 = w.w * M;
     const double *xin = use_p ? S.p : S.x;
-    int act[M];
-    int nact = 0;
+    int idx[M];
+    int k = 0;
 #pragma unroll
     for (int m = 0; m < M; m++) {
         const bool a = c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
-        act[m] = a ? c0 + m : -1;
-        nact += a;
+        idx[m] = -1;
+        if (a) idx[k++] = c0 + m;
     }
-    if (nact == 0) return;  // uniform across the workgroup
-    if (nact == 1) {
-        int one[1] = {-1};
-#pragma unroll
-        for (int m = 0; m < M; m++)
-            if (act[m] >= 0) one[0] = act[m];
+    if (k == 0) return;  // uniform across the workgroup
+    if constexpr (kPair >= 4) {
+        if (k == 4) {
+            const int four[4] = {idx[0], idx[1], idx[2], idx[3]};
+            bsym_body<R, 4, AMOEBA>(S, w, xin, four, lds);
+            return;
+        }
+        if (k == 3) {
+            const int three[3] = {idx[0], idx[1], idx[2]};
+            bsym_body<R, 3, AMOEBA>(S, w, xin, three, lds);
+            return;
+        }
+    }
+    if (k == 1) {
+        const int one[1] = {idx[0]};
         bsym_body<R, 1, AMOEBA>(S, w, xin, one, lds);
         return;
     }
-    if constexpr (M == 2) {
-        bsym_body<R, 2, AMOEBA>(S, w, xin, act, lds);
+    if constexpr (kPair >= 2 && (M == 2 || kPair >= 4)) {
+        // two active columns (the wider counts were handled above)
+        const int two[2] = {idx[0], idx[1]};
+        bsym_body<R, 2, AMOEBA>(S, w, xin, two, lds);
     } else {
-        // every active column of the group in turn, two at a time (an odd one alone)
-        int idx[M];
-        int k = 0;
-#pragma unroll
-        for (int m = 0; m < M; m++)
-            if (act[m] >= 0) idx[k++] = act[m];
+        // the active columns of the group in turn, two at a time (an odd one alone)
         int q = 0;
         for (; q + 1 < k; q += 2) {
             const int two[2] = {idx[q], idx[q + 1]};
-            bsym_body<R, 2, AMOEBA>(S, w, xin, two, lds);
+            bsym_body<R, kPair < 2 ? 1 : 2, AMOEBA>(S, w, xin, two, lds);
         }
         if (q < k) {
             const int one[1] = {idx[q]};
@@ -229,42 +237,62 @@ __global__ __launch_bounds__(kSymBlock) void k_bsym(const BSeg *__restrict__ seg
     }
 }
 
-// ------------------------------------------------- matrix-core FP32 matvec
-// Multi-RHS form of the on-the-fly matvec as a dense contraction on the matrix
-// cores: for a tile of 32 target sites (96 rows) and 128 columns, every tile of
-// 16 source sites contributes A (96 x 48, the damped dipole tensors k3 d d^T -
-// k2 I built in FP32 in LDS) times X (48 x 128, the columns' source dipoles in
-// LDS), accumulated by v_mfma_f32_16x16x4f32.  Wave w owns columns 32w..32w+31
-// (two 16-column blocks) and all six 16-row blocks.  Inside one instruction,
-// lane group g = lane / 16 supplies k = 12 g + kk, so each lane reads four
-// consecutive k with one 16-byte LDS load.  The LDS footprint (~47 KB) lets
-// three workgroups share a CU, so one workgroup's tile staging overlaps the
-// others' matrix-core work.  Partial fields go to part[split][column][3 npol]
-// as in k_bpairs; the finalize (k_bfinal) is shared.
+// ------------------------------------------------- bf16x3 matrix-core matvec
+// Multi-RHS form of the on-the-fly FP32 matvec as a dense contraction on the
+// bf16 matrix cores.  Every operand v is split as v = hi + lo (two bf16 values,
+// round to nearest: |v - hi - lo| <= 2^-16 |v|) and A X is summed as
+// A_hi X_hi + A_hi X_lo + A_lo X_hi, accumulated in FP32; the dropped A_lo X_lo
+// term is of the same 2^-16 relative size.  The inner solves of the mixed
+// solver tolerate that: the FP64 refinement steps measure the true residual.
+// Tiles: 32 target sites (96 rows, rows 3 t + a) x 32 source sites (K = 96,
+// k = 3 s + b, three steps of v_mfma_f32_16x16x32_bf16) x 128 columns.  The
+// damped dipole tensors k3 d d^T - k2 I of a tile are built in FP32 and stored
+// split in LDS; the X fragments of a lane (its column, 8 consecutive k) come
+// straight from the split copy of p in global memory, issued before the tile's
+// tensors are built so that the loads land meanwhile.  Wave w owns columns
+// 32 w .. 32 w + 31 (two 16-column blocks) and all six 16-row blocks; lane
+// group g = lane / 16 supplies k = 8 g .. 8 g + 7 of each step and holds rows
+// 4 g .. 4 g + 3 of each result block.  Partial fields go to
+// part[split][column][3 npol] as in k_bpairs; the finalize (k_bfinal) is shared.
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bx8 __attribute__((ext_vector_type(8)));
+typedef const bx8 __attribute__((address_space(1))) *gbx8;  // global-memory loads, not flat
 constexpr int kMfmaRows = 3 * kMfmaSites;  // 96
-constexpr int kMfmaSrc = 16;               // source sites per tile
-constexpr int kMfmaK = 3 * kMfmaSrc;       // 48
-constexpr int kMfmaLd = kMfmaK + 4;        // LDS row stride (floats): conflict-free 16-byte reads
+constexpr int kBxK = 3 * kBxSrc;           // 96
+constexpr int kBxLd = kBxK + 8;            // LDS row stride (bf16): conflict-free 16-byte reads
 
-constexpr int kMfmaStage = kMfmaCols * kMfmaK / kBlock;  // X-tile elements per thread (24)
+// loads through the global address space (flat loads would also hold up the
+// LDS counter that the tile's LDS accesses wait on)
+__device__ __forceinline__ float4 gload(const float4 *p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const v4 __attribute__((address_space(1))) *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float gload(const float *p) { return *(const float __attribute__((address_space(1))) *)p; }
+
+// hi and lo halves of two FP32 values, packed as bf16 pairs
+__device__ __forceinline__ void bx_split2(float v0, float v1, unsigned &hi, unsigned &lo) {
+    const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
+    const __bf16 l0 = (__bf16)(v0 - (float)h0), l1 = (__bf16)(v1 - (float)h1);
+    hi = (unsigned)__builtin_bit_cast(unsigned short, h0) | ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
+    lo = (unsigned)__builtin_bit_cast(unsigned short, l0) | ((unsigned)__builtin_bit_cast(unsigned short, l1) << 16);
+}
 
 template <bool AMOEBA>
-__global__ __launch_bounds__(kBlock) void k_bpairs_mfma(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
-                                                        const int4 *__restrict__ work) {
+__global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                       const int4 *__restrict__ work) {
     const int4 w = work[blockIdx.x];
     const BSeg &S = segs[w.x];
-    const int npol = S.npol, ncols = S.ncols;
+    const int npol = S.npol, ncols = S.ncols, ldx = S.ldx;
     const long n = 3L * npol;
     const int c0 = w.w * kMfmaCols;
     const float4 *__restrict__ ps32 = S.ps32;
     const float *__restrict__ pa32 = S.pa32;
-    const float *__restrict__ pf = S.pf + (long)c0 * n;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, l16 = lane & 15;
-    __shared__ float As[kMfmaRows][kMfmaLd];
-    __shared__ float Bs[kMfmaCols][kMfmaLd];
-    __shared__ float4 s_tp[kMfmaSites], s_sp[kMfmaSrc];
-    __shared__ float s_ta[kMfmaSites], s_sa[kMfmaSrc];
+    __shared__ __align__(16) __bf16 Ah[kMfmaRows][kBxLd];
+    __shared__ __align__(16) __bf16 Al[kMfmaRows][kBxLd];
+    __shared__ float4 s_tp[kMfmaSites], s_sp[kBxSrc];
+    __shared__ float s_ta[kMfmaSites], s_sa[kBxSrc];
     __shared__ int s_act[kMfmaCols];
     int mine = 0;
     if (tid < kMfmaCols) {
@@ -278,8 +306,8 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_mfma(const BSeg *__restrict__
         float4 p;
         float a;
         if (t < npol) {
-            p = ps32[t];
-            a = pa32[t];
+            p = gload(ps32 + t);
+            a = gload(pa32 + t);
         } else {
             p = make_float4(0.f, 0.f, 0.f, 1.f);
             a = 0.39f;
@@ -287,27 +315,27 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_mfma(const BSeg *__restrict__
         s_tp[tid] = p;
         s_ta[tid] = a;
     }
-    // Staging of one source tile (positions and the X tile) goes through
-    // registers: the loads of tile t + 1 are issued before the generation and
-    // matrix-core work of tile t and land in LDS after it.
     const int i0 = w.z * S.chunk;
     const int i1 = min(npol, i0 + S.chunk);
-    float xr[kMfmaStage];
+    // this lane's two columns and their split rows (inactive: never read)
+    bool cact[2];
+    gbx8 xrow_h[2], xrow_l[2];
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+        const int c = 32 * wv + 16 * cb + l16;
+        cact[cb] = s_act[c];
+        const long o = cact[cb] ? (long)(c0 + c) * ldx + 8 * g : 0;
+        xrow_h[cb] = (gbx8)(reinterpret_cast<const __bf16 *>(S.xh) + o);
+        xrow_l[cb] = (gbx8)(reinterpret_cast<const __bf16 *>(S.xl) + o);
+    }
+    // source positions of the next tile go through registers
     float4 spr = make_float4(1e15f, 1e15f, 1e15f, 1.f);
     float sar = 0.39f;
-    // thread t stages column t / 2, k = 24 (t % 2) .. 24 (t % 2) + 23
-    const int sc = tid >> 1, sk0 = (tid & 1) * kMfmaStage;
-    const bool sact = s_act[sc];
-    auto fetch = [&](int base) {
-        const int cnt = min(kMfmaSrc, i1 - base);
-        const int lim = 3 * cnt - sk0;
-        const float *src = pf + (long)sc * n + 3L * base + sk0;
-#pragma unroll
-        for (int j = 0; j < kMfmaStage; j++) xr[j] = (sact && j < lim) ? src[j] : 0.f;
-        if (tid < kMfmaSrc) {
-            if (tid < cnt) {
-                spr = ps32[base + tid];
-                sar = pa32[base + tid];
+    auto fetch_src = [&](int base) {
+        if (tid < kBxSrc) {
+            if (base + tid < i1) {
+                spr = gload(ps32 + base + tid);
+                sar = gload(pa32 + base + tid);
             } else {
                 spr = make_float4(1e15f, 1e15f, 1e15f, 1.f);
                 sar = 0.39f;
@@ -319,57 +347,91 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_mfma(const BSeg *__restrict__
     for (int cb = 0; cb < 2; cb++)
 #pragma unroll
         for (int rb = 0; rb < 6; rb++) acc[cb][rb] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-    if (i0 < i1) fetch(i0);
-    for (int base = i0; base < i1; base += kMfmaSrc) {
-        __syncthreads();  // previous tile's LDS reads are done
-#pragma unroll
-        for (int j = 0; j < kMfmaStage; j += 4)
-            *reinterpret_cast<float4 *>(&Bs[sc][sk0 + j]) = make_float4(xr[j], xr[j + 1], xr[j + 2], xr[j + 3]);
-        if (tid < kMfmaSrc) {
+    if (i0 < i1) {
+        fetch_src(i0);
+        if (tid < kBxSrc) {
             s_sp[tid] = spr;
             s_sa[tid] = sar;
         }
-        __syncthreads();
-        if (base + kMfmaSrc < i1) fetch(base + kMfmaSrc);
-        // A tile: damped dipole tensors of the 32 x 16 site pairs
-        for (int pidx = tid; pidx < kMfmaSites * kMfmaSrc; pidx += kBlock) {
-            const int ti = pidx / kMfmaSrc, sj = pidx - ti * kMfmaSrc;
-            float dx, dy, dz, k2, k3, arg;
-            bool near;
-            pair_geom<float, AMOEBA>(s_tp[ti], s_ta[ti], s_sp[sj], s_sa[sj], dx, dy, dz, k2, k3, arg, near);
-            pair_damp<float, AMOEBA>(arg, near, k2, k3);
-            const float d[3] = {dx, dy, dz};
+    }
+    __syncthreads();
+    const bx8 zero8 = {};
+    for (int base = i0; base < i1; base += kBxSrc) {
+        // X fragments of this tile: k = 3 base + 32 q + 8 g .. + 7 of the lane's columns
+        bx8 bh[2][3], bl[2][3];
+        const long e = (3L * base) / 8;
 #pragma unroll
-            for (int a = 0; a < 3; a++)
+        for (int cb = 0; cb < 2; cb++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) As[3 * ti + a][3 * sj + b] = k3 * d[a] * d[b] - (a == b ? k2 : 0.f);
-        }
-        __syncthreads();
-        const float *arow = &As[l16][12 * g];
-        const float *bcol = &Bs[32 * wv + l16][12 * g];
+            for (int q = 0; q < 3; q++) {
+                bh[cb][q] = cact[cb] ? xrow_h[cb][e + 4 * q] : zero8;
+                bl[cb][q] = cact[cb] ? xrow_l[cb][e + 4 * q] : zero8;
+            }
+        // A tile: two sources of one target per item, 3 rows x 6 k each
+        for (int it = tid; it < kMfmaSites * kBxSrc / 2; it += kBlock) {
+            const int ti = it >> 4, sp = it & 15;
+            float v[3][6];
 #pragma unroll
-        for (int kk = 0; kk < 12; kk += 4) {
-            const float4 b0 = *reinterpret_cast<const float4 *>(bcol + kk);
-            const float4 b1 = *reinterpret_cast<const float4 *>(bcol + 16 * kMfmaLd + kk);
+            for (int h = 0; h < 2; h++) {
+                const int sj = 2 * sp + h;
+                float dx, dy, dz, k2, k3, arg;
+                bool near;
+                pair_geom<float, AMOEBA>(s_tp[ti], s_ta[ti], s_sp[sj], s_sa[sj], dx, dy, dz, k2, k3, arg, near);
+                pair_damp<float, AMOEBA>(arg, near, k2, k3);
+                const float d[3] = {dx, dy, dz};
 #pragma unroll
-            for (int rb = 0; rb < 6; rb++) {
-                const float4 av = *reinterpret_cast<const float4 *>(arow + rb * 16 * kMfmaLd + kk);
-                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, b0.x, acc[0][rb], 0, 0, 0);
-                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, b1.x, acc[1][rb], 0, 0, 0);
-                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, b0.y, acc[0][rb], 0, 0, 0);
-                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, b1.y, acc[1][rb], 0, 0, 0);
-                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, b0.z, acc[0][rb], 0, 0, 0);
-                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, b1.z, acc[1][rb], 0, 0, 0);
-                acc[0][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, b0.w, acc[0][rb], 0, 0, 0);
-                acc[1][rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, b1.w, acc[1][rb], 0, 0, 0);
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) v[a][3 * h + b] = k3 * d[a] * d[b] - (a == b ? k2 : 0.f);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                unsigned *rh = reinterpret_cast<unsigned *>(&Ah[3 * ti + a][6 * sp]);
+                unsigned *rl = reinterpret_cast<unsigned *>(&Al[3 * ti + a][6 * sp]);
+#pragma unroll
+                for (int u = 0; u < 3; u++) bx_split2(v[a][2 * u], v[a][2 * u + 1], rh[u], rl[u]);
             }
         }
+        __syncthreads();
+        // the next tile's sources, while this tile's products run
+        if (base + kBxSrc < i1) {
+            fetch_src(base + kBxSrc);
+            if (tid < kBxSrc) {
+                s_sp[tid] = spr;
+                s_sa[tid] = sar;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            bx8 ah[6], al[6];
+#pragma unroll
+            for (int rb = 0; rb < 6; rb++) {
+                ah[rb] = *reinterpret_cast<const bx8 *>(&Ah[16 * rb + l16][32 * q + 8 * g]);
+                al[rb] = *reinterpret_cast<const bx8 *>(&Al[16 * rb + l16][32 * q + 8 * g]);
+            }
+#pragma unroll
+            for (int rb = 0; rb < 6; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++)
+                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh[cb][q], acc[cb][rb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 6; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++)
+                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bl[cb][q], acc[cb][rb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < 6; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++)
+                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bh[cb][q], acc[cb][rb], 0, 0, 0);
+        }
+        __syncthreads();  // this tile's A reads are done; the next sources are in LDS
     }
     // lane holds D[16 rb + 4 g + i][l16] of column blocks 2 wv, 2 wv + 1
 #pragma unroll
     for (int cb = 0; cb < 2; cb++) {
         const int c = 32 * wv + 16 * cb + l16;
-        if (!s_act[c]) continue;
+        if (!cact[cb]) continue;
         float *part = static_cast<float *>(S.part) + ((long)w.z * ncols + c0 + c) * n + 3L * t0;
 #pragma unroll
         for (int rb = 0; rb < 6; rb++)
@@ -381,7 +443,8 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_mfma(const BSeg *__restrict__
     }
 }
 
-// FP32 copy of the inner search directions for the matrix-core matvec.
+// Split bf16 copy (hi + lo) of the inner search directions for the matrix-core
+// matvec; the first block of a column also zeroes the row's padding.
 __global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                      const int4 *__restrict__ work) {
     const int4 w = work[blockIdx.x];
@@ -389,7 +452,18 @@ __global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ se
     const BSeg &S = segs[w.x];
     const long n = 3L * S.npol;
     const long k = (long)w.z * kBlock + threadIdx.x;
-    if (k < n) S.pf[(long)w.y * n + k] = (float)S.p[(long)w.y * n + k];
+    __bf16 *xh = reinterpret_cast<__bf16 *>(S.xh) + (long)w.y * S.ldx;
+    __bf16 *xl = reinterpret_cast<__bf16 *>(S.xl) + (long)w.y * S.ldx;
+    if (k < n) {
+        const float v = (float)S.p[(long)w.y * n + k];
+        const __bf16 h = (__bf16)v;
+        xh[k] = h;
+        xl[k] = (__bf16)(v - (float)h);
+    }
+    if (w.z == 0 && n + threadIdx.x < S.ldx) {
+        xh[n + threadIdx.x] = (__bf16)0.f;
+        xl[n + threadIdx.x] = (__bf16)0.f;
+    }
 }
 
 // h = -(sum of split partials) + (1 - s) corrections + x / alpha for the
@@ -711,7 +785,7 @@ static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, doubl
         mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.sym32, t.nsym32, t.fin32, t.nfin32, 1);
     } else if (t.npair32) {
         hipLaunchKernelGGL(k_bconvert, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
-        hipLaunchKernelGGL(k_bpairs_mfma<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);
+        hipLaunchKernelGGL(k_bpairs_bx3<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);
         hipLaunchKernelGGL((k_bfinal<float, kBatchM32, AMOEBA>), dim3(t.nfin32), dim3(kBlock), 0, st, t.segs,
                            t.cols, t.fin32, 1);
     }

@@ -364,10 +364,13 @@ template <typename R, int M, bool AMOEBA>
 __device__ __forceinline__ void sym_block_pair_any(int npol, int I, int J, const typename V4<R>::t *__restrict__ site,
                                                    const R *__restrict__ adamp, const double *const (&xcol)[M],
                                                    R *const (&out_i)[M], R *const (&out_j)[M], char *lds) {
-    if (kSymTwoTargets<R, M> && I != J)
-        sym_block_pair2<R, M, AMOEBA>(npol, I, J, site, adamp, xcol, out_i, out_j, lds);
-    else
-        sym_block_pair<R, M, AMOEBA>(npol, I, J, site, adamp, xcol, out_i, out_j, lds);
+    if constexpr (kSymTwoTargets<R, M>) {
+        if (I != J) {
+            sym_block_pair2<R, M, AMOEBA>(npol, I, J, site, adamp, xcol, out_i, out_j, lds);
+            return;
+        }
+    }
+    sym_block_pair<R, M, AMOEBA>(npol, I, J, site, adamp, xcol, out_i, out_j, lds);
 }
 
 }  // namespace ommp
