@@ -119,6 +119,14 @@ double *ommp_amd_get_pol(OMMP_SYSTEM_PRT s);
 double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const double *x, double *y,
                         int32_t repeat);
 
+/* The same y = T x for 1 to 128 host columns x, y [ncols][pol_atoms][3] through
+   the inner matvec of the multi-RHS batched solver: damped tensors built in
+   FP32, tensors and x split in three bf16 parts each, the six leading products
+   summed on the matrix cores with FP32 accumulation.  No reference counterpart:
+   it exposes that kernel alone so that its accuracy and speed can be checked.
+   Returns the mean device time per launch of repeat launches (ms). */
+double ommp_amd_block_tmatvec(OMMP_SYSTEM_PRT s, int32_t ncols, const double *x, double *y, int32_t repeat);
+
 /* Live kernel timing on the system's stream: when enabled, every pair-kernel
    launch of the T matvec (on-the-fly, both precisions) and every packed-T
    stream launch is bracketed by HIP events.  stats receives, per class

@@ -74,6 +74,7 @@ SIGNATURES = [
     ("ommp_amd_profile_kernels_every", None, [c_vp, ctypes.c_int32]),
     ("ommp_amd_get_kernel_stats", None, [c_vp, c_dp, ctypes.c_bool]),
     ("ommp_amd_tmatvec", ctypes.c_double, [c_vp, ctypes.c_int32, ctypes.c_int32, c_dp, c_dp, ctypes.c_int32]),
+    ("ommp_amd_block_tmatvec", ctypes.c_double, [c_vp, ctypes.c_int32, c_dp, c_dp, ctypes.c_int32]),
     ("ommp_potential_mmpol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_potential_mm2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),
     ("ommp_potential_pol2ext", None, [c_vp, ctypes.c_int32, c_dp, c_dp]),

@@ -43,7 +43,7 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
         L.nsplit = (npol + L.chunk - 1) / L.chunk;
     }
     L.vec = align256(sizeof(double) * ncols * n);
-    L.xs = m32 == kBatchM32 ? align256(sizeof(unsigned short) * ncols * (size_t)bx_ld(npol)) : 0;
+    L.xs = m32 == kBatchM32 ? align256(sizeof(unsigned short) * kBxParts * ncols * (size_t)bx_ld(npol)) : 0;
     // the pair-once sweep at the sizes where the single-system solver uses it
     // (the matrix-core FP32 matvec keeps its own tiling); many columns may take
     // up to 8 GiB of block-pair partials (15,000 sites x 128 columns: 2.8 GB,
@@ -60,7 +60,7 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
     L.eout = align256(sizeof(double) * nrhs);
     L.charges = align256(sizeof(double4) * kMaxCharges);
     // 8 vectors (b, x, r, z, p, h, d, r2), two kinds of r partials
-    L.total = 8 * L.vec + 2 * L.xs + L.part + L.red_hp + 2 * L.red_rz + L.ext + L.ered + L.eout + L.charges;
+    L.total = 8 * L.vec + L.xs + L.part + L.red_hp + 2 * L.red_rz + L.ext + L.ered + L.eout + L.charges;
     return L;
 }
 
@@ -201,9 +201,7 @@ int BatchEngine::add(const DeviceView &v, int nrhs, int extra) {
         off += L.vec;
     }
     d.ldx = bx_ld(v.npol);
-    d.xh = L.xs ? reinterpret_cast<unsigned short *>(mem + off) : nullptr;
-    off += L.xs;
-    d.xl = L.xs ? reinterpret_cast<unsigned short *>(mem + off) : nullptr;
+    d.xs = L.xs ? reinterpret_cast<unsigned short *>(mem + off) : nullptr;
     off += L.xs;
     d.part = mem + off;
     off += L.part;
@@ -353,6 +351,12 @@ void BatchEngine::commit() {
 void BatchEngine::tick(double eta, double frac) {
     commit();
     launch_batch_tick(stream, tables_, amoeba, m32, m64, eta, frac);
+}
+
+void BatchEngine::inner_matvec() {
+    if (m32 != kBatchM32) fatal("The inner matvec alone runs on wide batches only");
+    commit();
+    launch_batch_inner_matvec(stream, tables_, amoeba);
 }
 
 long BatchEngine::enqueue_readback() {

@@ -73,10 +73,10 @@ struct BSeg {
     const int *pp_ci;
     const double *pp_f;
     double *b, *x, *r, *z, *p, *h, *d;  // [ncols][3 npol]
-    // matrix-core path: p of the inner solves split as hi + lo, two bf16 values
-    // per element, [ncols][ldx], ldx = 3 * (npol rounded up to kBxSrc); the
-    // elements past 3 npol are zero
-    unsigned short *xh, *xl;
+    // matrix-core path: p of the inner solves split in kBxParts bf16 parts,
+    // [part][ncols][ldx], ldx = 3 * (npol rounded up to kBxSrc); the elements
+    // past 3 npol are zero
+    unsigned short *xs;
     int ldx;
     // partial sums of the matvec: per-target sweep [nsplit][ncols][3 npol]; pair-once
     // sweep (sym32 / sym64 set for that precision, tmv_sym.h) [block pair][ncols][3][256]
@@ -125,8 +125,13 @@ constexpr int kBatchM32 = 8;  // columns per FP32 finalize work item (multi-RHS)
 constexpr int kMfmaSites = 32;
 constexpr int kMfmaCols = 128;
 constexpr int kBxSrc = 32;
+constexpr int kBxParts = 3;  // bf16 parts per FP32 value (8 significant bits each)
 inline int bx_ld(int npol) { return 3 * ((npol + kBxSrc - 1) / kBxSrc) * kBxSrc; }
 constexpr int kBatchM64 = 4;  // columns per FP64 pair work item (multi-RHS)
+
+// The inner matvec of the wide (matrix-core) batch alone: h = T p for every
+// column whose inner phase is set (a diagnostic of its accuracy and speed).
+void launch_batch_inner_matvec(hipStream_t st, const BatchTables &t, bool amoeba);
 
 // Column-group width used for a segment of ncols columns.
 inline int batch_m32(int ncols) { return ncols <= 2 ? 2 : kBatchM32; }

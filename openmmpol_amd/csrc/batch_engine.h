@@ -45,6 +45,8 @@ class BatchEngine {
     // Uploads descriptors and work lists when the composition changed.
     void commit();
     void tick(double eta, double frac);
+    // h = T p of the inner phase alone (wide engines only; see launch_batch_inner_matvec).
+    void inner_matvec();
     // Copies the column states to the next pinned snapshot; returns its number.
     long enqueue_readback();
     // Waits for snapshot number seq and returns its column array.

@@ -237,13 +237,13 @@ __global__ __launch_bounds__(kSymBlock) void k_bsym(const BSeg *__restrict__ seg
     }
 }
 
-// ------------------------------------------------- bf16x3 matrix-core matvec
+// ------------------------------------------------- bf16x6 matrix-core matvec
 // Multi-RHS form of the on-the-fly FP32 matvec as a dense contraction on the
-// bf16 matrix cores.  Every operand v is split as v = hi + lo (two bf16 values,
-// round to nearest: |v - hi - lo| <= 2^-16 |v|) and A X is summed as
-// A_hi X_hi + A_hi X_lo + A_lo X_hi, accumulated in FP32; the dropped A_lo X_lo
-// term is of the same 2^-16 relative size.  The inner solves of the mixed
-// solver tolerate that: the FP64 refinement steps measure the true residual.
+// bf16 matrix cores, to FP32 accuracy.  Every FP32 operand v is split in three
+// bf16 parts, v = v0 + v1 + v2 (round to nearest at each step: 8 significant
+// bits per part, 24 in all, as in FP32), and A X is summed over the six products
+// A_i X_j with i + j <= 2, accumulated in FP32; the dropped ones (i + j >= 3)
+// are below 2^-24 relative, the size of FP32's own rounding.
 // Tiles: 32 target sites (96 rows, rows 3 t + a) x 32 source sites (K = 96,
 // k = 3 s + b, three steps of v_mfma_f32_16x16x32_bf16) x 128 columns.  The
 // damped dipole tensors k3 d d^T - k2 I of a tile are built in FP32 and stored
@@ -270,16 +270,20 @@ __device__ __forceinline__ float4 gload(const float4 *p) {
 }
 __device__ __forceinline__ float gload(const float *p) { return *(const float __attribute__((address_space(1))) *)p; }
 
-// hi and lo halves of two FP32 values, packed as bf16 pairs
-__device__ __forceinline__ void bx_split2(float v0, float v1, unsigned &hi, unsigned &lo) {
-    const __bf16 h0 = (__bf16)v0, h1 = (__bf16)v1;
-    const __bf16 l0 = (__bf16)(v0 - (float)h0), l1 = (__bf16)(v1 - (float)h1);
-    hi = (unsigned)__builtin_bit_cast(unsigned short, h0) | ((unsigned)__builtin_bit_cast(unsigned short, h1) << 16);
-    lo = (unsigned)__builtin_bit_cast(unsigned short, l0) | ((unsigned)__builtin_bit_cast(unsigned short, l1) << 16);
+// v = p0 + p1 + p2, three bf16 parts rounded to nearest in turn (each
+// subtraction is exact in FP32)
+__device__ __forceinline__ void bx_split3(float v, __bf16 (&p)[kBxParts]) {
+    p[0] = (__bf16)v;
+    const float r1 = v - (float)p[0];
+    p[1] = (__bf16)r1;
+    p[2] = (__bf16)(r1 - (float)p[1]);
+}
+__device__ __forceinline__ unsigned bx_pack(__bf16 a, __bf16 b) {
+    return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
 }
 
 template <bool AMOEBA>
-__global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+__global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                        const int4 *__restrict__ work) {
     const int4 w = work[blockIdx.x];
     const BSeg &S = segs[w.x];
@@ -289,8 +293,7 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ 
     const float4 *__restrict__ ps32 = S.ps32;
     const float *__restrict__ pa32 = S.pa32;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, l16 = lane & 15;
-    __shared__ __align__(16) __bf16 Ah[kMfmaRows][kBxLd];
-    __shared__ __align__(16) __bf16 Al[kMfmaRows][kBxLd];
+    __shared__ __align__(16) __bf16 As[kBxParts][kMfmaRows][kBxLd];
     __shared__ float4 s_tp[kMfmaSites], s_sp[kBxSrc];
     __shared__ float s_ta[kMfmaSites], s_sa[kBxSrc];
     __shared__ int s_act[kMfmaCols];
@@ -318,15 +321,15 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ 
     const int i0 = w.z * S.chunk;
     const int i1 = min(npol, i0 + S.chunk);
     // this lane's two columns and their split rows (inactive: never read)
+    const long plane = (long)ncols * ldx;
     bool cact[2];
-    gbx8 xrow_h[2], xrow_l[2];
+    gbx8 xrow[2];
 #pragma unroll
     for (int cb = 0; cb < 2; cb++) {
         const int c = 32 * wv + 16 * cb + l16;
         cact[cb] = s_act[c];
         const long o = cact[cb] ? (long)(c0 + c) * ldx + 8 * g : 0;
-        xrow_h[cb] = (gbx8)(reinterpret_cast<const __bf16 *>(S.xh) + o);
-        xrow_l[cb] = (gbx8)(reinterpret_cast<const __bf16 *>(S.xl) + o);
+        xrow[cb] = (gbx8)(reinterpret_cast<const __bf16 *>(S.xs) + o);
     }
     // source positions of the next tile go through registers
     float4 spr = make_float4(1e15f, 1e15f, 1e15f, 1.f);
@@ -358,15 +361,14 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ 
     const bx8 zero8 = {};
     for (int base = i0; base < i1; base += kBxSrc) {
         // X fragments of this tile: k = 3 base + 32 q + 8 g .. + 7 of the lane's columns
-        bx8 bh[2][3], bl[2][3];
+        bx8 bx[kBxParts][2][3];
         const long e = (3L * base) / 8;
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
+        for (int pt = 0; pt < kBxParts; pt++)
 #pragma unroll
-            for (int q = 0; q < 3; q++) {
-                bh[cb][q] = cact[cb] ? xrow_h[cb][e + 4 * q] : zero8;
-                bl[cb][q] = cact[cb] ? xrow_l[cb][e + 4 * q] : zero8;
-            }
+            for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) bx[pt][cb][q] = cact[cb] ? xrow[cb][pt * plane / 8 + e + 4 * q] : zero8;
         // A tile: two sources of one target per item, 3 rows x 6 k each
         for (int it = tid; it < kMfmaSites * kBxSrc / 2; it += kBlock) {
             const int ti = it >> 4, sp = it & 15;
@@ -385,12 +387,16 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ 
                     for (int b = 0; b < 3; b++) v[a][3 * h + b] = k3 * d[a] * d[b] - (a == b ? k2 : 0.f);
             }
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                unsigned *rh = reinterpret_cast<unsigned *>(&Ah[3 * ti + a][6 * sp]);
-                unsigned *rl = reinterpret_cast<unsigned *>(&Al[3 * ti + a][6 * sp]);
+            for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int u = 0; u < 3; u++) bx_split2(v[a][2 * u], v[a][2 * u + 1], rh[u], rl[u]);
-            }
+                for (int u = 0; u < 3; u++) {
+                    __bf16 p0[kBxParts], p1[kBxParts];
+                    bx_split3(v[a][2 * u], p0);
+                    bx_split3(v[a][2 * u + 1], p1);
+#pragma unroll
+                    for (int pt = 0; pt < kBxParts; pt++)
+                        reinterpret_cast<unsigned *>(&As[pt][3 * ti + a][6 * sp])[u] = bx_pack(p0[pt], p1[pt]);
+                }
         }
         __syncthreads();
         // the next tile's sources, while this tile's products run
@@ -403,27 +409,24 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ 
         }
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            bx8 ah[6], al[6];
 #pragma unroll
             for (int rb = 0; rb < 6; rb++) {
-                ah[rb] = *reinterpret_cast<const bx8 *>(&Ah[16 * rb + l16][32 * q + 8 * g]);
-                al[rb] = *reinterpret_cast<const bx8 *>(&Al[16 * rb + l16][32 * q + 8 * g]);
+                bx8 ax[kBxParts];
+#pragma unroll
+                for (int pt = 0; pt < kBxParts; pt++)
+                    ax[pt] = *reinterpret_cast<const bx8 *>(&As[pt][16 * rb + l16][32 * q + 8 * g]);
+                // the six products A_i X_j, i + j <= 2, smallest first
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++) {
+                    mfma_f4 c = acc[cb][rb];
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[2], bx[0][cb][q], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[1], bx[1][cb][q], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[0], bx[2][cb][q], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[1], bx[0][cb][q], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[0], bx[1][cb][q], c, 0, 0, 0);
+                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ax[0], bx[0][cb][q], c, 0, 0, 0);
+                }
             }
-#pragma unroll
-            for (int rb = 0; rb < 6; rb++)
-#pragma unroll
-                for (int cb = 0; cb < 2; cb++)
-                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb], bh[cb][q], acc[cb][rb], 0, 0, 0);
-#pragma unroll
-            for (int rb = 0; rb < 6; rb++)
-#pragma unroll
-                for (int cb = 0; cb < 2; cb++)
-                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bl[cb][q], acc[cb][rb], 0, 0, 0);
-#pragma unroll
-            for (int rb = 0; rb < 6; rb++)
-#pragma unroll
-                for (int cb = 0; cb < 2; cb++)
-                    acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb], bh[cb][q], acc[cb][rb], 0, 0, 0);
         }
         __syncthreads();  // this tile's A reads are done; the next sources are in LDS
     }
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx3(const BSeg *__restrict__ 
     }
 }
 
-// Split bf16 copy (hi + lo) of the inner search directions for the matrix-core
+// Three-part bf16 split of the inner search directions for the matrix-core
 // matvec; the first block of a column also zeroes the row's padding.
 __global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
                                                      const int4 *__restrict__ work) {
@@ -452,18 +455,17 @@ __global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ se
     const BSeg &S = segs[w.x];
     const long n = 3L * S.npol;
     const long k = (long)w.z * kBlock + threadIdx.x;
-    __bf16 *xh = reinterpret_cast<__bf16 *>(S.xh) + (long)w.y * S.ldx;
-    __bf16 *xl = reinterpret_cast<__bf16 *>(S.xl) + (long)w.y * S.ldx;
+    const long plane = (long)S.ncols * S.ldx;
+    __bf16 *xs = reinterpret_cast<__bf16 *>(S.xs) + (long)w.y * S.ldx;
     if (k < n) {
-        const float v = (float)S.p[(long)w.y * n + k];
-        const __bf16 h = (__bf16)v;
-        xh[k] = h;
-        xl[k] = (__bf16)(v - (float)h);
+        __bf16 pt[kBxParts];
+        bx_split3((float)S.p[(long)w.y * n + k], pt);
+#pragma unroll
+        for (int q = 0; q < kBxParts; q++) xs[q * plane + k] = pt[q];
     }
-    if (w.z == 0 && n + threadIdx.x < S.ldx) {
-        xh[n + threadIdx.x] = (__bf16)0.f;
-        xl[n + threadIdx.x] = (__bf16)0.f;
-    }
+    if (w.z == 0 && n + threadIdx.x < S.ldx)
+#pragma unroll
+        for (int q = 0; q < kBxParts; q++) xs[q * plane + n + threadIdx.x] = (__bf16)0.f;
 }
 
 // h = -(sum of split partials) + (1 - s) corrections + x / alpha for the
@@ -785,7 +787,7 @@ static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, doubl
         mv_launch<float, 2, AMOEBA>(st, t, t.pair32, t.npair32, t.sym32, t.nsym32, t.fin32, t.nfin32, 1);
     } else if (t.npair32) {
         hipLaunchKernelGGL(k_bconvert, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
-        hipLaunchKernelGGL(k_bpairs_bx3<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);
+        hipLaunchKernelGGL(k_bpairs_bx6<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);
         hipLaunchKernelGGL((k_bfinal<float, kBatchM32, AMOEBA>), dim3(t.nfin32), dim3(kBlock), 0, st, t.segs,
                            t.cols, t.fin32, 1);
     }
@@ -799,6 +801,21 @@ void launch_batch_tick(hipStream_t st, const BatchTables &t, bool amoeba, int m3
     if (t.nvec == 0) return;
     if (amoeba) tick_t<true>(st, t, m32, m64, eta, frac);
     else tick_t<false>(st, t, m32, m64, eta, frac);
+}
+
+template <bool AMOEBA>
+static void inner_mv_t(hipStream_t st, const BatchTables &t) {
+    hipLaunchKernelGGL(k_bconvert, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
+    hipLaunchKernelGGL(k_bpairs_bx6<AMOEBA>, dim3(t.npair32), dim3(kBlock), 0, st, t.segs, t.cols, t.pair32);
+    hipLaunchKernelGGL((k_bfinal<float, kBatchM32, AMOEBA>), dim3(t.nfin32), dim3(kBlock), 0, st, t.segs, t.cols,
+                       t.fin32, 1);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_batch_inner_matvec(hipStream_t st, const BatchTables &t, bool amoeba) {
+    if (t.nvec == 0 || t.npair32 == 0) return;
+    if (amoeba) inner_mv_t<true>(st, t);
+    else inner_mv_t<false>(st, t);
 }
 
 // ------------------------------------------------------------------ helpers

@@ -253,6 +253,10 @@ double ommp_amd_tmatvec(OMMP_SYSTEM_PRT s, int32_t kind, int32_t ncols, const do
     return ommp::tmatvec_host(*sys(s), kind, ncols, x, y, repeat);
 }
 
+double ommp_amd_block_tmatvec(OMMP_SYSTEM_PRT s, int32_t ncols, const double *x, double *y, int32_t repeat) {
+    return ommp::block_tmatvec_host(*sys(s), ncols, x, y, repeat);
+}
+
 void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable) { ommp::profile_enable(*sys(s), enable ? 1 : 0); }
 
 void ommp_amd_profile_kernels_every(OMMP_SYSTEM_PRT s, int32_t every) {

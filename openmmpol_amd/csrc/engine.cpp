@@ -1348,13 +1348,8 @@ void solve_multi(System &s, int nrhs, const double *ext, bool ext_dev, double *i
     const long max_ticks = (long)(kBatchMaxOuter + 1) * (s.max_iter + 2);
     long prev = -1;
     const BCol *snap = nullptr;
-    // the matrix-core inner matvec carries ~2^-16 relative error per product, so
-    // an inner solve buys about 1e-4 of true reduction whatever it is asked for;
-    // asking for more only costs iterations (measured at 15,000 sites x 128
-    // columns: 1e-3 to 1e-6 all take three FP64 steps per column)
-    const double eta = E.m32 == kBatchM32 ? 1e-4 : refine_eta();
     for (long t = 0;; t++) {
-        E.tick(eta, refine_frac());
+        E.tick(refine_eta(), refine_frac());
         const long seq = E.enqueue_readback();
         if (prev >= 0) {
             snap = E.wait_readback(prev);
@@ -1643,6 +1638,36 @@ double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, 
     for (int r = 0; r < repeat; r++) matvec(s, d, kind, ncols, d.vd, d.vr2, true, nullptr, nullptr);
     HCHECK(hipEventRecord(d.ev[1], d.stream));
     HCHECK(hipMemcpyAsync(y, d.vr2, sizeof(double) * ncols * n, hipMemcpyDeviceToHost, d.stream));
+    HCHECK(hipStreamSynchronize(d.stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);
+    return ms / repeat;
+}
+
+// y = T x for ncols host vectors through the inner matvec of the multi-RHS
+// batch (bf16 matrix cores, three-part split), with the diagonal; the columns
+// are grouped by dipole set as in a batch, so ncols need not be a multiple of
+// it.  Returns the mean device time per launch of repeat launches (ms).
+double block_tmatvec_host(System &s, int ncols, const double *x, double *y, int repeat) {
+    if (ncols < 1 || ncols > kMfmaCols) fatal("The block matvec takes 1 to 128 columns");
+    if (s.npol == 0) return 0.0;
+    DeviceView v = device_view(s);
+    DeviceState &d = *s.dev;
+    const long n = d.n;
+    const int nsets = s.n_ipd, nrhs = (ncols + nsets - 1) / nsets;
+    BatchEngine E(d.stream, s.amoeba, true, 1, nrhs * nsets);
+    if (E.add(v, nrhs) < 0) fatal("Cannot allocate device memory for the block matvec");
+    const SegInfo &S = E.segs[0];
+    std::vector<BCol> cols(nrhs * nsets);  // value-initialized: phase DONE
+    for (int c = 0; c < ncols; c++) cols[c].s[1].phase = kPhaseInner;
+    HCHECK(hipMemcpyAsync(E.d_cols, cols.data(), sizeof(BCol) * cols.size(), hipMemcpyHostToDevice, d.stream));
+    HCHECK(hipMemcpyAsync(S.dev.p, x, sizeof(double) * ncols * n, hipMemcpyHostToDevice, d.stream));
+    E.inner_matvec();
+    if (repeat < 1) repeat = 1;
+    HCHECK(hipEventRecord(d.ev[0], d.stream));
+    for (int r = 0; r < repeat; r++) E.inner_matvec();
+    HCHECK(hipEventRecord(d.ev[1], d.stream));
+    HCHECK(hipMemcpyAsync(y, S.dev.h, sizeof(double) * ncols * n, hipMemcpyDeviceToHost, d.stream));
     HCHECK(hipStreamSynchronize(d.stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, d.ev[0], d.ev[1]);

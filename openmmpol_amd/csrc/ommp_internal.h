@@ -143,6 +143,7 @@ void invalidate_precond(System &s);  // the near-field preconditioner is rebuilt
 const double *device_ipd(System &s);
 void refresh_host_m2d(System &s);
 double tmatvec_host(System &s, int kind, int ncols, const double *x, double *y, int repeat);
+double block_tmatvec_host(System &s, int ncols, const double *x, double *y, int repeat);
 int inner_precond_apply_host(System &s, int ncols, const double *r, double *z);
 void profile_enable(System &s, int every);  // 0 off, n > 0: every n-th launch per class
 void profile_stats(System &s, double *out, bool reset);

@@ -344,6 +344,17 @@ class OMMPSystem:
                                     y.ctypes.data_as(c_dp), int(repeat))
         return y, ms
 
+    def block_tmatvec(self, x, repeat=1):
+        """y = T x for up to 128 columns x [ncols, pol_atoms, 3] through the
+        multi-RHS solver's matrix-core inner matvec; returns (y, mean ms per launch)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, self.pol_atoms, 3)
+        if not 1 <= x.shape[0] <= 128:
+            raise ValueError("block_tmatvec takes 1 to 128 columns")
+        y = np.zeros_like(x)
+        ms = lib().ommp_amd_block_tmatvec(self.handler, x.shape[0], x.ctypes.data_as(c_dp),
+                                          y.ctypes.data_as(c_dp), int(repeat))
+        return y, ms
+
     def profile_kernels(self, enable=True, every=1):
         """Bracket pair-kernel launches with HIP events (every `every`-th launch of
         each class when enabled)."""
