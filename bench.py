@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/s @15k sites, mixed vs FP64 at ≤1e-9 rel E_pol err; p50/p99 solve latency"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0  # dense matrix cores
 PEAK_HBM_GBS = 8000.0
 N_SITES, RADIUS, SEED = 15000, 33.0, 2
 PROF_EVERY = 4
@@ -195,6 +196,18 @@ def run_multi(tol, steps, bufs):
             e.append(s.get_polelec_energy())
         return (time.perf_counter() - t0) / count, np.array(e)
 
+    # the block matvec of the inner solves alone, 128 columns: algorithmic FLOPs
+    # N(N-1) x 18 x 128 (the 3x3 tensor times each column, every ordered pair);
+    # the matrix cores run six bf16 products per pair of split operands over
+    # tiles padded to 32-site multiples
+    xb = np.random.default_rng(7).normal(0.0, 0.05, size=(128, n, 3))
+    _, bmv_ms = s.block_tmatvec(xb, repeat=5)
+    npad = -(-n // 32) * 32
+    bmv = {"ms_per_launch": bmv_ms, "columns": 128,
+           "algorithmic_tflops": n * (n - 1) * 18 * 128 / (bmv_ms * 1e-3) / 1e12,
+           "mfma_bf16_tflops": 6 * npad * npad * 9 * 2 * 128 / (bmv_ms * 1e-3) / 1e12,
+           "peak_bf16_tflops": PEAK_BF16_TFLOPS}
+    bmv["mfma_frac"] = bmv["mfma_bf16_tflops"] / PEAK_BF16_TFLOPS
     t_seq_mixed, e_seq_mixed = sequential("mixed", MULTI_NRHS)
     t_seq_fp64, e_seq_fp64 = sequential("fp64", 16)
     s.terminate()
@@ -212,9 +225,11 @@ def run_multi(tol, steps, bufs):
         "workload": f"15000-site AMOEBA sphere with a 6 A cavity, {MULTI_NRHS} fields of a 30-charge QM region "
                     "displaced N(0, 0.1 A) per field; guess = dipoles of the undisplaced-field solve",
         "rhs_solves_per_s": MULTI_NRHS / t_multi, "ms_per_call": t_multi * 1e3,
-        "inner_matvec": "T tiles and p split as bf16 hi + lo, hi*hi + hi*lo + lo*hi on the matrix cores "
-                        "(v_mfma_f32_16x16x32_bf16), FP32 accumulation; FP64 refinement residuals",
+        "inner_matvec": "FP32 tensors and p split in three bf16 parts each, the six products A_i X_j "
+                        "(i + j <= 2) on the matrix cores (v_mfma_f32_16x16x32_bf16), FP32 accumulation "
+                        "(FP32-class error); FP64 refinement residuals",
         "iterations": {"outer_fp64_max": it_multi[2], "inner_max": it_multi[3]},
+        "block_matvec": bmv,
         "sequential_mixed_solves_per_s": 1.0 / t_seq_mixed,
         "sequential_fp64_solves_per_s": 1.0 / t_seq_fp64,
         "speedup_vs_sequential_mixed": t_seq_mixed / t_multi * MULTI_NRHS,

@@ -270,16 +270,23 @@ __device__ __forceinline__ float4 gload(const float4 *p) {
 }
 __device__ __forceinline__ float gload(const float *p) { return *(const float __attribute__((address_space(1))) *)p; }
 
-// v = p0 + p1 + p2, three bf16 parts rounded to nearest in turn (each
-// subtraction is exact in FP32)
-__device__ __forceinline__ void bx_split3(float v, __bf16 (&p)[kBxParts]) {
-    p[0] = (__bf16)v;
-    const float r1 = v - (float)p[0];
-    p[1] = (__bf16)r1;
-    p[2] = (__bf16)(r1 - (float)p[1]);
+// Two FP32 values rounded to bf16 (nearest) and packed, a in the low half: one
+// v_cvt_pk_bf16_f32.
+__device__ __forceinline__ unsigned bx_cvt2(float a, float b) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, b2));
 }
-__device__ __forceinline__ unsigned bx_pack(__bf16 a, __bf16 b) {
-    return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+// v = p0 + p1 + p2 for two values at once: three bf16 parts, each rounded to
+// nearest from what the previous ones left (every subtraction is exact in FP32),
+// packed in pairs as the LDS tiles hold them.
+__device__ __forceinline__ void bx_split3(float v0, float v1, unsigned (&p)[kBxParts]) {
+#pragma unroll
+    for (int q = 0; q < kBxParts; q++) {
+        p[q] = bx_cvt2(v0, v1);
+        v0 -= __uint_as_float(p[q] << 16);
+        v1 -= __uint_as_float(p[q] & 0xffff0000u);
+    }
 }
 
 template <bool AMOEBA>
@@ -320,7 +327,9 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
     }
     const int i0 = w.z * S.chunk;
     const int i1 = min(npol, i0 + S.chunk);
-    // this lane's two columns and their split rows (inactive: never read)
+    // this lane's two columns and their split rows; an inactive column reads
+    // row 0 (in bounds, any values): it only feeds its own result column, which
+    // is not stored
     const long plane = (long)ncols * ldx;
     bool cact[2];
     gbx8 xrow[2];
@@ -358,7 +367,6 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
         }
     }
     __syncthreads();
-    const bx8 zero8 = {};
     for (int base = i0; base < i1; base += kBxSrc) {
         // X fragments of this tile: k = 3 base + 32 q + 8 g .. + 7 of the lane's columns
         bx8 bx[kBxParts][2][3];
@@ -368,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
 #pragma unroll
             for (int cb = 0; cb < 2; cb++)
 #pragma unroll
-                for (int q = 0; q < 3; q++) bx[pt][cb][q] = cact[cb] ? xrow[cb][pt * plane / 8 + e + 4 * q] : zero8;
+                for (int q = 0; q < 3; q++) bx[pt][cb][q] = xrow[cb][pt * plane / 8 + e + 4 * q];
         // A tile: two sources of one target per item, 3 rows x 6 k each
         for (int it = tid; it < kMfmaSites * kBxSrc / 2; it += kBlock) {
             const int ti = it >> 4, sp = it & 15;
@@ -379,7 +387,9 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
                 float dx, dy, dz, k2, k3, arg;
                 bool near;
                 pair_geom<float, AMOEBA>(s_tp[ti], s_ta[ti], s_sp[sj], s_sa[sj], dx, dy, dz, k2, k3, arg, near);
-                pair_damp<float, AMOEBA>(arg, near, k2, k3);
+                // every lane runs both items (512 per tile, 256 threads): the
+                // ballot is wave-uniform
+                if (__ballot(near)) pair_damp<float, AMOEBA>(arg, near, k2, k3);
                 const float d[3] = {dx, dy, dz};
 #pragma unroll
                 for (int a = 0; a < 3; a++)
@@ -390,12 +400,11 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
             for (int a = 0; a < 3; a++)
 #pragma unroll
                 for (int u = 0; u < 3; u++) {
-                    __bf16 p0[kBxParts], p1[kBxParts];
-                    bx_split3(v[a][2 * u], p0);
-                    bx_split3(v[a][2 * u + 1], p1);
+                    unsigned pk[kBxParts];
+                    bx_split3(v[a][2 * u], v[a][2 * u + 1], pk);
 #pragma unroll
                     for (int pt = 0; pt < kBxParts; pt++)
-                        reinterpret_cast<unsigned *>(&As[pt][3 * ti + a][6 * sp])[u] = bx_pack(p0[pt], p1[pt]);
+                        reinterpret_cast<unsigned *>(&As[pt][3 * ti + a][6 * sp])[u] = pk[pt];
                 }
         }
         __syncthreads();
@@ -458,10 +467,10 @@ __global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ se
     const long plane = (long)S.ncols * S.ldx;
     __bf16 *xs = reinterpret_cast<__bf16 *>(S.xs) + (long)w.y * S.ldx;
     if (k < n) {
-        __bf16 pt[kBxParts];
-        bx_split3((float)S.p[(long)w.y * n + k], pt);
+        unsigned pk[kBxParts];
+        bx_split3((float)S.p[(long)w.y * n + k], 0.f, pk);
 #pragma unroll
-        for (int q = 0; q < kBxParts; q++) xs[q * plane + k] = pt[q];
+        for (int q = 0; q < kBxParts; q++) reinterpret_cast<unsigned short *>(xs)[q * plane + k] = (unsigned short)pk[q];
     }
     if (w.z == 0 && n + threadIdx.x < S.ldx)
 #pragma unroll
