@@ -315,14 +315,21 @@ def run_serving(tol, nreq, seed, mean_gap_ms, max_active, bufs):
     lat = np.array(lat)
     seq_lat = np.array(seq_lat)
     req_size = np.concatenate([np.full(nsteps[k], sizes[k]) for k in range(nreq)])
+    # a request's first solve counts from its arrival (admission, upload and
+    # static field included); later solves from the previous solve's completion
+    first = np.concatenate([np.arange(nsteps[k]) == 0 for k in range(nreq)])
     by_size = {}
     for lo, hi in ((500, 2000), (2000, 10000), (10000, 40001)):
         sel = (req_size >= lo) & (req_size < hi)
         if sel.any():
+            later = sel & ~first
             by_size[f"{lo}-{min(hi, 40000)}"] = {
                 "solves": int(sel.sum()),
                 "server_p50_ms": float(np.percentile(lat[sel], 50)),
                 "server_p99_ms": float(np.percentile(lat[sel], 99)),
+                "server_first_solve_max_ms": float(lat[sel & first].max()),
+                "server_later_solves_p50_ms": float(np.percentile(lat[later], 50)),
+                "server_later_solves_p99_ms": float(np.percentile(lat[later], 99)),
                 "sequential_fp64_p50_ms": float(np.percentile(seq_lat[sel], 50)),
                 "sequential_fp64_p99_ms": float(np.percentile(seq_lat[sel], 99))}
     return {
