@@ -377,9 +377,15 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
             for (int cb = 0; cb < 2; cb++)
 #pragma unroll
                 for (int q = 0; q < 3; q++) bx[pt][cb][q] = xrow[cb][pt * plane / 8 + e + 4 * q];
-        // A tile: two sources of one target per item, 3 rows x 6 k each
-        for (int it = tid; it < kMfmaSites * kBxSrc / 2; it += kBlock) {
-            const int ti = it >> 4, sp = it & 15;
+        // A tile: thread (wave w, lane l) builds rows 3 t + a of targets
+        // t = l % 16 and l % 16 + 16 against sources 2 sp, 2 sp + 1 with
+        // sp = 4 w + l / 16 (3 rows x 6 k per target), so that it reads its two
+        // sources once for both targets; a wave covers 16 targets x 4 source
+        // pairs (packed stores at dword 156 t + 3 sp + const: distinct banks).
+        const int sp = 4 * (tid >> 6) + ((tid >> 4) & 3);
+#pragma unroll
+        for (int kt = 0; kt < 2; kt++) {
+            const int ti = (tid & 15) + 16 * kt;
             float v[3][6];
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -387,8 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
                 float dx, dy, dz, k2, k3, arg;
                 bool near;
                 pair_geom<float, AMOEBA>(s_tp[ti], s_ta[ti], s_sp[sj], s_sa[sj], dx, dy, dz, k2, k3, arg, near);
-                // every lane runs both items (512 per tile, 256 threads): the
-                // ballot is wave-uniform
+                // every lane runs both targets: the ballot is wave-uniform
                 if (__ballot(near)) pair_damp<float, AMOEBA>(arg, near, k2, k3);
                 const float d[3] = {dx, dy, dz};
 #pragma unroll
