@@ -184,10 +184,13 @@ void ommp_amd_set_external_fields_multi_device(OMMP_SYSTEM_PRT s, int32_t nrhs, 
 void ommp_amd_charges_field(OMMP_SYSTEM_PRT s, int32_t nq, const double *charges, double *out);
 
 /* ---------------------------------------------------------------- serving */
-/* A serving loop on the current GPU: a worker thread that admits requests
-   while batch slots (max_active) and the HBM budget (bytes, counted as the
-   device memory of each admitted system plus its batch state) allow, and runs
-   all admitted solves together in ticks of the batched mixed solver.  A request
+/* A serving loop on the current GPU, in two lanes by system size (below 4,096
+   polarizable sites, and the rest), each a worker thread with its own stream:
+   requests are admitted in arrival order while their lane's batch slots
+   (max_active per lane) and the shared HBM budget (bytes, counted as the
+   device memory of each admitted system plus its batch state) allow, and each
+   lane runs its admitted solves together in ticks of the batched mixed solver.
+   A request
    is one system and an SCF sequence of nsteps solves; the external field of
    step k is that of the point charges charges[k][nq][4] (x, y, z, q).  Each
    solve starts from the previous step's dipoles.  Solves that converge leave
