@@ -11,8 +11,10 @@ class Server:
 
     submit(system, charges) queues an SCF-like request: charges is
     [nsteps, nq, 4] (x, y, z, q) and step k's external field is that of
-    charges[k].  Requests are admitted while `max_active` slots and
-    `hbm_budget` bytes allow, and all admitted solves share every launch."""
+    charges[k].  Requests run in two lanes by system size (below 4,096
+    polarizable sites, and the rest); they are admitted in arrival order while
+    their lane's `max_active` slots and the shared `hbm_budget` bytes allow, and
+    the admitted solves of a lane share every launch of that lane."""
 
     def __init__(self, hbm_budget=64e9, max_active=16):
         self.h = lib().ommp_amd_server_create(float(hbm_budget), int(max_active))
