@@ -33,6 +33,14 @@ OMMP_SYSTEM_PRT ommp_init_mmp(const char *filename);
 void ommp_set_default_solver(OMMP_SYSTEM_PRT s, int32_t solver);
 /* :20; mod_electrostatics.F90 set_def_matv (fatal on unknown id) */
 void ommp_set_default_matv(OMMP_SYSTEM_PRT s, int32_t matv);
+/* :23; mod_interface.F90:134-168 + remove_null_pol (mod_electrostatics.F90:469-536):
+   the MM atoms nopol[0..n) (1-based; fatal outside 1..mm_atoms) lose their
+   polarizability and the polarizable set is compacted in order; the dipoles
+   are dropped, and the pointers of the polarizable-site getters (ipd, cpol,
+   polar_mm, pol) must be fetched again.  The Thole factors keep their values.
+   The reference's loop reads its counter where it means the list entry (:156);
+   the two agree for the lists 1..n its callers pass. */
+void ommp_turn_pol_off(OMMP_SYSTEM_PRT s, int32_t n, const int32_t *nopol);
 /* :24; mod_interface.F90:161-170 */
 void ommp_terminate(OMMP_SYSTEM_PRT s);
 /* :28; mod_io.F90:92-103 (clamped to 0..3) */

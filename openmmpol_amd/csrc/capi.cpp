@@ -87,6 +87,10 @@ double *ommp_get_ipd(OMMP_SYSTEM_PRT s) {
 
 int32_t *ommp_get_polar_mm(OMMP_SYSTEM_PRT s) { return sys(s)->polar_mm.data(); }
 
+void ommp_turn_pol_off(OMMP_SYSTEM_PRT s, int32_t n, const int32_t *nopol) {
+    ommp::turn_pol_off(*sys(s), n, nopol);
+}
+
 void ommp_update_coordinates(OMMP_SYSTEM_PRT s, const double *new_c) {
     // update_coordinates (mod_mmpol.F90:325-372)
     System &p = *sys(s);

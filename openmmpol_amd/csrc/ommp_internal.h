@@ -95,6 +95,7 @@ struct System {
 
 // prep.cpp
 System *read_mmp(const std::string &path);
+void turn_pol_off(System &s, int n, const int *nopol);  // 1-based MM indices
 System *build_from_arrays(int ff_type, int ff_rules, int nmm, const double *cmm, const double *q,
                           const double *pol, const double *thole_a, const int *adj_ri,
                           const int *adj_ci, const int *pgroup, const int *frames);

@@ -342,6 +342,61 @@ void prepare(System &s) {
     s.e_m2d.assign((size_t)s.n_ipd * s.npol * 3, 0.0);
 }
 
+// ommp_turn_pol_off (mod_interface.F90:134-168) and the remove_null_pol it
+// ends with (mod_electrostatics.F90:469-536): the listed MM atoms (1-based)
+// lose their polarizability, the polarizable set is compacted in order and the
+// dipoles are dropped (zero, no guess).  The Thole factors keep their values
+// (thole_init does not run again), so a switched-off atom's pairs stay damped
+// as sources, while its damping parameter a no longer enters (it is not
+// polarizable in the combination rule).  Two points where the reference is not
+// self-consistent are resolved: its loop reads mm_polar at the loop counter
+// (:156) rather than at the listed atom, the same atom for the lists 1..n its
+// callers pass; and it keeps the screening lists built for the old polarizable
+// numbering, which are rebuilt here.  The device state is released and built
+// again for the new set at the next use.
+void turn_pol_off(System &s, int n, const int *nopol) {
+    if (n <= 0) return;
+    const int mn = *std::min_element(nopol, nopol + n), mx = *std::max_element(nopol, nopol + n);
+    if (mn < 1) fatal("Atom indexes are 1-based, so no index should be less than 1");
+    if (mx > s.nmm) fatal("Atom indexes should be inside the topology range");
+    for (int k = 0; k < n; k++) {
+        const int p = s.mm_polar[nopol[k] - 1];
+        if (p >= 0) s.pol[p] = 0.0;
+        else message("Atom " + std::to_string(nopol[k]) + " is not polarizable. Ignoring.", 1);
+    }
+    std::vector<int> keep;
+    for (int p = 0; p < s.npol; p++)
+        if (std::fabs(s.pol[p]) > kEpsRp) keep.push_back(p);
+    if ((int)keep.size() == s.npol) return;
+    message("Removing " + std::to_string(s.npol - (int)keep.size()) + " polarizabilities out of " +
+                std::to_string(s.npol), 1);
+    std::vector<double> pol;
+    std::vector<int> polar_mm;
+    for (int p : keep) {
+        pol.push_back(s.pol[p]);
+        polar_mm.push_back(s.polar_mm[p]);
+    }
+    s.pol.swap(pol);
+    s.polar_mm.swap(polar_mm);
+    s.npol = (int)s.polar_mm.size();
+    s.mm_polar.assign(s.nmm, -1);
+    for (int p = 0; p < s.npol; p++) s.mm_polar[s.polar_mm[p]] = p;
+    s.cpol.resize(3 * (size_t)s.npol);
+    for (int p = 0; p < s.npol; p++)
+        for (int c = 0; c < 3; c++) s.cpol[3 * p + c] = s.cmm[3 * s.polar_mm[p] + c];
+    make_screening_lists(s);
+    device_release(s);
+    s.dev_geometry_dirty = true;
+    s.ipd.assign((size_t)s.n_ipd * s.npol * 3, 0.0);
+    s.e_m2d.assign((size_t)s.n_ipd * s.npol * 3, 0.0);
+    s.m2d_done = s.ipd_done = s.ipd_use_guess = false;
+    s.host_ipd_stale = false;
+    s.host_m2d_stale = true;
+    for (int &k : s.pred_iters) k = 0;
+    s.pred_outer = 0;
+    message("Removing null polarizable sites done", 1);
+}
+
 // rotation_matrix without derivatives (rotate_multipoles.F90:298-463) and
 // rotate_multipoles (:232-295).
 void rotate_multipoles(System &s) {

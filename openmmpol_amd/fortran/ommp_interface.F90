@@ -112,6 +112,12 @@ module ommp_interface
             type(c_ptr), value :: s
             real(c_double), intent(in) :: c(*)
         end subroutine
+        subroutine c_ommp_turn_pol_off(s, n, nopol) bind(c, name='ommp_turn_pol_off')
+            import :: c_ptr, c_int32_t
+            type(c_ptr), value :: s
+            integer(c_int32_t), value :: n
+            integer(c_int32_t), intent(in) :: nopol(*)
+        end subroutine
         subroutine c_ommp_potential_mmpol2ext(s, n, cext, v) bind(c, name='ommp_potential_mmpol2ext')
             import :: c_ptr, c_double, c_int32_t
             type(c_ptr), value :: s
@@ -175,7 +181,7 @@ module ommp_interface
     public :: ommp_set_external_field, ommp_set_external_field_nomm
     public :: ommp_get_polelec_energy, ommp_get_fixedelec_energy, ommp_get_full_ele_energy
     public :: ommp_get_pol_atoms, ommp_get_mm_atoms, ommp_get_n_ipd
-    public :: ommp_ipd, ommp_polar_mm, ommp_update_coordinates
+    public :: ommp_ipd, ommp_polar_mm, ommp_update_coordinates, ommp_turn_pol_off
 
 contains
 
@@ -288,6 +294,15 @@ contains
         integer(ommp_integer), pointer :: p(:)
         call c_f_pointer(c_ommp_get_polar_mm(s%h), p, [ommp_get_pol_atoms(s)])
     end function
+
+    ! mod_interface.F90:134-168: the listed MM atoms (1-based) lose their
+    ! polarizability (see include/openmmpol.h)
+    subroutine ommp_turn_pol_off(s, n, nopol)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), intent(in) :: n
+        integer(ommp_integer), intent(in) :: nopol(n)
+        call c_ommp_turn_pol_off(s%h, n, nopol)
+    end subroutine
 
     ! mod_mmpol.F90:325-372
     subroutine ommp_update_coordinates(s, new_c)

@@ -235,6 +235,13 @@ class OMMPSystem:
                                      g.ctypes.data_as(c_dp))
         return g
 
+    def turn_pol_off(self, nopol):
+        """Switch off the polarizability of the MM atoms nopol (1-based, as the
+        reference's turn_pol_off); the polarizable set shrinks and the dipoles
+        are dropped."""
+        idx = np.ascontiguousarray(nopol, dtype=np.int32).reshape(-1)
+        lib().ommp_turn_pol_off(self.handler, len(idx), idx.ctypes.data_as(c_ip))
+
     def update_coordinates(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)
         if c.ndim != 2 or c.shape[0] != self.mm_atoms or c.shape[1] != 3:

@@ -395,6 +395,58 @@ void oracle_destroy(void *h)
     free(s);
 }
 
+/*
+ * ommp_turn_pol_off (mod_interface.F90:134-168) followed by remove_null_pol
+ * (mod_electrostatics.F90:469-536): the listed MM atoms (1-based) lose their
+ * polarizability and the polarizable set is compacted in order; thole(:)
+ * keeps its values (thole_init does not run again), so a switched-off atom's
+ * pairs stay damped as sources, while its damping parameter a no longer enters
+ * (it counts as not polarizable in the damped kernel's combination rule).  Two
+ * points where the reference is not self-consistent are resolved: its loop
+ * reads mm_polar at the loop counter (:156) rather than at the listed atom,
+ * the same atom for the lists 1..n its callers pass; and it keeps the
+ * screening lists and thole_damping of the old polarizable numbering, which
+ * are rebuilt and compacted here.  Returns -1 after a fatal error.
+ */
+int oracle_turn_pol_off(void *h, int n, const int *nopol)
+{
+    osys *s = (osys *)h;
+    if (n <= 0) return 0;
+    int mn = nopol[0], mx = nopol[0];
+    for (int k = 1; k < n; k++) {
+        if (nopol[k] < mn) mn = nopol[k];
+        if (nopol[k] > mx) mx = nopol[k];
+    }
+    if (mn < 1) {
+        oracle_fatal(s, "Atom indexes are 1-based, so no index should be less than 1");
+        return -1;
+    }
+    if (mx > s->nmm) {
+        oracle_fatal(s, "Atom indexes should be inside the topology range");
+        return -1;
+    }
+    for (int k = 0; k < n; k++) {
+        const int p = s->mm_polar[nopol[k] - 1];
+        if (p >= 0) s->pol[p] = 0.0;
+    }
+    int np = 0;
+    for (int p = 0; p < s->npol; p++)
+        if (fabs(s->pol[p]) > EPS_RP) {
+            s->polar_mm[np] = s->polar_mm[p];
+            s->pol[np] = s->pol[p];
+            s->thole_damping[np] = s->thole_damping[p];
+            np++;
+        }
+    if (np == s->npol) return 0;
+    s->npol = np;
+    for (int i = 0; i < s->nmm; i++) s->mm_polar[i] = -1;
+    for (int p = 0; p < np; p++) s->mm_polar[s->polar_mm[p]] = p;
+    csr_free(&s->list_S_P_P); csr_free(&s->list_S_P_D); csr_free(&s->list_P_P); csr_free(&s->list_S_S);
+    csr_free(&s->tS_P_P); csr_free(&s->tS_P_D); csr_free(&s->tP_P);
+    make_screening_lists(s);
+    return 0;
+}
+
 int oracle_npol(void *h) { return ((osys *)h)->npol; }
 int oracle_fatal_flag(void *h) { return ((osys *)h)->fatal; }
 const char *oracle_fatal_msg(void *h) { return ((osys *)h)->msg; }

@@ -38,6 +38,8 @@ def lib():
                                     _ip, _ip, _ip, _dp]
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_npol.argtypes = [ctypes.c_void_p]
+        L.oracle_turn_pol_off.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
+        L.oracle_turn_pol_off.restype = ctypes.c_int
         L.oracle_fatal_flag.argtypes = [ctypes.c_void_p]
         L.oracle_fatal_msg.argtypes = [ctypes.c_void_p]
         L.oracle_fatal_msg.restype = ctypes.c_char_p
@@ -113,6 +115,17 @@ class OracleSystem:
         if getattr(self, "h", None):
             lib().oracle_destroy(self.h)
             self.h = None
+
+    def turn_pol_off(self, nopol):
+        """ommp_turn_pol_off on the listed MM atoms (1-based), see oracle.c."""
+        idx = np.ascontiguousarray(nopol, dtype=np.int32)
+        lib().oracle_turn_pol_off(self.h, len(idx), _ptr(idx, _ip) if len(idx) else None)
+        self.check()
+        self.npol = lib().oracle_npol(self.h)
+        self.ipd = np.zeros((self.n_ipd, self.npol, 3))
+        self.ipd_done = False
+        self.ipd_use_guess = False
+        self.e_m2d = None
 
     def check(self):
         if lib().oracle_fatal_flag(self.h):

@@ -150,3 +150,43 @@ def test_synthetic_groups_are_connected():
                     seen.add(int(v))
                     stack.append(int(v))
         assert seen == mem
+
+
+@pytest.mark.parametrize("name,ff", [("1crn", "AMOEBA"), ("1crn", "WANG_AL")])
+def test_turn_pol_off_host_state_matches_oracle(name, ff, fixture_file):
+    """ommp_turn_pol_off (mod_interface.F90:134-168): the polarizable set,
+    positions, polarizabilities, Thole factors and screening lists afterwards
+    against the oracle's restatement (tests/test_oracle.py pins that one)."""
+    from openmmpol_amd import OMMPSystem
+    s = OMMPSystem(fixture_file(name, f"input_{ff}.mmp"))
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path(name, f"input_{ff}.mmp"))
+    o = oa.OracleSystem(prep)
+    pm = o.polar_mm
+    off = [int(pm[k]) + 1 for k in (0, 5, 6, 100, 101, 102, 300)] + [int(pm[-1]) + 1]
+    s.turn_pol_off(off)
+    o.turn_pol_off(off)
+    assert s.pol_atoms == o.npol == len(pm) - len(off)
+    assert np.array_equal(s.polar_mm, o.polar_mm)
+    assert np.array_equal(s.cpol, prep["cmm"][o.polar_mm])
+    assert np.array_equal(s.pol, prep["pol"][o.polar_mm])
+    assert np.abs(s.thole - o.thole()).max() < 1e-15
+    assert s.ipd.shape == (s.n_ipd, s.pol_atoms, 3) and not s.ipd.any()
+    for which in range(0, 7):
+        if which == 1 and not prep["amoeba"]:
+            continue
+        nrow = s.pol_atoms if which == 2 else s.mm_atoms
+        assert _csr_dict(*s.get_list(which), nrow) == _csr_dict(*o.list_csr(which), nrow), which
+    s.terminate()
+
+
+def test_turn_pol_off_errors_exit_1_like_the_reference():
+    """Indices outside 1..mm_atoms are fatal (mod_interface.F90:149-154)."""
+    base = ("import openmmpol_amd as o, numpy as np\n"
+            "s = o.OMMPSystem.from_arrays(True, np.zeros((2,3)) + [[0,0,0],[3,0,0]], np.zeros((2,10)), "
+            "[1.0, 1.0], [0,0,0], [], pgroup=[0,1])\n")
+    r = _run(base + "s.turn_pol_off([0])")
+    assert r.returncode == 1 and "Atom indexes are 1-based, so no index should be less than 1" in r.stdout
+    r = _run(base + "s.turn_pol_off([3])")
+    assert r.returncode == 1 and "Atom indexes should be inside the topology range" in r.stdout
+    r = _run(base + "s.turn_pol_off([1, 2]); print('npol', s.pol_atoms)")
+    assert r.returncode == 0 and "npol 0" in r.stdout

@@ -223,3 +223,56 @@ def test_pair_once_matvec_amber():
         got, _ = s.tmatvec(x, kind=kind)
         assert np.abs(got - want).max() <= tol * np.abs(want).max(), kind
     s.terminate()
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_turn_pol_off_solves_match_oracle(ff, fixture_file):
+    """ommp_turn_pol_off after a solve (device state built for the old set):
+    FP64 and mixed solves of the reduced set against the oracle's, the
+    fixed-multipole energy unchanged (it does not involve polarizabilities)."""
+    s, o = _pair("1crn", ff, fixture_file)
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    e_mm = s.get_fixedelec_energy()
+    pm = o.polar_mm
+    off = [int(pm[k]) + 1 for k in (0, 5, 6, 100, 101, 102, 300)] + [int(pm[-1]) + 1]
+    s.turn_pol_off(off)
+    o.turn_pol_off(off)
+    assert s.pol_atoms == o.npol and np.array_equal(s.polar_mm, o.polar_mm)
+    ef = mmp_io.read_ef(mmp_io.fixture_path("1crn", "EF_1.txt"))[o.polar_mm]
+    # FP64 against the oracle's CG at the same tolerance (same iterates)
+    o.ipd_use_guess = False
+    o.set_external_field(ef)
+    eo = o.get_polelec_energy()
+    s.set_external_field(ef)
+    e = s.get_polelec_energy()
+    print(f"{ff} fp64: {s.pol_atoms} sites left, E_pol {e:.12e} oracle {eo:.12e}", flush=True)
+    assert np.abs(s.ipd - o.ipd).max() < 1e-9
+    assert abs(e - eo) <= 1e-10 * abs(eo)
+    # mixed against the exact dense solve of the reduced set
+    E = o.m2d() + ef[None]
+    T = o.tmat()
+    exact = np.stack([np.linalg.solve(T, E[m].ravel()).reshape(-1, 3) for m in range(o.n_ipd)])
+    ee = o.energy(exact)
+    s.set_precision("mixed")
+    s.set_solver_tolerance(1e-10, 500)
+    s.reset_guess(False)
+    s.set_external_field(ef)
+    e = s.get_polelec_energy()
+    print(f"{ff} mixed: E_pol {e:.12e} exact {ee:.12e}", flush=True)
+    assert np.abs(s.ipd - exact).max() < 1e-7
+    assert abs(e - ee) <= 1e-9 * abs(ee)
+    assert s.get_fixedelec_energy() == e_mm
+    s.terminate()
+
+
+def test_turn_pol_off_every_site(fixture_file):
+    """The reference's own use (tests/test_programs/C/test_SI_potential.c:88-95):
+    every atom of a system switched off; no polarizable site is left and the
+    polarization energy is zero."""
+    s, o = _pair("N-methylacetamide", "AMOEBA", fixture_file)
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    s.turn_pol_off(np.arange(1, s.mm_atoms + 1))
+    assert s.pol_atoms == 0
+    s.set_external_field(np.zeros((0, 3)))
+    assert s.get_polelec_energy() == 0.0
+    s.terminate()

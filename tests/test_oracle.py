@@ -442,3 +442,42 @@ def test_fixed_charge_energy_closed_form():
             sij = msc[d - 1] if d <= 4 else 1.0
             want += sij * q[i, 0] * q[j, 0] / np.linalg.norm(cmm[i] - cmm[j])
     assert abs(o.energy_mm() - want) <= 1e-13 * abs(want)
+
+
+# ---------------------------------------------------------- turn_pol_off
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_oracle_turn_pol_off_restricts_the_operator(ff):
+    """oracle_turn_pol_off (ommp_turn_pol_off + remove_null_pol) pinned by what
+    it must leave unchanged: T of the remaining sites is the original T
+    restricted to them, bit for bit (same Thole factors, same damping a, same
+    group and bond screening), and so is their static field (the .mmp inputs
+    give every site the same a, so a switched-off source is damped as before)."""
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path("1crn", f"input_{ff}.mmp"))
+    o0 = oa.OracleSystem(prep)
+    T0, E0, pm0 = o0.tmat(), o0.m2d(), o0.polar_mm
+    off = [int(pm0[k]) + 1 for k in (0, 5, 6, 100, 101, 102, 300)] + [int(pm0[-1]) + 1]
+    o1 = oa.OracleSystem(prep)
+    o1.turn_pol_off(off)
+    pm1 = o1.polar_mm
+    keep = np.array([k for k in range(len(pm0)) if pm0[k] + 1 not in off])
+    assert o1.npol == len(pm0) - len(off) and np.array_equal(pm1, pm0[keep])
+    rows = (3 * keep[:, None] + np.arange(3)[None, :]).reshape(-1)
+    assert np.array_equal(o1.tmat(), T0[np.ix_(rows, rows)])
+    assert np.array_equal(o1.m2d(), E0[:, keep, :])
+    assert np.array_equal(o1.thole(), o0.thole())
+    # an atom that is not polarizable (or already off) is ignored
+    o1.turn_pol_off([off[0]])
+    assert o1.npol == len(keep)
+
+
+def test_oracle_turn_pol_off_all_and_errors():
+    prep = mmp_io.prepare_mmp(mmp_io.fixture_path("N-methylacetamide", "input_AMOEBA.mmp"))
+    o = oa.OracleSystem(prep)
+    o.turn_pol_off(np.arange(1, o.nmm + 1))
+    assert o.npol == 0
+    o = oa.OracleSystem(prep)
+    with pytest.raises(RuntimeError, match="1-based"):
+        o.turn_pol_off([0])
+    o = oa.OracleSystem(prep)
+    with pytest.raises(RuntimeError, match="topology range"):
+        o.turn_pol_off([o.nmm + 1])
