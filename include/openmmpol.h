@@ -118,6 +118,10 @@ double *ommp_get_q(OMMP_SYSTEM_PRT s);
 double *ommp_get_ipd(OMMP_SYSTEM_PRT s);
 /* :79; mod_c_interface.F90:1010-1026: 0-based MM index of each polarizable site */
 int32_t *ommp_get_polar_mm(OMMP_SYSTEM_PRT s);
+/* :74; mod_c_interface.F90:950-959: atomic number of each MM atom [mm_atoms];
+   zero for systems read from .mmp files, whose loader skips that section
+   (mod_inputloader.F90:131) after topology_init zeroed it */
+int32_t *ommp_get_zmm(OMMP_SYSTEM_PRT s);
 
 /* :86; mod_mmpol.F90:325-372: new coordinates [mm_atoms][3]; clears the cached
    static field, the dipoles and the warm-start guess, re-rotates multipoles. */

@@ -43,6 +43,7 @@ SIGNATURES = [
     ("ommp_get_q", c_dp, [c_vp]),
     ("ommp_get_ipd", c_dp, [c_vp]),
     ("ommp_get_polar_mm", c_ip, [c_vp]),
+    ("ommp_get_zmm", c_ip, [c_vp]),
     ("ommp_update_coordinates", None, [c_vp, c_dp]),
     ("ommp_turn_pol_off", None, [c_vp, ctypes.c_int32, c_ip]),
     ("ommp_enable_fmm", None, [c_vp]),

@@ -86,6 +86,7 @@ double *ommp_get_ipd(OMMP_SYSTEM_PRT s) {
 }
 
 int32_t *ommp_get_polar_mm(OMMP_SYSTEM_PRT s) { return sys(s)->polar_mm.data(); }
+int32_t *ommp_get_zmm(OMMP_SYSTEM_PRT s) { return sys(s)->atz.data(); }
 
 void ommp_turn_pol_off(OMMP_SYSTEM_PRT s, int32_t n, const int32_t *nopol) {
     ommp::turn_pol_off(*sys(s), n, nopol);

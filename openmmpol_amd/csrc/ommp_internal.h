@@ -40,6 +40,7 @@ struct System {
     std::vector<double> pol;       // [npol] bohr^3
     std::vector<int> polar_mm;     // [npol]
     std::vector<int> mm_polar;     // [nmm], -1 if not polarizable
+    std::vector<int> atz;          // [nmm] atomic numbers (0: not read, as the .mmp loader)
     std::vector<double> cpol;      // [npol][3]
     std::vector<double> thole;     // [nmm]
     std::vector<double> thole_damping_mm;  // [nmm] damping a per MM site

@@ -246,6 +246,7 @@ System *build_from_arrays(int ff_type, int ff_rules, int nmm, const double *cmm,
     s.ld_cart = s.amoeba ? 10 : 1;
     s.n_ipd = s.amoeba ? 2 : 1;
     s.cmm.assign(cmm, cmm + 3 * (size_t)nmm);
+    s.atz.assign(nmm, 0);
     s.q0.assign(q, q + (size_t)s.ld_cart * nmm);
     if (s.amoeba)
         for (int i = 0; i < nmm; i++)

@@ -107,6 +107,11 @@ module ommp_interface
             type(c_ptr), value :: s
             type(c_ptr) :: p
         end function
+        function c_ommp_get_zmm(s) bind(c, name='ommp_get_zmm') result(p)
+            import :: c_ptr
+            type(c_ptr), value :: s
+            type(c_ptr) :: p
+        end function
         subroutine c_ommp_update_coordinates(s, c) bind(c, name='ommp_update_coordinates')
             import :: c_ptr, c_double
             type(c_ptr), value :: s
@@ -181,7 +186,7 @@ module ommp_interface
     public :: ommp_set_external_field, ommp_set_external_field_nomm
     public :: ommp_get_polelec_energy, ommp_get_fixedelec_energy, ommp_get_full_ele_energy
     public :: ommp_get_pol_atoms, ommp_get_mm_atoms, ommp_get_n_ipd
-    public :: ommp_ipd, ommp_polar_mm, ommp_update_coordinates, ommp_turn_pol_off
+    public :: ommp_ipd, ommp_polar_mm, ommp_zmm, ommp_update_coordinates, ommp_turn_pol_off
 
 contains
 
@@ -293,6 +298,13 @@ contains
         type(ommp_system), intent(in) :: s
         integer(ommp_integer), pointer :: p(:)
         call c_f_pointer(c_ommp_get_polar_mm(s%h), p, [ommp_get_pol_atoms(s)])
+    end function
+
+    ! mod_c_interface.F90:950-959: atomic numbers [mm_atoms] (zero from .mmp input)
+    function ommp_zmm(s) result(p)
+        type(ommp_system), intent(in) :: s
+        integer(ommp_integer), pointer :: p(:)
+        call c_f_pointer(c_ommp_get_zmm(s%h), p, [ommp_get_mm_atoms(s)])
     end function
 
     ! mod_interface.F90:134-168: the listed MM atoms (1-based) lose their

@@ -144,6 +144,10 @@ class OMMPSystem:
     def polar_mm(self):
         return _view(lib().ommp_get_polar_mm(self.handler), (self.pol_atoms,))
 
+    @property
+    def zmm(self):
+        return _view(lib().ommp_get_zmm(self.handler), (self.mm_atoms,))
+
     # --------------------------------------------------------------- methods
     def set_external_field(self, external_field, nomm=False, solver="none", matv="none"):
         ef = np.ascontiguousarray(external_field, dtype=np.float64)

@@ -71,6 +71,16 @@ def test_host_preparation_matches_oracle(name, ff, fixture_file):
     s.terminate()
 
 
+def test_atomic_numbers_are_zero_from_mmp(fixture_file):
+    """ommp_get_zmm (mod_c_interface.F90:950-959): the .mmp loader skips the
+    atomic-number section (mod_inputloader.F90:131) of an array that
+    topology_init zeroed, so the getter returns mm_atoms zeros."""
+    from openmmpol_amd import OMMPSystem
+    s = OMMPSystem(fixture_file("1crn", "input_AMOEBA.mmp"))
+    assert s.zmm.shape == (s.mm_atoms,) and not s.zmm.any()
+    s.terminate()
+
+
 def test_host_preparation_matches_summary(fixture_file):
     from openmmpol_amd import OMMPSystem
     s = OMMPSystem(fixture_file("1crn", "input_AMOEBA.mmp"))
