@@ -69,6 +69,21 @@ void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double
                     const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d,
                     hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
 
+// The same one-sided over the MM sources src[0..nsrc) (the non-polarizable
+// ones), without finalize: part[s][3 * npol] for the nsplit source splits.
+void launch_rhs_pairs_list(hipStream_t st, bool amoeba, int nsrc, const int *src, int npol, const double4 *ms64,
+                           const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
+                           double *part, int nsplit);
+// Static field with each unordered pair of polarizable sites evaluated once
+// (rhs_sym.hip): pairs[] and sympart as for launch_tmv_sym; the nmm - npol
+// non-polarizable sources np_src are swept one-sided into part_np (nsplit_np
+// splits of 3 * npol).  rhs_pair_once() says from which size it is used.
+bool rhs_pair_once(int npol);
+void launch_rhs_sym(hipStream_t st, bool amoeba, int nmm, int npol, const int2 *pairs, const double4 *ms64,
+                    const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
+                    const int *polar_mm, const int *np_src, const CsrDev &sp, void *sympart, double *part_np,
+                    int nsplit_np, double *e_m2d, hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
+
 // T matvec on the fly: y[m] = T x[m] (dodiag) or (T - diag) x[m], m < M.
 // prec 64 or 32 selects the pair arithmetic; corrections and the final sum are FP64.
 // If dotpart != nullptr, per-block partial sums of y.x go to dotpart[m * nblk + b]

@@ -71,6 +71,10 @@ struct DeviceState {
     int nsplit = 1;
     double *rhs_part = nullptr;
     int rhs_split = 1;
+    // static field with the pair-once sweep: the non-polarizable MM sources and
+    // the source split of their one-sided sweep
+    int *np_src = nullptr;
+    int rhs_split_np = 1;
     // packed T
     void *tpack = nullptr;
     int tpack_prec = 0;
@@ -402,8 +406,18 @@ DeviceState &ensure_device(System &s) {
             const int tb = std::max(1, site_blocks(s.npol)), ncu = prop.multiProcessorCount;
             const int max_s = std::max(1, (s.nmm + kBlock - 1) / kBlock);
             d.rhs_split = std::min(max_s, std::max(balanced_split(tb, s.nmm, ncu), (6 * ncu + tb - 1) / tb));
+            const int nnp = s.nmm - s.npol, max_np = std::max(1, (nnp + kBlock - 1) / kBlock);
+            d.rhs_split_np = std::min(max_np, std::max(balanced_split(tb, std::max(nnp, 1), ncu), (6 * ncu + tb - 1) / tb));
         }
-        d.rhs_part = dalloc<double>(d, (size_t)d.rhs_split * n + 1);
+        d.rhs_part = dalloc<double>(d, (size_t)std::max(d.rhs_split, d.rhs_split_np) * n + 1);
+        if (s.nmm > s.npol) {
+            std::vector<int> np_src;
+            np_src.reserve(s.nmm - s.npol);
+            for (int i = 0; i < s.nmm; i++)
+                if (s.mm_polar[i] < 0) np_src.push_back(i);
+            d.np_src = dalloc<int>(d, np_src.size());
+            upload(np_src.data(), d.np_src, sizeof(int) * np_src.size());
+        }
         HCHECK(hipMemset(d.ipd, 0, sizeof(double) * 2 * n));
         HCHECK(hipMemset(d.e_m2d, 0, sizeof(double) * 2 * n));
         s.dev = dp;
@@ -513,6 +527,20 @@ void ensure_fmm(System &s, DeviceState &d) {
     d.fmm_valid = true;
 }
 
+// Block pairs and block-pair partials of the pair-once sweeps (T matvec and
+// static field), on first use.
+void ensure_sym(const System &s, DeviceState &d) {
+    if (d.sym_pairs) return;
+    const int nb = tmv_sym_blocks(s.npol);
+    std::vector<int2> pr;
+    pr.reserve((size_t)nb * (nb + 1) / 2);
+    for (int I = 0; I < nb; I++)
+        for (int J = I; J < nb; J++) pr.push_back(make_int2(I, J));
+    d.sym_pairs = dalloc<int2>(d, pr.size());
+    upload(pr.data(), d.sym_pairs, sizeof(int2) * pr.size());
+    d.sympart = dmalloc(d, tmv_sym_part_bytes(s.npol, 2, 64));
+}
+
 void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double *y, bool dodiag,
             double *dotpart, const CgState *skip) {
     const long n = d.n;
@@ -528,16 +556,7 @@ void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double 
         kind = kind == kOtf32 ? (sym ? kSym32 : kPlain32) : (sym ? kSym64 : kPlain64);
     }
     if (kind == kSym64 || kind == kSym32) {
-        if (!d.sym_pairs) {
-            const int nb = tmv_sym_blocks(s.npol);
-            std::vector<int2> pr;
-            pr.reserve((size_t)nb * (nb + 1) / 2);
-            for (int I = 0; I < nb; I++)
-                for (int J = I; J < nb; J++) pr.push_back(make_int2(I, J));
-            d.sym_pairs = dalloc<int2>(d, pr.size());
-            upload(pr.data(), d.sym_pairs, sizeof(int2) * pr.size());
-            d.sympart = dmalloc(d, tmv_sym_part_bytes(s.npol, 2, 64));
-        }
+        ensure_sym(s, d);
         const int pi = prof_begin(d, kind == kSym32 ? kOtf32 : kOtf64);
         launch_tmv_sym(d.stream, kind == kSym32 ? 32 : 64, s.amoeba, s.npol, M, d.sym_pairs, d.ps64, d.pa64, d.ps32,
                        d.pa32, d.alpha, d.pp, x, n, y, n, dodiag, dotpart, d.sympart,
@@ -1072,6 +1091,12 @@ void prepare_polelec(System &s) {
         fmm_launch_static_fields(d.stream, d.fmm, s.amoeba, s.ld_cart, d.fmm_rhs_part);
         launch_rhs_finalize(d.stream, s.amoeba, s.npol, kFmmNearSplits + 1, d.fmm_rhs_part, d.ms64, d.ma64, d.mq,
                             d.ps64, d.pa64, d.sp, d.e_m2d);
+    } else if (s.npol > 0 && rhs_pair_once(s.npol)) {
+        ensure_sym(s, d);
+        const int pi = prof_begin(d, 4);
+        launch_rhs_sym(d.stream, s.amoeba, s.nmm, s.npol, d.sym_pairs, d.ms64, d.ma64, d.mq, d.ps64, d.pa64,
+                       d.polar_mm, d.np_src, d.sp, d.sympart, d.rhs_part, d.rhs_split_np, d.e_m2d,
+                       pi >= 0 ? d.prof_pool[2 * pi] : nullptr, pi >= 0 ? d.prof_pool[2 * pi + 1] : nullptr);
     } else if (s.npol > 0) {
         const int pi = prof_begin(d, 4);
         launch_rhs_m2d(d.stream, s.amoeba, s.nmm, s.npol, d.ms64, d.ma64, d.mq, d.ps64, d.pa64, d.polar_mm, d.sp,
@@ -1254,12 +1279,12 @@ void refresh_host_ipd(System &s) {
 size_t device_footprint_estimate(const System &s) {
     const size_t np = s.npol, nm = s.nmm, n = 3 * np;
     size_t b = np * (sizeof(double4) + sizeof(float4) + 3 * sizeof(double) + sizeof(float) + sizeof(int));
-    b += nm * (sizeof(double4) + sizeof(double) + sizeof(double) * s.ld_cart);
+    b += nm * (sizeof(double4) + sizeof(double) + sizeof(double) * s.ld_cart + sizeof(int));
     b += sizeof(double) * (2 * n * 2 + n + 8 * 2 * n);     // e_m2d, ipd, ext, work vectors
     b += 3 * 24 * sizeof(double) * (n / kBlock + 16);      // reduction scratch
     const int nsplit = tmv_split(s.npol);
     b += tmv_part_bytes(s.npol, 2, 64, nsplit) + sizeof(double) * n * 16;  // pair and static-field partials
-    if (tmv_pair_once(s.npol, 64, 1)) {  // block-pair partials and list of the pair-once sweep (smallest size)
+    if (tmv_pair_once(s.npol, 64, 1) || rhs_pair_once(s.npol)) {  // block-pair partials and list of the pair-once sweeps
         const size_t nb = tmv_sym_blocks(s.npol);
         b += tmv_sym_part_bytes(s.npol, 2, 64) + sizeof(int2) * nb * (nb + 1) / 2;
     }

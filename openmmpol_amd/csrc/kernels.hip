@@ -28,53 +28,11 @@ int vec_blocks(long n) { return (int)((n + kBlock - 1) / kBlock); }
 int site_blocks(int npol) { return (npol + kBlock - 1) / kBlock; }
 
 // ====================================================================== RHS
-// Field at polarizable target j of MM source i (charge, dipole, quadrupole),
-// damped kernel with maxder 3 (AMOEBA) or 1 (Wang), as damped_coulomb_kernel +
-// q/mu/quad_elec_prop (mod_electrostatics.F90:1018-1405).  d = r_j - r_i.
+// Field of sources [blockIdx.y * chunk, +chunk) of the nsrc sources at every
+// polarizable target: the MM sites themselves (src == nullptr, the self pair
+// skipped) or the MM sites src[0..nsrc) (non-polarizable ones: never a target).
 template <bool AMOEBA>
-__device__ __forceinline__ void m2d_pair(double dx, double dy, double dz, double ir, double ti,
-                                         double tj, double ai, double aj, const double *q,
-                                         double &ex, double &ey, double &ez) {
-    const double ir2 = ir * ir;
-    double k2 = ir * ir2;
-    const double r = (dx * dx + dy * dy + dz * dz) * ir;
-    const double u = r * ti * tj;
-    if (AMOEBA) {
-        double k3 = k2 * ir2, k4 = k3 * ir2;
-        const double a = fmin(ai, aj);
-        const double au3 = a * u * u * u;
-        const bool near = au3 < 50.0;
-        if (__ballot(near)) {
-            // au3 = +inf for a non-polarizable source: masked by the select
-            const double e = exp(-au3);
-            k2 = near ? k2 * (1.0 - e) : k2;
-            k3 = near ? k3 * (1.0 - (1.0 + au3) * e) : k3;
-            k4 = near ? k4 * (1.0 - (1.0 + au3 + 0.6 * au3 * au3) * e) : k4;
-        }
-        const double md = q[1] * dx + q[2] * dy + q[3] * dz;
-        // quadrupole storage xx, xy, yy, xz, yz, zz
-        const double qx = q[4] * dx + q[5] * dy + q[7] * dz;
-        const double qy = q[5] * dx + q[6] * dy + q[8] * dz;
-        const double qz = q[7] * dx + q[8] * dy + q[9] * dz;
-        const double qrr = qx * dx + qy * dy + qz * dz;
-        const double cd = q[0] * k2 + 3.0 * md * k3 + 15.0 * qrr * k4;
-        ex = cd * dx - q[1] * k2 - 6.0 * qx * k3;
-        ey = cd * dy - q[2] * k2 - 6.0 * qy * k3;
-        ez = cd * dz - q[3] * k2 - 6.0 * qz * k3;
-    } else {
-        const bool near = u < 1.0;
-        if (__ballot(near)) {
-            const double u3 = u * u * u;
-            k2 = near ? k2 * (4.0 * u3 - 3.0 * u3 * u) : k2;
-        }
-        ex = q[0] * k2 * dx;
-        ey = q[0] * k2 * dy;
-        ez = q[0] * k2 * dz;
-    }
-}
-
-template <bool AMOEBA>
-__global__ __launch_bounds__(kBlock) void k_rhs_pairs(int nmm, int npol, int chunk,
+__global__ __launch_bounds__(kBlock) void k_rhs_pairs(int nsrc, int npol, int chunk, const int *__restrict__ src,
                                                       const double4 *__restrict__ ms,
                                                       const double *__restrict__ ma,
                                                       const double *__restrict__ mq,
@@ -91,18 +49,19 @@ __global__ __launch_bounds__(kBlock) void k_rhs_pairs(int nmm, int npol, int chu
     const bool valid = j < npol;
     const double4 pj = valid ? ps[j] : make_double4(0.0, 0.0, 0.0, 1.0);
     const double aj = valid ? pa[j] : 0.39;
-    const int jm = valid ? polar_mm[j] : -1;
+    const int jm = valid && !src ? polar_mm[j] : -1;
     double ex = 0.0, ey = 0.0, ez = 0.0;
     const int i0 = blockIdx.y * chunk;
-    const int i1 = min(nmm, i0 + chunk);
+    const int i1 = min(nsrc, i0 + chunk);
     for (int base = i0; base < i1; base += kBlock) {
         const int cnt = min(kBlock, i1 - base);
         __syncthreads();
         if (tid < cnt) {
-            s_pos[tid] = ms[base + tid];
-            s_a[tid] = ma[base + tid];
+            const int si = src ? src[base + tid] : base + tid;
+            s_pos[tid] = ms[si];
+            s_a[tid] = ma[si];
 #pragma unroll
-            for (int c = 0; c < LD; c++) s_q[c][tid] = mq[(long)LD * (base + tid) + c];
+            for (int c = 0; c < LD; c++) s_q[c][tid] = mq[(long)LD * si + c];
         }
         __syncthreads();
         for (int k = 0; k < cnt; k++) {
@@ -180,6 +139,21 @@ __global__ __launch_bounds__(kBlock) void k_rhs_finalize(int npol, int nsplit, c
     }
 }
 
+void launch_rhs_pairs_list(hipStream_t st, bool amoeba, int nsrc, const int *src, int npol, const double4 *ms64,
+                           const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
+                           double *part, int nsplit) {
+    if (npol == 0 || nsrc == 0) return;
+    const int chunk = (nsrc + nsplit - 1) / nsplit;
+    dim3 grid(site_blocks(npol), nsplit);
+    if (amoeba)
+        hipLaunchKernelGGL(k_rhs_pairs<true>, grid, dim3(kBlock), 0, st, nsrc, npol, chunk, src, ms64, ma64, mq, ps64,
+                           pa64, nullptr, part);
+    else
+        hipLaunchKernelGGL(k_rhs_pairs<false>, grid, dim3(kBlock), 0, st, nsrc, npol, chunk, src, ms64, ma64, mq,
+                           ps64, pa64, nullptr, part);
+    HIP_CHECK(hipGetLastError());
+}
+
 void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double4 *ms64,
                     const double *ma64, const double *mq, const double4 *ps64, const double *pa64,
                     const int *polar_mm, const CsrDev &sp, double *part, int nsplit, double *e_m2d,
@@ -189,10 +163,10 @@ void launch_rhs_m2d(hipStream_t st, bool amoeba, int nmm, int npol, const double
     dim3 grid(site_blocks(npol), nsplit);
     if (evb) HIP_CHECK(hipEventRecord(evb, st));
     if (amoeba)
-        hipLaunchKernelGGL(k_rhs_pairs<true>, grid, dim3(kBlock), 0, st, nmm, npol, chunk, ms64, ma64, mq,
+        hipLaunchKernelGGL(k_rhs_pairs<true>, grid, dim3(kBlock), 0, st, nmm, npol, chunk, nullptr, ms64, ma64, mq,
                            ps64, pa64, polar_mm, part);
     else
-        hipLaunchKernelGGL(k_rhs_pairs<false>, grid, dim3(kBlock), 0, st, nmm, npol, chunk, ms64, ma64, mq,
+        hipLaunchKernelGGL(k_rhs_pairs<false>, grid, dim3(kBlock), 0, st, nmm, npol, chunk, nullptr, ms64, ma64, mq,
                            ps64, pa64, polar_mm, part);
     HIP_CHECK(hipGetLastError());
     if (eve) HIP_CHECK(hipEventRecord(eve, st));

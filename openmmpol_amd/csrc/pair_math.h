@@ -149,4 +149,51 @@ __device__ __forceinline__ void pair_accum(R (&acc)[M][3], const typename V4<R>:
     }
 }
 
+// ----------------------------------------------------------- static field
+// Field at polarizable target j of MM source i (charge, dipole, quadrupole),
+// damped kernel with maxder 3 (AMOEBA) or 1 (Wang), as damped_coulomb_kernel +
+// q/mu/quad_elec_prop (mod_electrostatics.F90:1018-1405).  d = r_j - r_i.
+template <bool AMOEBA>
+__device__ __forceinline__ void m2d_pair(double dx, double dy, double dz, double ir, double ti,
+                                         double tj, double ai, double aj, const double *q,
+                                         double &ex, double &ey, double &ez) {
+    const double ir2 = ir * ir;
+    double k2 = ir * ir2;
+    const double r = (dx * dx + dy * dy + dz * dz) * ir;
+    const double u = r * ti * tj;
+    if (AMOEBA) {
+        double k3 = k2 * ir2, k4 = k3 * ir2;
+        const double a = fmin(ai, aj);
+        const double au3 = a * u * u * u;
+        const bool near = au3 < 50.0;
+        if (__ballot(near)) {
+            // au3 = +inf for a non-polarizable source: masked by the select
+            const double e = exp(-au3);
+            k2 = near ? k2 * (1.0 - e) : k2;
+            k3 = near ? k3 * (1.0 - (1.0 + au3) * e) : k3;
+            k4 = near ? k4 * (1.0 - (1.0 + au3 + 0.6 * au3 * au3) * e) : k4;
+        }
+        const double md = q[1] * dx + q[2] * dy + q[3] * dz;
+        // quadrupole storage xx, xy, yy, xz, yz, zz
+        const double qx = q[4] * dx + q[5] * dy + q[7] * dz;
+        const double qy = q[5] * dx + q[6] * dy + q[8] * dz;
+        const double qz = q[7] * dx + q[8] * dy + q[9] * dz;
+        const double qrr = qx * dx + qy * dy + qz * dz;
+        const double cd = q[0] * k2 + 3.0 * md * k3 + 15.0 * qrr * k4;
+        ex = cd * dx - q[1] * k2 - 6.0 * qx * k3;
+        ey = cd * dy - q[2] * k2 - 6.0 * qy * k3;
+        ez = cd * dz - q[3] * k2 - 6.0 * qz * k3;
+    } else {
+        const bool near = u < 1.0;
+        if (__ballot(near)) {
+            const double u3 = u * u * u;
+            k2 = near ? k2 * (4.0 * u3 - 3.0 * u3 * u) : k2;
+        }
+        ex = q[0] * k2 * dx;
+        ey = q[0] * k2 * dy;
+        ez = q[0] * k2 * dz;
+    }
+}
+
+
 }  // namespace ommp
