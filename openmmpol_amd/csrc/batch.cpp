@@ -78,6 +78,8 @@ BatchEngine::BatchEngine(hipStream_t st, bool amo, bool wide, int max_segs, int 
     BCHECK(hipMalloc((void **)&d_cols, sizeof(BCol) * std::max(max_cols, 1)));
     BCHECK(hipMemsetAsync(d_cols, 0, sizeof(BCol) * std::max(max_cols, 1), stream));
     BCHECK(hipMalloc((void **)&d_segs_, sizeof(BSeg) * std::max(max_segs, 1)));
+    BCHECK(hipMalloc((void **)&d_act_, sizeof(int) * std::max(max_cols, 1)));
+    BCHECK(hipMalloc((void **)&d_actn_, sizeof(int) * std::max(max_segs, 1)));
     for (int k = 0; k < 2; k++) {
         BCHECK(hipHostMalloc((void **)&snap_[k], sizeof(BCol) * std::max(max_cols, 1), hipHostMallocDefault));
         BCHECK(hipEventCreateWithFlags(&snap_ev_[k], hipEventDisableTiming));
@@ -85,6 +87,8 @@ BatchEngine::BatchEngine(hipStream_t st, bool amo, bool wide, int max_segs, int 
     BCHECK(hipEventCreateWithFlags(&stage_ev_, hipEventDisableTiming));
     tables_.segs = d_segs_;
     tables_.cols = d_cols;
+    tables_.act = d_act_;
+    tables_.act_n = d_actn_;
 }
 
 BatchEngine::~BatchEngine() {
@@ -92,6 +96,8 @@ BatchEngine::~BatchEngine() {
     for (void *p : owned_) (void)hipFree(p);
     (void)hipFree(d_cols);
     (void)hipFree(d_segs_);
+    (void)hipFree(d_act_);
+    (void)hipFree(d_actn_);
     int4 *lists[] = {d_pair32_, d_fin32_, d_pair64_, d_fin64_, d_vec_, d_sym32_, d_sym64_};
     for (auto p : lists)
         if (p) (void)hipFree(p);
@@ -345,6 +351,9 @@ void BatchEngine::commit() {
     tables_.nsym32 = (int)s32.size();
     tables_.sym64 = d_sym64_;
     tables_.nsym64 = (int)s64.size();
+    tables_.act = d_act_;
+    tables_.act_n = d_actn_;
+    tables_.nseg = (int)nseg;
     dirty_ = false;
 }
 

@@ -99,6 +99,10 @@ struct BSeg {
 //   pair-once items int4 {segment, I | J << 16, block pair index, column group}
 //   finalize items int4 {segment, target block, column group, 0}
 //   vector items   int4 {segment, column in segment, vector block, batch column}
+// In a wide batch the FP64 pair items' column group g is the g-th run of
+// kBatchM64 entries of the segment's active-column list (built by a small
+// kernel before those launches), so that the columns still refining share as
+// few groups as possible; otherwise group g is columns [g M, g M + M).
 struct BatchTables {
     const BSeg *segs;
     BCol *cols;
@@ -116,6 +120,12 @@ struct BatchTables {
     int nsym64;
     const int4 *vec;
     int nvec;
+    // the active columns of each segment, in increasing order, when the FP64
+    // pair kernels of a wide batch pack their column groups:
+    // act[col0 + k] = k-th active column of the segment, act_n[segment] = count
+    int *act;
+    int *act_n;
+    int nseg;
 };
 
 constexpr int kBatchM32 = 8;  // columns per FP32 finalize work item (multi-RHS)

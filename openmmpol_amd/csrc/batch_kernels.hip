@@ -133,17 +133,20 @@ __device__ __forceinline__ void bpairs_body(const BSeg &S, const int4 w, const d
 // (the same per-pair arithmetic, so results do not depend on the other column).
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kBlock) void k_bpairs(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
-                                                   const int4 *__restrict__ work, int use_p) {
+                                                   const int4 *__restrict__ work, int use_p,
+                                                   const int *__restrict__ act, const int *__restrict__ act_n) {
     const int4 w = work[blockIdx.x];
     const BSeg &S = segs[w.x];
     const int c0 = w.w * M;
     const double *xin = use_p ? S.p : S.x;
     int cidx[M];
     int nact = 0;
+    const int na = act ? act_n[w.x] : 0;
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        const bool a = c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
-        cidx[m] = a ? c0 + m : -1;
+        const bool a = act ? c0 + m < na
+                           : c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
+        cidx[m] = a ? (act ? act[S.col0 + c0 + m] : c0 + m) : -1;
         nact += a;
     }
     if (nact == 0) return;
@@ -184,7 +187,8 @@ __device__ __forceinline__ void bsym_body(const BSeg &S, const int4 w, const dou
 
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kSymBlock) void k_bsym(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
-                                                    const int4 *__restrict__ work, int use_p) {
+                                                    const int4 *__restrict__ work, int use_p,
+                                                    const int *__restrict__ act, const int *__restrict__ act_n) {
     // columns per sweep: FP64 sweeps take up to four (the pair's tensor, the
     // costly part in FP64, is then built once for all of them), FP32 two
     constexpr int kPair = sizeof(R) == 8 ? (M < 4 ? M : 4) : (M < 2 ? M : 2);
@@ -195,11 +199,13 @@ __global__ __launch_bounds__(kSymBlock) void k_bsym(const BSeg *__restrict__ seg
     const double *xin = use_p ? S.p : S.x;
     int idx[M];
     int k = 0;
+    const int na = act ? act_n[w.x] : 0;
 #pragma unroll
     for (int m = 0; m < M; m++) {
-        const bool a = c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
+        const bool a = act ? c0 + m < na
+                           : c0 + m < S.ncols && mv_active(cols[S.col0 + c0 + m], use_p);
         idx[m] = -1;
-        if (a) idx[k++] = c0 + m;
+        if (a) idx[k++] = act ? act[S.col0 + c0 + m] : c0 + m;
     }
     if (k == 0) return;  // uniform across the workgroup
     if constexpr (kPair >= 4) {
@@ -599,29 +605,62 @@ __global__ __launch_bounds__(kBlock) void k_bfinal(const BSeg *__restrict__ segs
 
 // ------------------------------------------------------------ vector kernels
 // Work item {segment, column, vector block, batch column}; element k of the
-// column belongs to site k / 3.
+// column belongs to site k / 3.  The residual and update kernels that apply
+// the preconditioner take one site per thread instead (work item z < tblocks:
+// sites [256 z, 256 z + 256); the other items of the column write zero
+// partials), so that each neighbour's three components are read once.
 //
 // With the near-field preconditioner (segment lists present, the column not
 // switched off): z = M^-1 r, M^-1 = 2 alpha - alpha T_near alpha (precond.hip),
-// element k summing row k % 3 of B_ij r_j over the neighbours j of its site,
-// the neighbours' r formed by the same expression as their own elements form
-// it.  Convergence is measured on r.(alpha r) as the reference does
-// (mod_solvers.F90:188-199), beta on r.z.  Without it z = alpha r and the two
-// partial sums coincide.
+// summing B_ij r_j over the neighbours j of the site, the neighbours' r formed
+// by the same expression as their own sites form it.  Convergence is measured
+// on r.(alpha r) as the reference does (mod_solvers.F90:188-199), beta on r.z.
+// Without it z = alpha r and the two partial sums coincide.
 constexpr double kBatchNbDiag = 2.0;
 
-// sum_j (B_ij r_j)_c over the neighbours of site i; r_j = rj(oj) with oj the
-// offset of site j's x component in this column
+// acc = sum_j B_ij r_j over the neighbours of site i; r_j = rj(oj + c) with oj
+// the offset of site j's x component in this column
+// (neighbours taken kNbChunk at a time, their loads issued together; the sum
+// keeps the list order)
+constexpr int kNbChunk = 4;
 template <typename RF>
-__device__ __forceinline__ double nb_row(const BSeg &S, int i, int c, long colbase, RF rj) {
-    double acc = 0.0;
-    const int e0 = c == 0 ? 0 : (c == 1 ? 1 : 2), e1 = c == 0 ? 1 : (c == 1 ? 3 : 4), e2 = c == 0 ? 2 : (c == 1 ? 4 : 5);
-    for (int q = S.nb_row_ptr[i]; q < S.nb_row_ptr[i + 1]; q++) {
-        const long oj = colbase + 3L * S.nb_ci[q];
-        const float *b = S.nb_b + 6L * q;
-        acc += (double)b[e0] * rj(oj) + (double)b[e1] * rj(oj + 1) + (double)b[e2] * rj(oj + 2);
+__device__ __forceinline__ void nb_site(const BSeg &S, int i, long colbase, RF rj, double (&acc)[3]) {
+    acc[0] = acc[1] = acc[2] = 0.0;
+    const int q0 = S.nb_row_ptr[i], q1 = S.nb_row_ptr[i + 1];
+    for (int qb = q0; qb < q1; qb += kNbChunk) {
+        int jn[kNbChunk];
+#pragma unroll
+        for (int u = 0; u < kNbChunk; u++) jn[u] = qb + u < q1 ? S.nb_ci[qb + u] : -1;
+        double r[kNbChunk][3];
+        float b[kNbChunk][6];
+#pragma unroll
+        for (int u = 0; u < kNbChunk; u++) {
+            if (jn[u] >= 0) {
+                const long oj = colbase + 3L * jn[u];
+#pragma unroll
+                for (int c = 0; c < 3; c++) r[u][c] = rj(oj + c);
+#pragma unroll
+                for (int e = 0; e < 6; e++) b[u][e] = S.nb_b[6L * (qb + u) + e];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kNbChunk; u++) {
+            if (jn[u] < 0) break;
+            const double bxx = b[u][0], bxy = b[u][1], bxz = b[u][2], byy = b[u][3], byz = b[u][4], bzz = b[u][5];
+            acc[0] += bxx * r[u][0] + bxy * r[u][1] + bxz * r[u][2];
+            acc[1] += bxy * r[u][0] + byy * r[u][1] + byz * r[u][2];
+            acc[2] += bxz * r[u][0] + byz * r[u][1] + bzz * r[u][2];
+        }
     }
-    return acc;
+}
+
+// r.z and r.(alpha r) partials of a site-block item (zero for the items past
+// the last site block)
+__device__ __forceinline__ void site_partials(const BSeg &S, const int4 w, double vz, double va, double *sh) {
+    vz = block_sum_all(vz, sh);
+    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = vz;
+    va = block_sum_all(va, sh);
+    if (threadIdx.x == 0) S.red_ra[(long)w.y * S.vblocks + w.z] = va;
 }
 
 // Outer step, part 1: r = b - T x (x = 0: r = b; known: r kept), z = M^-1 r,
@@ -633,31 +672,40 @@ __global__ __launch_bounds__(kBlock) void k_bresid(const BSeg *__restrict__ segs
     const BCol &C = cols[w.w];
     if (C.s[0].phase != kPhaseOuter) return;
     const BSeg &S = segs[w.x];
+    if (w.z >= S.tblocks) {
+        if (threadIdx.x == 0) {
+            S.red_rz[(long)w.y * S.vblocks + w.z] = 0.0;
+            S.red_ra[(long)w.y * S.vblocks + w.z] = 0.0;
+        }
+        return;
+    }
     const long n = 3L * S.npol;
-    const long k = (long)w.z * kBlock + threadIdx.x;
-    const long colbase = (long)w.y * n, o = colbase + k;
+    const int i = w.z * kBlock + threadIdx.x;
+    const long colbase = (long)w.y * n;
     const int rinit = C.rinit;
     const bool nb = S.nb_row_ptr != nullptr && !C.nb_off;
     auto rule = [&](long q) {
         return rinit == kResXZero ? S.b[q] : (rinit == kResKnown ? S.r[q] : S.b[q] - S.h[q]);
     };
     double vz = 0.0, va = 0.0;
-    if (k < n) {
-        const double rr = rule(o);
-        const int i = (int)(k / 3), c = (int)(k - 3L * i);
+    if (i < S.npol) {
+        const long o = colbase + 3L * i;
         const double al = S.alpha[i];
-        double zz = al * rr;
-        if (nb) zz = kBatchNbDiag * zz + nb_row(S, i, c, colbase, rule);
-        // a known residual is already in r (and its neighbours read it)
-        if (rinit != kResKnown) S.r[o] = rr;
-        S.z[o] = zz;
-        vz = rr * zz;
-        va = rr * (al * rr);
+        double rr[3], zz[3], acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 3; c++) rr[c] = rule(o + c);
+        if (nb) nb_site(S, i, colbase, rule, acc);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            zz[c] = nb ? kBatchNbDiag * (al * rr[c]) + acc[c] : al * rr[c];
+            // a known residual is already in r (and its neighbours read it)
+            if (rinit != kResKnown) S.r[o + c] = rr[c];
+            S.z[o + c] = zz[c];
+            vz += rr[c] * zz[c];
+            va += rr[c] * (al * rr[c]);
+        }
     }
-    vz = block_sum_all(vz, sh);
-    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = vz;
-    va = block_sum_all(va, sh);
-    if (threadIdx.x == 0) S.red_ra[(long)w.y * S.vblocks + w.z] = va;
+    site_partials(S, w, vz, va, sh);
 }
 
 // Outer step, part 2 (state copy 0 -> 1): rms test; if not converged, start an
@@ -708,6 +756,13 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(const BSeg *__restrict__ seg
     const BCol &C = cols[w.w];
     if (C.s[1].phase != kPhaseInner) return;
     const BSeg &S = segs[w.x];
+    if (w.z >= S.tblocks) {
+        if (threadIdx.x == 0) {
+            S.red_rz[(long)w.y * S.vblocks + w.z] = 0.0;
+            S.red_ra[(long)w.y * S.vblocks + w.z] = 0.0;
+        }
+        return;
+    }
     const long n = 3L * S.npol;
     const double gama = reduce_parts(S.red_hp + (long)w.y * S.tblocks, S.tblocks, sh);
     if (gama == 0.0) return;  // the direction kernel ends this inner solve
@@ -716,25 +771,30 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(const BSeg *__restrict__ seg
     const double *rold = odd ? S.r2 : S.r;
     double *rnew = odd ? S.r : S.r2;
     const bool nb = S.nb_row_ptr != nullptr && !C.nb_off;
-    const long k = (long)w.z * kBlock + threadIdx.x;
-    const long colbase = (long)w.y * n, o = colbase + k;
+    const int i = w.z * kBlock + threadIdx.x;
+    const long colbase = (long)w.y * n;
     double vz = 0.0, va = 0.0;
-    if (k < n) {
-        S.d[o] += a * S.p[o];
-        const double rr = fma(-a, S.h[o], rold[o]);
-        const int i = (int)(k / 3), c = (int)(k - 3L * i);
+    if (i < S.npol) {
+        const long o = colbase + 3L * i;
         const double al = S.alpha[i];
-        double zz = al * rr;
-        if (nb) zz = kBatchNbDiag * zz + nb_row(S, i, c, colbase, [&](long q) { return fma(-a, S.h[q], rold[q]); });
-        rnew[o] = rr;
-        S.z[o] = zz;
-        vz = rr * zz;
-        va = rr * (al * rr);
+        auto rj = [&](long q) { return fma(-a, S.h[q], rold[q]); };
+        double rr[3], acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            S.d[o + c] += a * S.p[o + c];
+            rr[c] = rj(o + c);
+        }
+        if (nb) nb_site(S, i, colbase, rj, acc);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double zz = nb ? kBatchNbDiag * (al * rr[c]) + acc[c] : al * rr[c];
+            rnew[o + c] = rr[c];
+            S.z[o + c] = zz;
+            vz += rr[c] * zz;
+            va += rr[c] * (al * rr[c]);
+        }
     }
-    vz = block_sum_all(vz, sh);
-    if (threadIdx.x == 0) S.red_rz[(long)w.y * S.vblocks + w.z] = vz;
-    va = block_sum_all(va, sh);
-    if (threadIdx.x == 0) S.red_ra[(long)w.y * S.vblocks + w.z] = va;
+    site_partials(S, w, vz, va, sh);
 }
 
 // Inner PCG, direction (state copy 1 -> 0): inner convergence ends the
@@ -779,14 +839,46 @@ __global__ __launch_bounds__(kBlock) void k_bdirection(const BSeg *__restrict__ 
     }
 }
 
+// The active columns of each segment in increasing order (one workgroup per
+// segment): act[col0 + k] = k-th column with mv_active, act_n[segment] = count.
+__global__ __launch_bounds__(kBlock) void k_bcompact(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
+                                                     int use_p, int *__restrict__ act, int *__restrict__ act_n) {
+    __shared__ int s_cnt[kBlock / 64];
+    const BSeg &S = segs[blockIdx.x];
+    const int nc = S.ncols, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < nc; c0 += kBlock) {
+        const int c = c0 + threadIdx.x;
+        const bool a = c < nc && mv_active(cols[S.col0 + c], use_p);
+        const unsigned long long m = __ballot(a);
+        if (lane == 0) s_cnt[wv] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int q = 0; q < wv; q++) off += s_cnt[q];
+        if (a) act[S.col0 + off + __popcll(m & ((1ull << lane) - 1ull))] = c;
+        for (int q = 0; q < kBlock / 64; q++) base += s_cnt[q];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) act_n[blockIdx.x] = base;
+}
+
+// pack: column groups from the active-column lists (wide batches, FP64)
 template <typename R, int M, bool AMOEBA>
 static void mv_launch(hipStream_t st, const BatchTables &t, const int4 *pair, int npair, const int4 *sym, int nsym,
-                      const int4 *fin, int nfin, int use_p) {
+                      const int4 *fin, int nfin, int use_p, bool pack = false) {
     if (npair == 0 && nsym == 0) return;
+    const int *act = nullptr, *act_n = nullptr;
+    if (pack && t.nseg > 0) {
+        hipLaunchKernelGGL(k_bcompact, dim3(t.nseg), dim3(kBlock), 0, st, t.segs, t.cols, use_p, t.act, t.act_n);
+        act = t.act;
+        act_n = t.act_n;
+    }
     if (npair)
-        hipLaunchKernelGGL((k_bpairs<R, M, AMOEBA>), dim3(npair), dim3(kBlock), 0, st, t.segs, t.cols, pair, use_p);
+        hipLaunchKernelGGL((k_bpairs<R, M, AMOEBA>), dim3(npair), dim3(kBlock), 0, st, t.segs, t.cols, pair, use_p,
+                           act, act_n);
     if (nsym)
-        hipLaunchKernelGGL((k_bsym<R, M, AMOEBA>), dim3(nsym), dim3(kSymBlock), 0, st, t.segs, t.cols, sym, use_p);
+        hipLaunchKernelGGL((k_bsym<R, M, AMOEBA>), dim3(nsym), dim3(kSymBlock), 0, st, t.segs, t.cols, sym, use_p,
+                           act, act_n);
     hipLaunchKernelGGL((k_bfinal<R, M, AMOEBA>), dim3(nfin), dim3(kBlock), 0, st, t.segs, t.cols, fin, use_p);
 }
 
@@ -794,7 +886,8 @@ template <bool AMOEBA>
 static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, double eta, double frac) {
     if (m64 == 2) mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0);
     else
-        mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0);
+        mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0,
+                                             true);
     hipLaunchKernelGGL(k_bresid, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec);
     hipLaunchKernelGGL(k_bdecide, dim3(t.nvec), dim3(kBlock), 0, st, t.segs, t.cols, t.vec, eta, frac);
     if (m32 == 2) {
