@@ -143,7 +143,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_nb(int npol, const double *_
 
 // conjugate_gradient_solver loop body, first half, with the near-field
 // preconditioner: x += a p, r_new = r_old - a h, z = M^-1 r_new; partials of
-// r.z (for beta) and r.(alpha r) (the reference's convergence measure).
+// r.z (for beta) and r.(alpha r) (the reference's convergence measure).  The
+// loads that do not depend on a (the column's state, the site's own vectors,
+// its first neighbours' h, r_old and B) are issued before the reduction of p.Tp,
+// so that their latency overlaps it; the arithmetic and its order are those of
+// nb_quad_sum.
+constexpr int kNbPre = 3;  // neighbour entries per lane loaded ahead of the reduction
 __global__ __launch_bounds__(kBlock) void k_cg_update_nb(int npol, int it, const double *__restrict__ h,
                                                          double *__restrict__ x, const double *__restrict__ rold,
                                                          double *__restrict__ rnew, double *__restrict__ z,
@@ -155,14 +160,43 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_nb(int npol, int it, const
                                                          const int *__restrict__ nb_ci,
                                                          const float *__restrict__ nb_b) {
     __shared__ double sh[4];
-    __shared__ int s_done, s_off;
     const int m = blockIdx.y;
-    if (threadIdx.x == 0) {
-        s_done = cs[m].done;
-        s_off = cs[m].pad;
+    if (cs[m].done) return;  // uniform over the workgroup
+    const int off = cs[m].pad;
+    const long n = 3L * npol, base = m * n;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int i = t >> 2, q = t & 3;
+    const bool site = i < npol, nbr = site && !off;
+    const int k0 = nbr ? row_ptr[i] + q : 0, k1 = nbr ? row_ptr[i + 1] : 0;
+    double hj[kNbPre][3], rj[kNbPre][3];
+    float bj[kNbPre][6];
+#pragma unroll
+    for (int u = 0; u < kNbPre; u++) {
+        const int k = k0 + 4 * u;
+        if (k < k1) {
+            const long o = base + 3L * nb_ci[k];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                hj[u][c] = h[o + c];
+                rj[u][c] = rold[o + c];
+            }
+#pragma unroll
+            for (int e = 0; e < 6; e++) bj[u][e] = nb_b[6L * k + e];
+        }
     }
-    __syncthreads();
-    if (s_done) return;
+    const long o = base + 3L * i;
+    double xo[3] = {0.0, 0.0, 0.0}, po[3] = {0.0, 0.0, 0.0}, ho[3] = {0.0, 0.0, 0.0}, ro[3] = {0.0, 0.0, 0.0};
+    double al = 0.0;
+    if (site && q == 0) {
+        al = alpha[i];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            xo[c] = x[o + c];
+            po[c] = p[o + c];
+            ho[c] = h[o + c];
+            ro[c] = rold[o + c];
+        }
+    }
     const double gama = reduce_parts(red_hp + m * nred_hp, nred_hp, sh);
     if (fabs(gama) < cs[m].eps || gama == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -173,19 +207,44 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_nb(int npol, int it, const
         return;
     }
     const double a = cs[m].g[(it + 1) & 1] / gama;
-    const long n = 3L * npol, base = m * n;
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int i = t >> 2, q = t & 3;
     double zz[3] = {0.0, 0.0, 0.0};
-    if (i < npol && !s_off) nb_quad_sum(i, q, base, a, rold, h, row_ptr, nb_ci, nb_b, zz);
-    double vz = 0.0, va = 0.0;
-    if (i < npol && q == 0) {
-        const long o = base + 3L * i;
-        const double al = alpha[i];
-        const double ci = s_off ? al : kNbDiag * al;
+    if (nbr) {
+        int k = k0;
+#pragma unroll
+        for (int u = 0; u < kNbPre; u++, k += 4) {
+            if (k >= k1) break;
+            const double r0 = fma(-a, hj[u][0], rj[u][0]);
+            const double r1 = fma(-a, hj[u][1], rj[u][1]);
+            const double r2 = fma(-a, hj[u][2], rj[u][2]);
+            const double bxx = bj[u][0], bxy = bj[u][1], bxz = bj[u][2], byy = bj[u][3], byz = bj[u][4],
+                         bzz = bj[u][5];
+            zz[0] += bxx * r0 + bxy * r1 + bxz * r2;
+            zz[1] += bxy * r0 + byy * r1 + byz * r2;
+            zz[2] += bxz * r0 + byz * r1 + bzz * r2;
+        }
+        for (; k < k1; k += 4) {
+            const long oj = base + 3L * nb_ci[k];
+            const double r0 = fma(-a, h[oj], rold[oj]);
+            const double r1 = fma(-a, h[oj + 1], rold[oj + 1]);
+            const double r2 = fma(-a, h[oj + 2], rold[oj + 2]);
+            const float *b = nb_b + 6L * k;
+            const double bxx = b[0], bxy = b[1], bxz = b[2], byy = b[3], byz = b[4], bzz = b[5];
+            zz[0] += bxx * r0 + bxy * r1 + bxz * r2;
+            zz[1] += bxy * r0 + byy * r1 + byz * r2;
+            zz[2] += bxz * r0 + byz * r1 + bzz * r2;
+        }
+#pragma unroll
         for (int c = 0; c < 3; c++) {
-            x[o + c] += a * p[o + c];
-            const double rn = fma(-a, h[o + c], rold[o + c]);
+            zz[c] += __shfl_xor(zz[c], 1, 64);
+            zz[c] += __shfl_xor(zz[c], 2, 64);
+        }
+    }
+    double vz = 0.0, va = 0.0;
+    if (site && q == 0) {
+        const double ci = off ? al : kNbDiag * al;
+        for (int c = 0; c < 3; c++) {
+            x[o + c] = xo[c] + a * po[c];
+            const double rn = fma(-a, ho[c], ro[c]);
             const double zc = ci * rn + zz[c];
             rnew[o + c] = rn;
             z[o + c] = zc;
@@ -202,17 +261,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_nb(int npol, int it, const
 // Second half: convergence on sqrt(r.(alpha r) / n), as the reference measures
 // it (mod_solvers.F90:188-199), beta = r.z / r.z_old; r.z <= 0 means the
 // preconditioner is not positive definite here: the column stops (its progress
-// so far is kept) and nb_off switches it off until the next geometry.
+// so far is kept) and nb_off switches it off until the next geometry.  z and p
+// are loaded before the reductions.
 __global__ __launch_bounds__(kBlock) void k_cg_direction_nb(long n, int it, const double *__restrict__ z,
                                                             double *__restrict__ p, const double *__restrict__ red_rz,
                                                             const double *__restrict__ red_ra, int nred, CgState *cs,
                                                             int *__restrict__ nb_off) {
     __shared__ double sh[4];
-    __shared__ int s_done;
     const int m = blockIdx.y;
-    if (threadIdx.x == 0) s_done = cs[m].done;
-    __syncthreads();
-    if (s_done) return;
+    if (cs[m].done) return;  // uniform over the workgroup
+    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+    double zk = 0.0, pk = 0.0;
+    if (k < n) {
+        zk = z[m * n + k];
+        pk = p[m * n + k];
+    }
     const double gnew = reduce_parts(red_rz + m * nred, nred, sh);
     const double ra = reduce_parts(red_ra + m * nred, nred, sh);
     const double rms = sqrt(ra / (double)n);
@@ -227,8 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction_nb(long n, int it, cons
         return;
     }
     const double beta = gnew / cs[m].g[(it + 1) & 1];
-    const long k = (long)blockIdx.x * kBlock + threadIdx.x;
-    if (k < n) p[m * n + k] = beta * p[m * n + k] + z[m * n + k];
+    if (k < n) p[m * n + k] = beta * pk + zk;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         cs[m].g[it & 1] = gnew;
         cs[m].rms = rms;
