@@ -160,8 +160,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_nb(int npol, int it, const
                                                          const int *__restrict__ nb_ci,
                                                          const float *__restrict__ nb_b) {
     __shared__ double sh[4];
+    __shared__ int s_done;
     const int m = blockIdx.y;
-    if (cs[m].done) return;  // uniform over the workgroup
+    // done may be set by block 0 of this launch (small-step exit): one thread
+    // reads it for the whole workgroup; pad is fixed for the inner solve
+    if (threadIdx.x == 0) s_done = cs[m].done;
     const int off = cs[m].pad;
     const long n = 3L * npol, base = m * n;
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -197,6 +200,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_nb(int npol, int it, const
             ro[c] = rold[o + c];
         }
     }
+    __syncthreads();
+    if (s_done) return;  // uniform over the workgroup
     const double gama = reduce_parts(red_hp + m * nred_hp, nred_hp, sh);
     if (fabs(gama) < cs[m].eps || gama == 0.0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -268,14 +273,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_direction_nb(long n, int it, cons
                                                             const double *__restrict__ red_ra, int nred, CgState *cs,
                                                             int *__restrict__ nb_off) {
     __shared__ double sh[4];
+    __shared__ int s_done;
     const int m = blockIdx.y;
-    if (cs[m].done) return;  // uniform over the workgroup
+    // done may be set by block 0 of this launch: one thread reads it
+    if (threadIdx.x == 0) s_done = cs[m].done;
     const long k = (long)blockIdx.x * kBlock + threadIdx.x;
     double zk = 0.0, pk = 0.0;
     if (k < n) {
         zk = z[m * n + k];
         pk = p[m * n + k];
     }
+    __syncthreads();
+    if (s_done) return;  // uniform over the workgroup
     const double gnew = reduce_parts(red_rz + m * nred, nred, sh);
     const double ra = reduce_parts(red_ra + m * nred, nred, sh);
     const double rms = sqrt(ra / (double)n);
