@@ -51,7 +51,17 @@ __device__ __forceinline__ double fast_exp(double x) { return exp(x); }
 // r2 is either 0 (self pair, masked by the caller) or >= ~1 bohr^2, never
 // denormal, so the raw v_rsq_f32 (1 ulp) needs no range fix-up.
 __device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ double fast_rsqrt(double x) { return rsqrt(x); }
+// FP64: the hardware estimate y of x^-1/2 refined by one second-order step of
+// the binomial series, x^-1/2 = y (1 - e)^-1/2 = y (1 + e/2 + 3 e^2/8 + ...)
+// with e = 1 - x y^2 (the estimate's error cubed remains).  For a normal,
+// positive, finite x only: the squared distance of a pair, which the callers
+// keep away from zero, infinity and the denormals, so the special-value
+// handling of a general rsqrt is left out.
+__device__ __forceinline__ double fast_rsqrt(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
 __device__ __forceinline__ float fmaa(float a, float b, float c) { return fmaf(a, b, c); }
 __device__ __forceinline__ double fmaa(double a, double b, double c) { return fma(a, b, c); }
 

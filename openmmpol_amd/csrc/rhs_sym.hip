@@ -42,7 +42,7 @@ __device__ __forceinline__ void rhs_geom(const double4 &pt, double at, const dou
     dy = pt.y - ps.y;
     dz = pt.z - ps.z;
     const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-    const double ir = rsqrt(r2);
+    const double ir = fast_rsqrt(r2);
     const double ir2 = ir * ir;
     k2 = ir * ir2;
     if (AMOEBA) {
