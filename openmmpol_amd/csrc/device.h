@@ -165,6 +165,10 @@ struct GradArgs {
     double *part_pol = nullptr, *part_mm = nullptr;
     int split_s = 1, split_p = 1, split_m = 1;
     double *gpol = nullptr, *gtr = nullptr, *emm = nullptr, *egrd = nullptr, *rotc = nullptr, *out = nullptr;
+    // set: the dipole-dipole part runs pair-once over these block pairs of
+    // polarizable sites, its block-pair partials in sympart
+    const int2 *sym_pairs = nullptr;
+    double *sympart = nullptr;
 };
 // out[nmm][3] = d E_pol / d r (pair sweeps + rotation term); emm/egrd receive
 // E_D2M and Egrd_D2M.  evb/eve bracket the three pair sweeps.

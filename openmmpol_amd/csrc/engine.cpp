@@ -1622,6 +1622,11 @@ void polelec_geomgrad_host(System &s, double *grd) {
         if (eve) HCHECK(hipEventRecord(eve, d.stream));
         launch_geomgrad_finish(d.stream, a, fmm_grad_parts_pol(s.amoeba), fmm_grad_parts_mm());
     } else {
+        if (rhs_pair_once(s.npol)) {  // the dipole-dipole part pair-once, as the static field
+            ensure_sym(s, d);
+            a.sym_pairs = d.sym_pairs;
+            a.sympart = static_cast<double *>(d.sympart);
+        }
         launch_geomgrad(d.stream, a, evb, eve);
     }
     HCHECK(hipMemcpyAsync(grd, d.g_out, sizeof(double) * 3L * s.nmm, hipMemcpyDeviceToHost, d.stream));

@@ -32,6 +32,8 @@
 
 #include "device.h"
 #include "grad_math.h"
+#include "pair_math.h"
+#include "tmv_sym.h"
 
 namespace ommp {
 
@@ -163,6 +165,164 @@ __global__ __launch_bounds__(kBlock) void k_gpol_dd(int npol, int chunk, const d
         o[j] = gx;
         o[npol + j] = gy;
         o[2L * npol + j] = gz;
+    }
+}
+
+// The same dipole-dipole part with each unordered pair evaluated once, from
+// 5,000 polarizable sites: the pair's term is odd in d and symmetric in the two
+// dipoles it contracts, so the term at i equals minus the term at j; it is
+// formed once and added to j (target side) and subtracted from i (source
+// side), over the block pairs of the pair-once matvec (tmv_sym.h: sources of J
+// in LDS, source-side sums carried by the DPP wave rotate).  Target-side sums
+// of component c go to part_i[p][c][t], source-side ones to part_j[p][c][t].
+// Padding sites sit far away with zero dipoles, which add exactly 0.
+template <bool AMOEBA>
+__global__ __launch_bounds__(kSymBlock) void k_gpol_dd_sym(int npol, const int2 *__restrict__ pairs,
+                                                           const double4 *__restrict__ ps,
+                                                           const double *__restrict__ pa,
+                                                           const double *__restrict__ ipd,
+                                                           double *__restrict__ part_i, double *__restrict__ part_j) {
+    constexpr int NS = AMOEBA ? 6 : 3;  // mu_P then mu_D (AMOEBA), mu (AMBER)
+    __shared__ double4 s_pos[kSymBlock];
+    __shared__ double s_a[kSymBlock];
+    __shared__ double s_m[NS][kSymBlock];
+    __shared__ double s_acc[kSymWaves][3][64];
+    const int2 pr = pairs[blockIdx.x];
+    const int I = pr.x, J = pr.y;
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const long n = 3L * npol;
+    const double far = 1e15, gap = 1e9, f = AMOEBA ? 0.5 : 1.0;
+    {
+        const int jj = J * kSymBlock + tid;
+        if (jj < npol) {
+            s_pos[tid] = ps[jj];
+            s_a[tid] = pa[jj];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                s_m[c][tid] = AMOEBA ? ipd[n + 3L * jj + c] : ipd[3L * jj + c];
+                if (AMOEBA) s_m[3 + c][tid] = ipd[3L * jj + c];
+            }
+        } else {
+            s_pos[tid] = make_double4(far + gap * (double)tid, far + gap * (double)tid, far + gap * (double)tid, 1.0);
+            s_a[tid] = 0.39;
+#pragma unroll
+            for (int c = 0; c < NS; c++) s_m[c][tid] = 0.0;
+        }
+    }
+    // the thread's target: mu_D (a) and mu_P (b) for AMOEBA, mu (a) for AMBER
+    const int i = I * kSymBlock + tid;
+    double4 pt;
+    double at = 0.39, ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+    if (i < npol) {
+        pt = ps[i];
+        at = pa[i];
+        ax = ipd[3L * i];
+        ay = ipd[3L * i + 1];
+        az = ipd[3L * i + 2];
+        if (AMOEBA) {
+            bx = ipd[n + 3L * i];
+            by = ipd[n + 3L * i + 1];
+            bz = ipd[n + 3L * i + 2];
+        }
+    } else {
+        pt = make_double4(far + gap * (double)tid, far + gap * (double)tid, far + gap * (double)tid, 1.0);
+    }
+    // term of the pair at the target (d = r_t - r_s)
+    auto term = [&](int js, double &gx, double &gy, double &gz) {
+        const double4 psr = s_pos[js];
+        const double dx = pt.x - psr.x, dy = pt.y - psr.y, dz = pt.z - psr.z;
+        const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        double k2, k3, k4, k5;
+        damp_k5<AMOEBA>(r2, fast_rsqrt(r2), psr.w, pt.w, s_a[js], at, k2, k3, k4, k5);
+        gx = gy = gz = 0.0;
+        dip_grad_pair(dx, dy, dz, k3, k4, ax, ay, az, s_m[0][js], s_m[1][js], s_m[2][js], f, gx, gy, gz);
+        if (AMOEBA)
+            dip_grad_pair(dx, dy, dz, k3, k4, bx, by, bz, s_m[3][js], s_m[4][js], s_m[5][js], f, gx, gy, gz);
+    };
+    double acc[3] = {0.0, 0.0, 0.0};
+    __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < kSymWaves; k++) {
+        const bool mirror = I == J && k < w;  // wave-uniform
+        const bool diag = I == J && k == w;
+        double jacc[3] = {0.0, 0.0, 0.0};
+        if (diag) {
+            // one-sided over the other 63 sites of the sub-tile
+            for (int st = 1; st < 64; st++) {
+                double gx, gy, gz;
+                term(k * 64 + ((l + st) & 63), gx, gy, gz);
+                acc[0] += gx;
+                acc[1] += gy;
+                acc[2] += gz;
+            }
+        } else if (!mirror) {
+            for (int st = 0; st < 64; st++) {
+                double gx, gy, gz;
+                term(k * 64 + ((l + st) & 63), gx, gy, gz);
+                acc[0] += gx;
+                acc[1] += gy;
+                acc[2] += gz;
+                jacc[0] -= gx;
+                jacc[1] -= gy;
+                jacc[2] -= gz;
+#pragma unroll
+                for (int c = 0; c < 3; c++) jacc[c] = lane_next(jacc[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) s_acc[w][c][l] = jacc[c];
+        __syncthreads();
+        if (tid < 3 * 64) {
+            const int c = tid >> 6, jl = tid & 63;
+            double sum = s_acc[0][c][jl];
+#pragma unroll
+            for (int q = 1; q < kSymWaves; q++) sum += s_acc[q][c][jl];
+            part_j[((long)blockIdx.x * 3 + c) * kSymBlock + k * 64 + jl] = sum;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) part_i[((long)blockIdx.x * 3 + c) * kSymBlock + tid] = acc[c];
+}
+
+// Sum of a site's row (pairs (b, J >= b): target sides) and column (pairs
+// (I <= b, b): source sides) block-pair partials into out[c * npol + j], in a
+// fixed order: kQ groups of kSymBlock threads per site block, group q taking
+// every kQ-th partial, the groups added in order.
+__global__ __launch_bounds__(4 * kSymBlock) void k_sym3_reduce(int npol, int nb, const double *__restrict__ part,
+                                                               double *__restrict__ out) {
+    constexpr int kQ = 4;
+    __shared__ double sh[kQ - 1][3][kSymBlock];
+    const int q = threadIdx.x / kSymBlock, li = threadIdx.x % kSymBlock;
+    const int b = blockIdx.x, j = b * kSymBlock + li;
+    const long np = (long)nb * (nb + 1) / 2, stride = 3L * kSymBlock;
+    double sum[3] = {0.0, 0.0, 0.0};
+    if (j < npol) {
+        const long row = (long)b * nb - (long)b * (b - 1) / 2;
+        for (int J = b + q; J < nb; J += kQ) {
+            const double *p = part + (row + J - b) * stride + li;
+#pragma unroll
+            for (int c = 0; c < 3; c++) sum[c] += p[(long)c * kSymBlock];
+        }
+        const double *part_j = part + np * stride;
+        for (int I = q; I <= b; I += kQ) {
+            const double *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + li;
+#pragma unroll
+            for (int c = 0; c < 3; c++) sum[c] += p[(long)c * kSymBlock];
+        }
+    }
+    if (q > 0)
+#pragma unroll
+        for (int c = 0; c < 3; c++) sh[q - 1][c][li] = sum[c];
+    __syncthreads();
+    if (q != 0 || j >= npol) return;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        double v = sum[c];
+#pragma unroll
+        for (int g = 0; g < kQ - 1; g++) v += sh[g][c][li];
+        out[(long)c * npol + j] = v;
     }
 }
 
@@ -523,25 +683,39 @@ void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb, hipEvent
         const int cs = (nmm + a.split_s - 1) / a.split_s, cp = (npol + a.split_p - 1) / a.split_p;
         double *part_p = a.part_pol + (long)a.split_s * 3 * npol;
         const int cm = (npol + a.split_m - 1) / a.split_m;
+        // the dipole-dipole part pair-once into part_p[0] (one partial), or per target
+        const int nb = a.sym_pairs ? tmv_sym_blocks(npol) : 0;
+        const long np = (long)nb * (nb + 1) / 2;
+        double *sym_i = a.sympart, *sym_j = a.sym_pairs ? a.sympart + np * 3 * kSymBlock : nullptr;
         if (a.amoeba) {
             hipLaunchKernelGGL(k_gpol_static<true>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms,
                                a.ma, a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
-            hipLaunchKernelGGL(k_gpol_dd<true>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa,
-                               a.ipd, part_p);
+            if (a.sym_pairs)
+                hipLaunchKernelGGL(k_gpol_dd_sym<true>, dim3((unsigned)np), dim3(kSymBlock), 0, st, npol,
+                                   a.sym_pairs, a.ps, a.pa, a.ipd, sym_i, sym_j);
+            else
+                hipLaunchKernelGGL(k_gpol_dd<true>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa,
+                                   a.ipd, part_p);
             hipLaunchKernelGGL(k_gmm_pairs<true>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms,
                                a.ma, a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
         } else {
             hipLaunchKernelGGL(k_gpol_static<false>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms,
                                a.ma, a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
-            hipLaunchKernelGGL(k_gpol_dd<false>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa,
-                               a.ipd, part_p);
+            if (a.sym_pairs)
+                hipLaunchKernelGGL(k_gpol_dd_sym<false>, dim3((unsigned)np), dim3(kSymBlock), 0, st, npol,
+                                   a.sym_pairs, a.ps, a.pa, a.ipd, sym_i, sym_j);
+            else
+                hipLaunchKernelGGL(k_gpol_dd<false>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps,
+                                   a.pa, a.ipd, part_p);
             hipLaunchKernelGGL(k_gmm_pairs<false>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms,
                                a.ma, a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
         }
+        if (a.sym_pairs)
+            hipLaunchKernelGGL(k_sym3_reduce, dim3(nb), dim3(4 * kSymBlock), 0, st, npol, nb, a.sympart, part_p);
         HIP_CHECK(hipGetLastError());
     }
     if (eve) HIP_CHECK(hipEventRecord(eve, st));
-    launch_geomgrad_finish(st, a, a.split_s + a.split_p, a.split_m);
+    launch_geomgrad_finish(st, a, a.split_s + (a.sym_pairs ? 1 : a.split_p), a.split_m);
 }
 
 void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm) {

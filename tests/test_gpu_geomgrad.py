@@ -4,7 +4,9 @@ mod_geomgrad.F90:113-273; ommp_rotation_geomgrad, rotate_multipoles.F90:3-230).
 - against the oracle's restatement, fed the GPU's own dipoles so that the
   comparison isolates the gradient kernels: 1e-11 of the largest component
   (FP64 sums of the same terms in another order), reference fixtures (AMOEBA
-  and Wang-AL), a cluster with every frame convention, a 3,000-site sphere;
+  and Wang-AL), a cluster with every frame convention, a 3,000-site sphere,
+  7,001-site spheres (AMOEBA, Wang-AL) where the dipole-dipole part runs
+  pair-once;
 - against the 'Grad EP' listings of the reference's FULL_POTENTIAL.ref
   (kcal/mol/angstrom), as the oracle is;
 - at the 15,000-site benchmark size, through properties: the gradient sums to
@@ -120,6 +122,30 @@ def test_sphere3000_matches_oracle():
     g = s.polelec_geomgrad()
     want = _oracle_grad(synth.oracle_prep(spec), s.ipd.copy())
     assert np.abs(g - want).max() <= 1e-11 * np.abs(want).max()
+    s.terminate()
+
+
+@pytest.mark.parametrize("ff", ["AMOEBA", "WANG_AL"])
+def test_sphere7001_pair_once_matches_oracle(ff):
+    """From 5,000 polarizable sites the dipole-dipole part is evaluated once per
+    unordered pair (its term added at one site, subtracted at the other); 7,001
+    sites (27 x 256 + 89) leave a partial last block."""
+    from openmmpol_amd import synth
+    n = 7001
+    spec = dict(synth.sphere_system(n, (3.0 * n / (4.0 * np.pi * 0.0996)) ** (1.0 / 3.0), seed=24))
+    if ff == "WANG_AL":
+        spec.update(amoeba=False, q=spec["q"][:, :1].copy(), pgroup=None, scales=None)
+    s = synth.make_system(spec)
+    s.set_external_field(np.zeros((s.pol_atoms, 3)))
+    g = s.polelec_geomgrad()
+    prep = synth.oracle_prep(spec)
+    if ff == "WANG_AL":
+        prep["scales"] = mmp_io.SCALES_WANG_AL.copy()
+        prep["thole_scale"] = mmp_io.THOLE_SCALE_WANG_AL
+    want = _oracle_grad(prep, s.ipd.copy())
+    err = np.abs(g - want).max() / np.abs(want).max()
+    print(f"{ff} {n} sites: gradient err {err:.2e} of the largest component", flush=True)
+    assert err <= 1e-11
     s.terminate()
 
 
