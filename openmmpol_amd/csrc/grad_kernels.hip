@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_gpol_static(int nmm, int npol, int c
             const double4 pi = s_pos[k];
             const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            const double ir = (base + k) == jm ? 0.0 : rsqrt(r2);
+            const double ir = (base + k) == jm ? 0.0 : fast_rsqrt(r2);
             double k2, k3, k4, k5;
             damp_k5<AMOEBA>(r2, ir, pi.w, pj.w, s_a[k], aj, k2, k3, k4, k5);
             double q[LD];
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_gpol_dd(int npol, int chunk, const d
             const double4 pi = s_pos[k];
             const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
-            const double ir = (base + k) == j ? 0.0 : rsqrt(r2);
+            const double ir = (base + k) == j ? 0.0 : fast_rsqrt(r2);
             double k2, k3, k4, k5;
             damp_k5<AMOEBA>(r2, ir, pi.w, pj.w, s_a[k], aj, k2, k3, k4, k5);
             dip_grad_pair(dx, dy, dz, k3, k4, ax, ay, az, s_m[0][k], s_m[1][k], s_m[2][k], f, gx, gy, gz);
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_gmm_pairs(int nmm, int npol, int chu
             const double4 pi = s_pos[k];
             const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
             const double r2 = ex * ex + ey * ey + ez * ez;
-            const double ir = (base + k) == mp ? 0.0 : rsqrt(r2);
+            const double ir = (base + k) == mp ? 0.0 : fast_rsqrt(r2);
             double k2, k3, k4, k5;
             damp_k5<AMOEBA>(r2, ir, pi.w, pm.w, s_a[k], am, k2, k3, k4, k5);
             dip_at_mm<AMOEBA>(ex, ey, ez, k2, k3, k4, k5, s_n[0][k], s_n[1][k], s_n[2][k], Q, trQ, acc);

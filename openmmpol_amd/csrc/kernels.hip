@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_pairs(int nsrc, int npol, int ch
             const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             const double r2 = dx * dx + dy * dy + dz * dz;
             const bool self = (base + k) == jm;
-            const double ir = self ? 0.0 : rsqrt(r2);
+            const double ir = self ? 0.0 : fast_rsqrt(r2);
             double q[LD];
 #pragma unroll
             for (int c = 0; c < LD; c++) q[c] = s_q[c][k];

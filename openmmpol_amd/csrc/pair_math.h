@@ -54,9 +54,10 @@ __device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_r
 // FP64: the hardware estimate y of x^-1/2 refined by one second-order step of
 // the binomial series, x^-1/2 = y (1 - e)^-1/2 = y (1 + e/2 + 3 e^2/8 + ...)
 // with e = 1 - x y^2 (the estimate's error cubed remains).  For a normal,
-// positive, finite x only: the squared distance of a pair, which the callers
-// keep away from zero, infinity and the denormals, so the special-value
-// handling of a general rsqrt is left out.
+// positive, finite x only: the squared distance of a pair of distinct sites
+// (coincident sites are rejected when the coordinates are set; a self pair's
+// result is discarded by its caller), so the special-value handling of a
+// general rsqrt is left out.
 __device__ __forceinline__ double fast_rsqrt(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     const double e = fma(-x * y, y, 1.0);
