@@ -684,6 +684,12 @@ def main():
     # at 15,000 sites both pair kernels are the pair-once sweep k_tmv_sym
     flops = sym_pair_flops(n, 2)
     achieved = flops / (avg_ms * 1e-3) / 1e12 if calls else None
+    # the rate non-packed vector FMAs reach on this device (the build emits no
+    # packed FP32 instructions; the published FP32 peak counts them)
+    from openmmpol_amd._lib import lib as ommp_lib
+    vfma = {"fp32_tflops": ommp_lib().ommp_amd_vector_fma_tflops(32),
+            "fp64_tflops": ommp_lib().ommp_amd_vector_fma_tflops(64),
+            "method": "8 independent FMA chains per thread, 8192 x 256 threads, no packed FP32 instructions"}
     out = {
         "metric": METRIC,
         "value": value,
@@ -710,7 +716,9 @@ def main():
         "roofline": {"bound": "mfma", "engine": "valu",
                      "compute": "compute-bound (the 'mfma' category of the two this line allows), executed on "
                                 "the vector ALUs, not the matrix cores: rsqrt, exp and 3x3 tensor products per "
-                                "pair; priced against the published FP32 peak; FLOP = N(N-1)/2 (29 + 36m), the "
+                                "pair; priced against the published FP32 peak, which counts packed FP32 "
+                                "instructions that this build does not emit (vector_fma_measured gives the rate "
+                                "non-packed FMAs reach on this device); FLOP = N(N-1)/2 (29 + 36m), the "
                                 "pair-once count (each unordered pair once, m = 2 columns)",
                      "flop_formula": "N(N-1)/2*(29+36m)",
                      "kernel": "k_tmv_sym<float,2,AMOEBA> (on-the-fly FP32 pair kernel, each unordered pair "
@@ -719,7 +727,9 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS if achieved else None,
                      "traffic": profile_traffic("k_tmv_sym<float,2,AMOEBA>"),
                      "launches_timed": calls, "timed_every": PROF_EVERY, "avg_ms": avg_ms,
-                     "flop_per_launch": flops},
+                     "flop_per_launch": flops,
+                     "vector_fma_measured": vfma,
+                     "frac_of_measured_fp32_vector_fma": achieved / vfma["fp32_tflops"] if achieved else None},
         "kernel_time_ms_timed": {k: v[1] for k, v in mixed["stats"].items()},
         "kernel_launches_timed": {k: v[0] for k, v in mixed["stats"].items()},
     }
@@ -761,7 +771,10 @@ def main():
                                  "achieved_tflops": sym_pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12
                                  if c64 else None, "peak": PEAK_FP64_TFLOPS,
                                  "frac": sym_pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
-                                 if c64 else None},
+                                 if c64 else None,
+                                 "frac_of_measured_fp64_vector_fma":
+                                     sym_pair_flops(n, 1) / (ms64 / max(c64, 1) * 1e-3) / 1e12 / vfma["fp64_tflops"]
+                                     if c64 else None},
         })
         if "cpu" in legs:
             print("[bench] cpu baseline", file=sys.stderr, flush=True)

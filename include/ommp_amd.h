@@ -141,6 +141,13 @@ void ommp_amd_profile_kernels(OMMP_SYSTEM_PRT s, bool enable);
 void ommp_amd_profile_kernels_every(OMMP_SYSTEM_PRT s, int32_t every);
 void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset);
 
+/* No counterpart in the reference.  The FMA rate, in TFLOP/s (FMA = 2), that
+   non-packed vector code reaches on the current device: eight independent FMA
+   chains per thread over 8,192 x 256 threads, prec 32 (FP32) or 64 (FP64),
+   built like the library's kernels (without packed FP32 instructions).  The
+   reference point for the vector-ALU-bound pair kernels; about 0.1 s. */
+double ommp_amd_vector_fma_tflops(int32_t prec);
+
 /* Timing of the last solve on the device, in milliseconds: out[0] static field,
    out[1] solver, out[2] energy/copy-back.  out must hold 3 entries. */
 void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out);

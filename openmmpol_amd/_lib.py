@@ -72,6 +72,7 @@ SIGNATURES = [
     ("ommp_amd_get_thole", c_dp, [c_vp]),
     ("ommp_amd_get_pol", c_dp, [c_vp]),
     ("ommp_amd_get_last_timings", None, [c_vp, c_dp]),
+    ("ommp_amd_vector_fma_tflops", ctypes.c_double, [ctypes.c_int32]),
     ("ommp_amd_profile_kernels", None, [c_vp, ctypes.c_bool]),
     ("ommp_amd_profile_kernels_every", None, [c_vp, ctypes.c_int32]),
     ("ommp_amd_get_kernel_stats", None, [c_vp, c_dp, ctypes.c_bool]),

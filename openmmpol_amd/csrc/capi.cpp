@@ -272,6 +272,11 @@ void ommp_amd_get_kernel_stats(OMMP_SYSTEM_PRT s, double *stats, bool reset) {
     ommp::profile_stats(*sys(s), stats, reset);
 }
 
+double ommp_amd_vector_fma_tflops(int32_t prec) {
+    if (prec != 32 && prec != 64) ommp::fatal("vector_fma_tflops: precision must be 32 or 64");
+    return ommp::vector_fma_tflops(prec);
+}
+
 void ommp_amd_get_last_timings(OMMP_SYSTEM_PRT s, double *out) {
     for (int k = 0; k < 3; k++) out[k] = sys(s)->last_ms[k];
 }

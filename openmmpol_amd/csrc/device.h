@@ -117,6 +117,9 @@ void launch_rhs_finalize(hipStream_t st, bool amoeba, int npol, int nsplit, cons
 int tmv_split(int npol);
 int balanced_split(int tb, int nsrc, int num_cu);
 size_t tmv_part_bytes(int npol, int M, int prec, int nsplit);
+// FMA rate (TFLOP/s, FMA = 2) of non-packed vector code on the current device,
+// prec 32 or 64: eight independent chains per thread, 8,192 x 256 threads.
+double vector_fma_tflops(int prec);
 int vec_blocks(long n);     // blocks used by the element-wise kernels
 int site_blocks(int npol);  // blocks used by the per-site kernels
 

@@ -1584,4 +1584,60 @@ void launch_lu_apply(hipStream_t st, long n, const double *A, const int *piv, do
     HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------------- vector FMA rate
+// Eight independent FMA chains per thread over enough workgroups to fill every
+// SIMD, built like every kernel here without packed FP32 instructions: the rate
+// that non-packed vector code reaches on this device, the reference point of
+// the pair kernels, whose arithmetic is mostly such FMAs.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fma_rate(T *__restrict__ out, int iters, T m, T c) {
+    T a0 = (T)threadIdx.x * T(1e-7) + T(0.1), a1 = a0 + T(0.01), a2 = a0 + T(0.02), a3 = a0 + T(0.03);
+    T a4 = a0 + T(0.04), a5 = a0 + T(0.05), a6 = a0 + T(0.06), a7 = a0 + T(0.07);
+    for (int i = 0; i < iters; i++) {
+        a0 = fmaa(a0, m, c);
+        a1 = fmaa(a1, m, c);
+        a2 = fmaa(a2, m, c);
+        a3 = fmaa(a3, m, c);
+        a4 = fmaa(a4, m, c);
+        a5 = fmaa(a5, m, c);
+        a6 = fmaa(a6, m, c);
+        a7 = fmaa(a7, m, c);
+    }
+    out[(long)blockIdx.x * kBlock + threadIdx.x] = ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7));
+}
+
+template <typename T>
+static double fma_rate_t() {
+    constexpr int kBlocks = 8192, kIters = 20000, kReps = 5;
+    T *out = nullptr;
+    HIP_CHECK(hipMalloc((void **)&out, sizeof(T) * kBlocks * kBlock));
+    hipStream_t st;
+    HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t a, b;
+    HIP_CHECK(hipEventCreate(&a));
+    HIP_CHECK(hipEventCreate(&b));
+    hipLaunchKernelGGL(k_fma_rate<T>, dim3(kBlocks), dim3(kBlock), 0, st, out, kIters, T(0.999), T(0.001));
+    HIP_CHECK(hipEventRecord(a, st));
+    for (int r = 0; r < kReps; r++)
+        hipLaunchKernelGGL(k_fma_rate<T>, dim3(kBlocks), dim3(kBlock), 0, st, out, kIters, T(0.999), T(0.001));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(b, st));
+    HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    HIP_CHECK(hipEventDestroy(a));
+    HIP_CHECK(hipEventDestroy(b));
+    HIP_CHECK(hipStreamDestroy(st));
+    HIP_CHECK(hipFree(out));
+    const double flop = 2.0 * 8.0 * kIters * (double)kBlock * kBlocks * kReps;
+    return ms > 0.f ? flop / (ms * 1e-3) / 1e12 : 0.0;
+}
+
+double vector_fma_tflops(int prec) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        fatal("No GPU available: this library runs the polarization solver on an AMD Instinct GPU only");
+    return prec == 32 ? fma_rate_t<float>() : fma_rate_t<double>();
+}
+
 }  // namespace ommp

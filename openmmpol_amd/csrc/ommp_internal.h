@@ -158,6 +158,10 @@ void fmm_settings_changed(System &s);  // drops the tree, the static field and t
 void polelec_geomgrad_host(System &s, double *grd);
 void rotation_geomgrad_host(System &s, const double *E, const double *Egrd, double *grd);
 
+// kernels.hip: FMA rate (TFLOP/s) of non-packed vector code on the current
+// device, prec 32 or 64 (fatal without a GPU)
+double vector_fma_tflops(int prec);
+
 // server.cpp
 void *server_create(double budget, int max_active);
 void server_destroy(void *srv);

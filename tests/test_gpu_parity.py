@@ -276,3 +276,15 @@ def test_turn_pol_off_every_site(fixture_file):
     s.set_external_field(np.zeros((0, 3)))
     assert s.get_polelec_energy() == 0.0
     s.terminate()
+
+
+def test_vector_fma_rate_is_plausible():
+    """ommp_amd_vector_fma_tflops: the FMA rate of non-packed vector code, the
+    reference point bench.py reports beside the pair kernels' rooflines; between
+    1 TFLOP/s and twice the published vector peaks (FP64 78.6, FP32 157.3)."""
+    from openmmpol_amd._lib import lib
+    r64 = lib().ommp_amd_vector_fma_tflops(64)
+    r32 = lib().ommp_amd_vector_fma_tflops(32)
+    print(f"non-packed FMA rate: FP32 {r32:.1f}, FP64 {r64:.1f} TFLOP/s", flush=True)
+    assert 1.0 < r64 < 2 * 78.6
+    assert 1.0 < r32 < 2 * 157.3
