@@ -102,6 +102,8 @@ void launch_tmv_finalize64(hipStream_t st, bool amoeba, int npol, int M, const d
 // pairs[] = the block pairs (I <= J) of kSymBlock-site blocks, row-major;
 // part holds tmv_sym_part_bytes(npol, M, prec).
 int tmv_sym_blocks(int npol);
+// Per-column y.x partials written by the pair-once matvec's finalize.
+int tmv_sym_dot_parts(int npol);
 size_t tmv_sym_part_bytes(int npol, int M, int prec);
 // Whether the on-the-fly matvec of prec (32 / 64) with ncols columns uses the
 // pair-once sweep at npol sites (its partial sums at most max_bytes).

@@ -541,6 +541,13 @@ void ensure_sym(const System &s, DeviceState &d) {
     d.sympart = dmalloc(d, tmv_sym_part_bytes(s.npol, 2, 64));
 }
 
+// Number of y.x partials per column that matvec() of this kind writes to dotpart.
+static int matvec_dot_parts(const System &s, int kind, int M) {
+    if ((kind == kOtf64 || kind == kOtf32) && tmv_pair_once(s.npol, kind == kOtf32 ? 32 : 64, M))
+        return tmv_sym_dot_parts(s.npol);
+    return site_blocks(s.npol);
+}
+
 void matvec(System &s, DeviceState &d, int kind, int M, const double *x, double *y, bool dodiag,
             double *dotpart, const CgState *skip) {
     const long n = d.n;
@@ -736,7 +743,7 @@ CgResult cg_solve(System &s, DeviceState &d, int kind, int M, const double *b, d
                           nullptr);
     else
         launch_cg_init(d.stream, n, M, b, d.vh, x, d.vr, d.vz, d.vp, d.alpha, d.red_a, d.cs, zero_guess, d.tol_dev);
-    const int nred_hp = site_blocks(s.npol);
+    const int nred_hp = matvec_dot_parts(s, kind, M);
     int it = 1;
     // first batch: as many iterations as this call site needed last time
     const int pred = (slot >= 0 && slot < 16) ? s.pred_iters[slot] : 0;
@@ -922,7 +929,7 @@ int mixed_solve_ahead(System &s, DeviceState &d, int M, const double *b, double 
         HCHECK(hipHostMalloc((void **)&d.cs_spec_host, W * sizeof(CgState), hipHostMallocDefault));
     }
     CgState *out = d.cs_spec;
-    const int nred_hp = site_blocks(s.npol);
+    const int nred_hp = matvec_dot_parts(s, kind32, M);
     bool xzero = zero_guess;
     for (int o = 0; o < K; o++) {
         if (xzero) {

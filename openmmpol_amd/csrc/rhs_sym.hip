@@ -211,25 +211,27 @@ __global__ __launch_bounds__(kSymBlock) void k_rhs_sym(int npol, const int2 *__r
     for (int c = 0; c < 3; c++) part_i[((long)blockIdx.x * 3 + c) * kSymBlock + tid] = acc[c];
 }
 
-// E_M2D of one block of kSymBlock sites per workgroup: kRhsQ groups of
-// kSymBlock threads share each site; group q adds every kRhsQ-th partial of the
-// site's row (pairs (b, J >= b): target sides), column (pairs (I <= b, b): source
-// sides) and non-polarizable source split, and every kRhsQ-th entry of its
-// correction list (f = 1 - s for the D and P sets); group 0 adds the group
-// results in a fixed order.
-constexpr int kRhsQ = 4;
+// E_M2D of 64 sites per workgroup (kRhsSub workgroups per block of kSymBlock):
+// kRhsQ groups of 64 threads share each site; group q adds every kRhsQ-th
+// partial of the site's row (pairs (b, J >= b): target sides), column (pairs
+// (I <= b, b): source sides) and non-polarizable source split, and every
+// kRhsQ-th entry of its correction list (f = 1 - s for the D and P sets); group
+// 0 adds the group results in a fixed order.
+constexpr int kRhsQ = 8;
+constexpr int kRhsSub = kSymBlock / 64;
 
 template <bool AMOEBA>
-__global__ __launch_bounds__(kRhsQ *kSymBlock) void k_rhs_sym_finalize(
+__global__ __launch_bounds__(kRhsQ * 64) void k_rhs_sym_finalize(
     int npol, int nb, const double *__restrict__ part, const double *__restrict__ part_np, int nsplit_np,
     const double4 *__restrict__ ms, const double *__restrict__ ma, const double *__restrict__ mq,
     const double4 *__restrict__ ps, const double *__restrict__ pa, const int *__restrict__ sp_ri,
     const int *__restrict__ sp_ci, const double2 *__restrict__ sp_f, double *__restrict__ e_m2d) {
     constexpr int LD = AMOEBA ? 10 : 1;
-    __shared__ double shd[kRhsQ - 1][3][kSymBlock];
-    __shared__ double shp[kRhsQ - 1][3][kSymBlock];
-    const int q = threadIdx.x / kSymBlock, li = threadIdx.x % kSymBlock;
-    const int b = blockIdx.x, j = b * kSymBlock + li;
+    __shared__ double shd[kRhsQ - 1][3][64];
+    __shared__ double shp[kRhsQ - 1][3][64];
+    const int q = threadIdx.x >> 6, li = threadIdx.x & 63;
+    const int b = blockIdx.x / kRhsSub, t = (blockIdx.x % kRhsSub) * 64 + li;  // t: site within block b
+    const int j = b * kSymBlock + t;
     const bool valid = j < npol;
     const long np = (long)nb * (nb + 1) / 2, stride = 3L * kSymBlock;
     double ed[3] = {0.0, 0.0, 0.0}, ep[3] = {0.0, 0.0, 0.0};
@@ -238,13 +240,13 @@ __global__ __launch_bounds__(kRhsQ *kSymBlock) void k_rhs_sym_finalize(
         // pair (I, J) has index I nb - I (I - 1) / 2 + J - I
         const long row = (long)b * nb - (long)b * (b - 1) / 2;
         for (int J = b + q; J < nb; J += kRhsQ) {
-            const double *p = part + (row + J - b) * stride + li;
+            const double *p = part + (row + J - b) * stride + t;
 #pragma unroll
             for (int c = 0; c < 3; c++) sum[c] += p[(long)c * kSymBlock];
         }
         const double *part_j = part + np * stride;
         for (int I = q; I <= b; I += kRhsQ) {
-            const double *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + li;
+            const double *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + t;
 #pragma unroll
             for (int c = 0; c < 3; c++) sum[c] += p[(long)c * kSymBlock];
         }
@@ -339,10 +341,10 @@ void launch_rhs_sym(hipStream_t st, bool amoeba, int nmm, int npol, const int2 *
     if (eve) HIP_CHECK(hipEventRecord(eve, st));
     const int nsp = nnp > 0 ? nsplit_np : 0;
     if (amoeba)
-        hipLaunchKernelGGL(k_rhs_sym_finalize<true>, dim3(nb), dim3(kRhsQ * kSymBlock), 0, st, npol, nb, part_i,
+        hipLaunchKernelGGL(k_rhs_sym_finalize<true>, dim3(nb * kRhsSub), dim3(kRhsQ * 64), 0, st, npol, nb, part_i,
                            part_np, nsp, ms64, ma64, mq, ps64, pa64, sp.ri, sp.ci, sp.f2, e_m2d);
     else
-        hipLaunchKernelGGL(k_rhs_sym_finalize<false>, dim3(nb), dim3(kRhsQ * kSymBlock), 0, st, npol, nb, part_i,
+        hipLaunchKernelGGL(k_rhs_sym_finalize<false>, dim3(nb * kRhsSub), dim3(kRhsQ * 64), 0, st, npol, nb, part_i,
                            part_np, nsp, ms64, ma64, mq, ps64, pa64, sp.ri, sp.ci, sp.f2, e_m2d);
     HIP_CHECK(hipGetLastError());
 }

@@ -6,7 +6,7 @@
 //
 // Outputs, per block pair p: target side part_i[p][M * 3][256] (sites of I) and
 // source side part_j[p][M * 3][256] (sites of J); k_tmv_sym_finalize adds, for
-// every site, the nb partials of its row and column in a fixed order.
+// every site, the nb + 1 partials of its row and column in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -19,7 +19,7 @@
 namespace ommp {
 
 
-static_assert(kSymBlock == kBlock, "the finalize kernel maps one block of sites per workgroup");
+static_assert(kSymBlock == kBlock, "pair-once blocks are the blocks of the per-site kernels");
 
 template <typename R, int M, bool AMOEBA>
 __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__restrict__ pairs,
@@ -46,16 +46,19 @@ __global__ __launch_bounds__(kSymBlock) void k_tmv_sym(int npol, const int2 *__r
 }
 
 // y = -(row and column block-pair partials) + list_P_P corrections (+ x / alpha),
-// optional y.x partials, for one block of kSymBlock sites per workgroup.  kSymQ
-// groups of kSymBlock threads share each site: group q sums every kSymQ-th
+// optional y.x partials.  One workgroup per 64 sites (kSymSub per block of
+// kSymBlock), kSymQ groups of 64 threads each: group q sums every kSymQ-th
 // partial of the site's row (pairs (b, J >= b): target sides) and column (pairs
-// (I <= b, b): source sides) and every kSymQ-th entry of its correction list; group
-// 0 adds the group results in a fixed order.  (With one thread per site the
-// finalize waited on the latency of 2 nb dependent partial loads per thread.)
-constexpr int kSymQ = 4;
+// (I <= b, b): source sides) and every kSymQ-th entry of its correction list;
+// group 0 (one wave) adds the group results in a fixed order.  Spreading the
+// 2 nb partials of a site over 8 threads and the blocks over 4 workgroups keeps
+// each thread's chain of dependent loads short and puts the reads of a 15,000-site
+// system on ~240 CUs (one workgroup of 4 x 256 threads per block used 59).
+constexpr int kSymQ = 8;
+constexpr int kSymSub = kSymBlock / 64;
 
 template <typename R, int M, bool AMOEBA>
-__global__ __launch_bounds__(kSymQ *kSymBlock) void k_tmv_sym_finalize(
+__global__ __launch_bounds__(kSymQ * 64) void k_tmv_sym_finalize(
     int npol, int nb, const R *__restrict__ part, const double4 *__restrict__ ps, const double *__restrict__ pa,
     const double *__restrict__ alpha, const int *__restrict__ pp_ri, const int *__restrict__ pp_ci,
     const double *__restrict__ pp_f, const double *__restrict__ x, long xstride, double *__restrict__ y,
@@ -65,11 +68,11 @@ __global__ __launch_bounds__(kSymQ *kSymBlock) void k_tmv_sym_finalize(
         for (int m = 0; m < skipM; m++) all = all && skip[m].done;
         if (all) return;
     }
-    __shared__ R shs[kSymQ - 1][M * 3][kSymBlock];
-    __shared__ double shc[kSymQ - 1][M * 3][kSymBlock];
-    __shared__ double shd[M][kSymBlock / 64];
-    const int q = threadIdx.x / kSymBlock, li = threadIdx.x % kSymBlock;
-    const int b = blockIdx.x, j = b * kSymBlock + li;
+    __shared__ R shs[kSymQ - 1][M * 3][64];
+    __shared__ double shc[kSymQ - 1][M * 3][64];
+    const int q = threadIdx.x >> 6, li = threadIdx.x & 63;
+    const int b = blockIdx.x / kSymSub, t = (blockIdx.x % kSymSub) * 64 + li;  // t: site within block b
+    const int j = b * kSymBlock + t;
     const bool valid = j < npol;
     const long np = (long)nb * (nb + 1) / 2, stride = (long)M * 3 * kSymBlock;
     R sum[M * 3];
@@ -85,14 +88,14 @@ __global__ __launch_bounds__(kSymQ *kSymBlock) void k_tmv_sym_finalize(
         constexpr int kUnroll = sizeof(R) == 4 ? 4 : 2;
 #pragma unroll kUnroll
         for (int J = b + q; J < nb; J += kSymQ) {
-            const R *p = part + (row + J - b) * stride + li;
+            const R *p = part + (row + J - b) * stride + t;
 #pragma unroll
             for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
         }
         const R *part_j = part + np * stride;
 #pragma unroll kUnroll
         for (int I = q; I <= b; I += kSymQ) {
-            const R *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + li;
+            const R *p = part_j + ((long)I * nb - (long)I * (I - 1) / 2 + b - I) * stride + t;
 #pragma unroll
             for (int c = 0; c < M * 3; c++) sum[c] += p[(long)c * kSymBlock];
         }
@@ -124,21 +127,22 @@ __global__ __launch_bounds__(kSymQ *kSymBlock) void k_tmv_sym_finalize(
         }
     }
     __syncthreads();
+    if (q != 0) return;  // wave 0 from here on
     double v[M];
 #pragma unroll
     for (int m = 0; m < M; m++) v[m] = 0.0;
-    if (q == 0 && valid) {
+    if (valid) {
         double acc[M * 3];
 #pragma unroll
         for (int c = 0; c < M * 3; c++) {
-            R t = sum[c];
+            R s = sum[c];
             double u = corr[c];
 #pragma unroll
             for (int g = 0; g < kSymQ - 1; g++) {
-                t += shs[g][c][li];
+                s += shs[g][c][li];
                 u += shc[g][c][li];
             }
-            acc[c] = u - (double)t;
+            acc[c] = u - (double)s;
         }
         if (dodiag) {
             const double ia = 1.0 / alpha[j];
@@ -153,20 +157,11 @@ __global__ __launch_bounds__(kSymQ *kSymBlock) void k_tmv_sym_finalize(
                 v[m] += acc[3 * m + c] * x[m * xstride + 3L * j + c];
             }
     }
-    if (!dotpart) return;  // uniform over the workgroup
-    const int w = threadIdx.x >> 6;
-    if (q == 0) {
+    if (!dotpart) return;
 #pragma unroll
-        for (int m = 0; m < M; m++) {
-            const double t = wave_sum(v[m]);
-            if ((threadIdx.x & 63) == 0) shd[m][w] = t;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < M) {
-        const int m = threadIdx.x;
-        static_assert(kSymBlock / 64 == 4, "four waves per site group");
-        dotpart[m * gridDim.x + blockIdx.x] = (shd[m][0] + shd[m][1]) + (shd[m][2] + shd[m][3]);
+    for (int m = 0; m < M; m++) {
+        const double s = wave_sum(v[m]);
+        if (li == 0) dotpart[m * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -175,17 +170,18 @@ static void sym_finalize_t(hipStream_t st, bool amoeba, int npol, const R *part,
                            const double *pa64, const double *alpha, const CsrDev &pp, const double *x, long xs,
                            double *y, long ys, bool dodiag, double *dotpart, const CgState *skip, int skipM) {
     const int nb = tmv_sym_blocks(npol);
+    const dim3 grid(nb * kSymSub), block(kSymQ * 64);
     if (amoeba)
-        hipLaunchKernelGGL((k_tmv_sym_finalize<R, M, true>), dim3(nb), dim3(kSymQ * kSymBlock), 0, st, npol, nb, part,
-                           ps64, pa64, alpha, pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag, dotpart, skip, skipM);
+        hipLaunchKernelGGL((k_tmv_sym_finalize<R, M, true>), grid, block, 0, st, npol, nb, part, ps64, pa64, alpha,
+                           pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag, dotpart, skip, skipM);
     else
-        hipLaunchKernelGGL((k_tmv_sym_finalize<R, M, false>), dim3(nb), dim3(kSymQ * kSymBlock), 0, st, npol, nb,
-                           part, ps64, pa64, alpha, pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag, dotpart, skip,
-                           skipM);
+        hipLaunchKernelGGL((k_tmv_sym_finalize<R, M, false>), grid, block, 0, st, npol, nb, part, ps64, pa64, alpha,
+                           pp.ri, pp.ci, pp.f, x, xs, y, ys, (int)dodiag, dotpart, skip, skipM);
     HIP_CHECK(hipGetLastError());
 }
 
 int tmv_sym_blocks(int npol) { return (npol + kSymBlock - 1) / kSymBlock; }
+int tmv_sym_dot_parts(int npol) { return tmv_sym_blocks(npol) * kSymSub; }
 
 // The pair-once sweep evaluates half the pairs, at the price of a workgroup per
 // block pair and of its block-pair partial sums; below a few thousand sites it
