@@ -44,12 +44,11 @@ SegLayout seg_layout(int npol, int nrhs, int nsets, int m32, int num_cu, int ext
     }
     L.vec = align256(sizeof(double) * ncols * n);
     L.xs = m32 == kBatchM32 ? align256(sizeof(unsigned short) * kBxParts * ncols * (size_t)bx_ld(npol)) : 0;
-    // the pair-once sweep at the sizes where the single-system solver uses it
-    // (the matrix-core FP32 matvec keeps its own tiling); many columns may take
-    // up to 8 GiB of block-pair partials (15,000 sites x 128 columns: 2.8 GB,
-    // written and read once per FP64 matvec of ~20 ms)
+    // the pair-once sweep at the sizes where the single-system solver uses it;
+    // wide batches run both matvecs on the matrix cores with their own tiling
+    // (partials per source split)
     L.sym32 = m32 == 2 && tmv_pair_once(npol, 32, ncols, size_t(8) << 30);
-    L.sym64 = tmv_pair_once(npol, 64, ncols, size_t(8) << 30);
+    L.sym64 = m32 == 2 && tmv_pair_once(npol, 64, ncols, size_t(8) << 30);
     size_t part = sizeof(double) * L.nsplit * ncols * n;
     if (L.sym32 || L.sym64) part = std::max(part, tmv_sym_part_bytes(npol, ncols, 64));
     L.part = align256(part);
@@ -98,7 +97,7 @@ BatchEngine::~BatchEngine() {
     (void)hipFree(d_segs_);
     (void)hipFree(d_act_);
     (void)hipFree(d_actn_);
-    int4 *lists[] = {d_pair32_, d_fin32_, d_pair64_, d_fin64_, d_vec_, d_sym32_, d_sym64_};
+    int4 *lists[] = {d_pair32_, d_fin32_, d_pair64_, d_fin64_, d_vec_, d_sym32_, d_sym64_, d_pair64m_};
     for (auto p : lists)
         if (p) (void)hipFree(p);
     for (int k = 0; k < 2; k++) {
@@ -257,7 +256,7 @@ void BatchEngine::commit() {
     // largest systems first: their blocks start early, small ones fill the tail
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return segs[a].dev.npol > segs[b].dev.npol; });
-    std::vector<int4> p32, f32, p64, f64, vec, s32, s64;
+    std::vector<int4> p32, f32, p64, f64, vec, s32, s64, p64m;
     for (int si : order) {
         const BSeg &d = segs[si].dev;
         const int g32 = (d.ncols + m32 - 1) / m32, g64 = (d.ncols + m64 - 1) / m64;
@@ -284,7 +283,14 @@ void BatchEngine::commit() {
         }
         for (int g = 0; g < g32; g++)
             for (int tb = 0; tb < d.tblocks; tb++) f32.push_back(make_int4(si, tb, g, 0));
-        if (d.sym64)
+        if (m32 == kBatchM32) {
+            // FP64 refinement on the matrix cores, over the FP32 path's splits
+            const int ttiles = (d.npol + kMfmaSites - 1) / kMfmaSites;
+            const int gm = (d.ncols + kD64Cols - 1) / kD64Cols;
+            for (int g = 0; g < gm; g++)
+                for (int sp = 0; sp < d.nsplit; sp++)
+                    for (int tt = 0; tt < ttiles; tt++) p64m.push_back(make_int4(si, tt, sp, g));
+        } else if (d.sym64)
             sym_items(s64, g64);
         else
             for (int g = 0; g < g64; g++)
@@ -302,9 +308,10 @@ void BatchEngine::commit() {
     ensure_capacity(d_vec_, cap_vec_, (int)vec.size());
     ensure_capacity(d_sym32_, cap_sym32_, (int)s32.size());
     ensure_capacity(d_sym64_, cap_sym64_, (int)s64.size());
+    ensure_capacity(d_pair64m_, cap_pair64m_, (int)p64m.size());
     const size_t nseg = segs.size();
     const size_t need = sizeof(BSeg) * nseg + sizeof(int4) * (p32.size() + f32.size() + p64.size() + f64.size() +
-                                                              vec.size() + s32.size() + s64.size());
+                                                              vec.size() + s32.size() + s64.size() + p64m.size());
     if (stage_pending_) {
         BCHECK(hipEventSynchronize(stage_ev_));
         stage_pending_ = false;
@@ -333,6 +340,7 @@ void BatchEngine::commit() {
     put(vec.data(), sizeof(int4) * vec.size(), d_vec_);
     put(s32.data(), sizeof(int4) * s32.size(), d_sym32_);
     put(s64.data(), sizeof(int4) * s64.size(), d_sym64_);
+    put(p64m.data(), sizeof(int4) * p64m.size(), d_pair64m_);
     BCHECK(hipEventRecord(stage_ev_, stream));
     stage_pending_ = true;
     tables_.segs = d_segs_;
@@ -351,6 +359,8 @@ void BatchEngine::commit() {
     tables_.nsym32 = (int)s32.size();
     tables_.sym64 = d_sym64_;
     tables_.nsym64 = (int)s64.size();
+    tables_.pair64m = d_pair64m_;
+    tables_.npair64m = (int)p64m.size();
     tables_.act = d_act_;
     tables_.act_n = d_actn_;
     tables_.nseg = (int)nseg;

@@ -118,6 +118,10 @@ struct BatchTables {
     int nsym32;
     const int4 *sym64;
     int nsym64;
+    // wide batches: FP64 matrix-core items {segment, 32-site target tile, split,
+    // group of kD64Cols entries of the active-column list}
+    const int4 *pair64m;
+    int npair64m;
     const int4 *vec;
     int nvec;
     // the active columns of each segment, in increasing order, when the FP64
@@ -138,6 +142,7 @@ constexpr int kBxSrc = 32;
 constexpr int kBxParts = 3;  // bf16 parts per FP32 value (8 significant bits each)
 inline int bx_ld(int npol) { return 3 * ((npol + kBxSrc - 1) / kBxSrc) * kBxSrc; }
 constexpr int kBatchM64 = 4;  // columns per FP64 pair work item (multi-RHS)
+constexpr int kD64Cols = 64;  // refining columns per FP64 matrix-core work item (multi-RHS)
 
 // The inner matvec of the wide (matrix-core) batch alone: h = T p for every
 // column whose inner phase is set (a diagnostic of its accuracy and speed).

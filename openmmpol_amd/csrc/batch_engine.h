@@ -84,9 +84,9 @@ class BatchEngine {
     BSeg *d_segs_ = nullptr;
     int *d_act_ = nullptr, *d_actn_ = nullptr;  // active-column lists (BatchTables::act)
     int4 *d_pair32_ = nullptr, *d_fin32_ = nullptr, *d_pair64_ = nullptr, *d_fin64_ = nullptr, *d_vec_ = nullptr;
-    int4 *d_sym32_ = nullptr, *d_sym64_ = nullptr;
+    int4 *d_sym32_ = nullptr, *d_sym64_ = nullptr, *d_pair64m_ = nullptr;
     int cap_pair32_ = 0, cap_fin32_ = 0, cap_pair64_ = 0, cap_fin64_ = 0, cap_vec_ = 0;
-    int cap_sym32_ = 0, cap_sym64_ = 0;
+    int cap_sym32_ = 0, cap_sym64_ = 0, cap_pair64m_ = 0;
     BatchTables tables_{};
     bool dirty_ = true;
     // pinned staging for table uploads (reused only after its copy finished)

@@ -466,6 +466,165 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_bx6(const BSeg *__restrict__ 
     }
 }
 
+// ------------------------------------------------- FP64 matrix-core matvec
+// The FP64 refinement matvec of a wide batch as a dense contraction on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64, whose rate on MI355X equals the FP64
+// vector rate: what it saves is the per-column instruction stream of the
+// on-the-fly sweeps).  Work item: 32 target sites (96 rows, rows 3 t + a) x
+// kD64Cols packed refining columns (the segment's active-column list) x one
+// source split, in tiles of 16 sources (K = 48, k = 3 s + b).  The damped
+// tensors k3 d d^T - k2 I of a tile are built in FP64 into LDS.  Lane
+// (n = lane % 16, g = lane / 16) of wave w supplies, at step q of the twelve,
+// A[m][12 g + q] (m = n within the row block) and X[12 g + q][column 16 w + n]:
+// the four lane groups take the tile's K in four contiguous runs, so that a
+// lane's A and X fragments are 12 consecutive doubles each; the lane holds
+// D[g + 4 i][n], i = 0..3, of its row blocks.  With three or four 16-column
+// blocks in the item, wave w takes column block w and all six row blocks; with
+// one or two, the waves share the row blocks of a column block (w of a pair
+// takes three; with one block the waves take 2, 2, 1 and 1), so that an item
+// with few refining columns still spreads its products over the four SIMDs.
+// Partial fields go to part[split][column][3 npol] (FP64), summed by k_bfinal.
+constexpr int kD64Src = 16;
+constexpr int kD64K = 3 * kD64Src;  // 48
+constexpr int kD64Ld = kD64K + 2;   // LDS row stride (doubles)
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_bpairs_d64(const BSeg *__restrict__ segs, const int4 *__restrict__ work,
+                                                       const int *__restrict__ act, const int *__restrict__ act_n) {
+    const int4 w = work[blockIdx.x];
+    const BSeg &S = segs[w.x];
+    const int na = act_n[w.x];
+    const int c0 = w.w * kD64Cols;
+    if (c0 >= na) return;  // uniform across the workgroup
+    const int npol = S.npol, ncols = S.ncols;
+    const long n = 3L * npol;
+    const double4 *__restrict__ ps = S.ps64;
+    const double *__restrict__ pa = S.pa64;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, l16 = lane & 15;
+    __shared__ __align__(16) double As[3 * kMfmaSites][kD64Ld];
+    __shared__ double4 s_tp[kMfmaSites], s_sp[kD64Src];
+    __shared__ double s_ta[kMfmaSites], s_sa[kD64Src];
+    const int t0 = w.y * kMfmaSites;
+    if (tid < kMfmaSites) {
+        const int t = t0 + tid;
+        // targets past npol: far apart, their rows are not stored
+        s_tp[tid] = t < npol ? ps[t] : make_double4(1e15 + 1e9 * tid, 1e15, 1e15, 1.0);
+        s_ta[tid] = t < npol ? pa[t] : 0.39;
+    }
+    // this wave's column block and row blocks [rb0, rb1) (wave-uniform)
+    const int cblocks = (min(kD64Cols, na - c0) + 15) / 16;
+    int cb = wv, rb0 = 0, rb1 = 6;
+    if (cblocks == 2) {
+        cb = wv >> 1;
+        rb0 = 3 * (wv & 1);
+        rb1 = rb0 + 3;
+    } else if (cblocks == 1) {
+        cb = 0;
+        rb0 = wv < 2 ? 2 * wv : wv + 2;
+        rb1 = wv < 2 ? rb0 + 2 : rb0 + 1;
+    }
+    // this lane's column: position cl of the packed list; an inactive lane feeds
+    // zeros to its own result column, which is not stored
+    const int cl = c0 + 16 * cb + l16;
+    const bool cact = cl < na;
+    const int col = cact ? act[S.col0 + cl] : 0;
+    const double *__restrict__ xc = S.x + (long)col * n;
+    const bool wact = cb < cblocks;
+    const int i0 = w.z * S.chunk, i1 = min(npol, i0 + S.chunk);
+    auto fetch_src = [&](int base) {
+        if (tid < kD64Src) {
+            const int i = base + tid;
+            s_sp[tid] = i < i1 ? ps[i] : make_double4(-1e15 - 1e9 * tid, -1e15, -1e15, 1.0);
+            s_sa[tid] = i < i1 ? pa[i] : 0.39;
+        }
+    };
+    // X fragment of the tile at base: k = 12 g .. 12 g + 11, i.e. sources
+    // base + 4 g .. base + 4 g + 3, zero past the split's end
+    auto fetch_x = [&](int base, double (&xf)[12]) {
+        const int s0 = base + 4 * g;
+        if (cact && s0 + 3 < i1) {
+            // a column starts 8-byte aligned only (3 npol doubles per column)
+#pragma unroll
+            for (int q = 0; q < 12; q++) xf[q] = xc[3L * s0 + q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 12; q++) xf[q] = cact && s0 + q / 3 < i1 ? xc[3L * s0 + q] : 0.0;
+        }
+    };
+    mfma_d4 acc[6];
+#pragma unroll
+    for (int rb = 0; rb < 6; rb++) acc[rb] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+    double xf[12];
+    if (i0 < i1) {
+        fetch_src(i0);
+        fetch_x(i0, xf);
+    }
+    __syncthreads();
+    for (int base = i0; base < i1; base += kD64Src) {
+        // tensors of the tile: thread (t = tid % 32, s = tid / 32) builds the
+        // 3 x 3 blocks of target t against sources s and s + 8
+        {
+            const int ti = tid & 31;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int sj = (tid >> 5) + 8 * h;
+                double dx, dy, dz, k2, k3, arg;
+                bool near;
+                pair_geom<double, AMOEBA>(s_tp[ti], s_ta[ti], s_sp[sj], s_sa[sj], dx, dy, dz, k2, k3, arg, near);
+                if (__ballot(near)) pair_damp<double, AMOEBA>(arg, near, k2, k3);
+                const double d[3] = {dx, dy, dz};
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) As[3 * ti + a][3 * sj + b] = k3 * d[a] * d[b] - (a == b ? k2 : 0.0);
+            }
+        }
+        double xn[12];
+        __syncthreads();
+        // the next tile's sources and X fragments, while this tile's products run
+        const bool more = base + kD64Src < i1;
+        if (more) {
+            fetch_src(base + kD64Src);
+            fetch_x(base + kD64Src, xn);
+        }
+        if (wact) {
+#pragma unroll
+            for (int rb = 0; rb < 6; rb++) {
+                if (rb < rb0 || rb >= rb1) continue;  // wave-uniform
+                double af[12];
+                const double2 *arow = reinterpret_cast<const double2 *>(&As[16 * rb + l16][12 * g]);
+#pragma unroll
+                for (int u = 0; u < 6; u++) {
+                    const double2 v = arow[u];
+                    af[2 * u] = v.x;
+                    af[2 * u + 1] = v.y;
+                }
+                mfma_d4 c = acc[rb];
+#pragma unroll
+                for (int q = 0; q < 12; q++) c = __builtin_amdgcn_mfma_f64_16x16x4f64(af[q], xf[q], c, 0, 0, 0);
+                acc[rb] = c;
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < 12; q++) xf[q] = xn[q];
+        }
+        __syncthreads();  // this tile's A reads are done; the next sources are in LDS
+    }
+    if (!cact) return;
+    double *part = static_cast<double *>(S.part) + ((long)w.z * ncols + col) * n + 3L * t0;
+#pragma unroll
+    for (int rb = 0; rb < 6; rb++) {
+        if (rb < rb0 || rb >= rb1) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = 16 * rb + g + 4 * i;
+            if (t0 + row / 3 < npol) part[row] = acc[rb][i];
+        }
+    }
+}
+
 // Three-part bf16 split of the inner search directions for the matrix-core
 // matvec; the first block of a column also zeroes the row's padding.
 __global__ __launch_bounds__(kBlock) void k_bconvert(const BSeg *__restrict__ segs, const BCol *__restrict__ cols,
@@ -884,7 +1043,14 @@ static void mv_launch(hipStream_t st, const BatchTables &t, const int4 *pair, in
 
 template <bool AMOEBA>
 static void tick_t(hipStream_t st, const BatchTables &t, int m32, int m64, double eta, double frac) {
-    if (m64 == 2) mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0);
+    if (t.npair64m) {
+        hipLaunchKernelGGL(k_bcompact, dim3(t.nseg), dim3(kBlock), 0, st, t.segs, t.cols, 0, t.act, t.act_n);
+        hipLaunchKernelGGL(k_bpairs_d64<AMOEBA>, dim3(t.npair64m), dim3(kBlock), 0, st, t.segs, t.pair64m, t.act,
+                           t.act_n);
+        hipLaunchKernelGGL((k_bfinal<double, kBatchM64, AMOEBA>), dim3(t.nfin64), dim3(kBlock), 0, st, t.segs, t.cols,
+                           t.fin64, 0);
+    } else if (m64 == 2)
+        mv_launch<double, 2, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0);
     else
         mv_launch<double, kBatchM64, AMOEBA>(st, t, t.pair64, t.npair64, t.sym64, t.nsym64, t.fin64, t.nfin64, 0,
                                              true);
