@@ -42,13 +42,18 @@ def sym_pair_flops(n, m):
 
 def grad_flops(npol, nmm):
     """FP64 FLOP per gradient (FMA = 2, exponentials of damped pairs not
-    counted): the static-multipole sweep 135 per (polarizable target, MM source)
-    pair, the dipole-dipole sweep 113 per polarizable pair (each unordered pair
-    once from 5,000 polarizable sites, where that sweep is pair-once; every
-    ordered pair below), the MM-target sweep 166 per (MM target, polarizable
-    source) pair."""
-    dd = npol * (npol - 1) // 2 if npol >= 5000 else npol * npol
-    return 135 * npol * nmm + 113 * dd + 166 * nmm * npol
+    counted).  Below 5,000 polarizable sites: the static-multipole sweep 135 per
+    (polarizable target, MM source) pair, the MM-target sweep 166 per (MM
+    target, polarizable source) pair, the dipole-dipole sweep 113 per ordered
+    polarizable pair.  From 5,000, where each (polarizable, MM) pair of the
+    static part is evaluated once for both ends and each unordered polarizable
+    pair once in the dipole-dipole part: 200 per (polarizable, MM) pair (the
+    135 of the polarizable end, 58 for the field and field gradient of the
+    dipoles at the MM end, 6 for the two ends' sums) and 113 per unordered
+    polarizable pair."""
+    if npol >= 5000:
+        return 200 * npol * nmm + 113 * (npol * (npol - 1) // 2)
+    return 135 * npol * nmm + 113 * npol * npol + 166 * nmm * npol
 
 
 def incore_bytes(n, m, s):

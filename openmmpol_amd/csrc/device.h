@@ -174,14 +174,20 @@ struct GradArgs {
     // polarizable sites, its block-pair partials in sympart
     const int2 *sym_pairs = nullptr;
     double *sympart = nullptr;
+    // set (grad_both_ends): the static part evaluates each (polarizable, MM)
+    // pair once for both ends; part_pol then holds ceil(nmm / kBlock) static
+    // partials and part_mm ceil(npol / kBlock)
+    bool fused = false;
 };
+bool grad_both_ends(int npol, int nmm, bool amoeba);
 // out[nmm][3] = d E_pol / d r (pair sweeps + rotation term); emm/egrd receive
 // E_D2M and Egrd_D2M.  evb/eve bracket the three pair sweeps.
 void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb = nullptr, hipEvent_t eve = nullptr);
 // The same after the pair sums: finalize (sum of nparts partials in the layouts
 // above, screened-pair corrections), rotation term and per-atom assembly, for
 // partials computed elsewhere (fast multipoles).
-void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm);
+// direct: the MM partials carry the translational gradient (k_gstat_both).
+void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm, bool direct = false);
 // out[nmm][3] = rotation term only, for the field a.emm and gradient a.egrd.
 void launch_rotation_grad(hipStream_t st, const GradArgs &a);
 

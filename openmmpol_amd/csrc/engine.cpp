@@ -1553,8 +1553,12 @@ void ensure_grad(System &s, DeviceState &d) {
     d.g_split_s = balanced_split(site_blocks(npol), nmm, cu);
     d.g_split_p = balanced_split(site_blocks(npol), npol, cu);
     d.g_split_m = balanced_split(site_blocks(nmm), npol, cu);
-    d.g_part_pol = dalloc<double>(d, (size_t)(d.g_split_s + d.g_split_p) * 3 * npol);
-    d.g_part_mm = dalloc<double>(d, (size_t)d.g_split_m * (s.amoeba ? 12 : 3) * nmm);
+    // partials of the per-target sweeps, or of the both-ends static sweep
+    // (one per block of the other end) and the dipole-dipole part
+    const bool both = grad_both_ends(npol, nmm, s.amoeba);
+    const size_t nstat = both ? site_blocks(nmm) : d.g_split_s, nmmp = both ? site_blocks(npol) : d.g_split_m;
+    d.g_part_pol = dalloc<double>(d, (nstat + d.g_split_p) * 3 * npol);
+    d.g_part_mm = dalloc<double>(d, nmmp * (s.amoeba ? 12 : 3) * nmm);
     d.g_pol = dalloc<double>(d, 3L * npol);
     d.g_tr = dalloc<double>(d, 3L * nmm);
     d.g_emm = dalloc<double>(d, 3L * nmm);
@@ -1634,6 +1638,7 @@ void polelec_geomgrad_host(System &s, double *grd) {
             a.sym_pairs = d.sym_pairs;
             a.sympart = static_cast<double *>(d.sympart);
         }
+        a.fused = grad_both_ends(s.npol, s.nmm, s.amoeba);
         launch_geomgrad(d.stream, a, evb, eve);
     }
     HCHECK(hipMemcpyAsync(grd, d.g_out, sizeof(double) * 3L * s.nmm, hipMemcpyDeviceToHost, d.stream));

@@ -326,6 +326,14 @@ __global__ __launch_bounds__(4 * kSymBlock) void k_sym3_reduce(int npol, int nb,
     }
 }
 
+// The finalizes take 64 sites per workgroup with kGfQ groups of 64 threads:
+// group q sums every kGfQ-th partial and every kGfQ-th listed pair of its
+// site, and group 0 adds the group results in a fixed order (one thread per
+// site waited on the latency of its whole chain of partials and pairs).
+constexpr int kGfQ = 4;
+constexpr int kGfSites = kBlock / kGfQ;
+static_assert(kGfSites == 64, "one wave of sites per group");
+
 template <bool AMOEBA>
 __global__ __launch_bounds__(kBlock) void k_gpol_finalize(int npol, int nparts, const double *__restrict__ part,
                                                           const double4 *__restrict__ ms,
@@ -341,64 +349,80 @@ __global__ __launch_bounds__(kBlock) void k_gpol_finalize(int npol, int nparts, 
                                                           const double *__restrict__ pp_f,
                                                           double *__restrict__ gpol) {
     constexpr int LD = AMOEBA ? 10 : 1;
-    const int j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= npol) return;
+    __shared__ double sh[kGfQ - 1][3][kGfSites];
+    const int q = threadIdx.x / kGfSites, li = threadIdx.x % kGfSites;
+    const int j = blockIdx.x * kGfSites + li;
+    const bool valid = j < npol;
     const long n = 3L * npol;
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    for (int s = 0; s < nparts; s++) {
-        const double *p = part + (long)s * 3 * npol;
-        gx += p[j];
-        gy += p[npol + j];
-        gz += p[2L * npol + j];
-    }
-    const double4 pj = ps[j];
-    const double aj = pa[j];
-    const double dX = ipd[3L * j], dY = ipd[3L * j + 1], dZ = ipd[3L * j + 2];
-    const double pX = AMOEBA ? ipd[n + 3L * j] : dX, pY = AMOEBA ? ipd[n + 3L * j + 1] : dY,
-                 pZ = AMOEBA ? ipd[n + 3L * j + 2] : dZ;
-    // listed static sources: remove (1 - s_p) of the mu_D contraction and
-    // (1 - s_d) of the mu_P one (AMBER: both slots hold 1 - s)
-    for (int k = sp_ri[j]; k < sp_ri[j + 1]; k++) {
-        const int i = sp_ci[k];
-        const double2 f = sp_f[k];
-        const double wx = AMOEBA ? 0.5 * (f.y * dX + f.x * pX) : f.y * dX;
-        const double wy = AMOEBA ? 0.5 * (f.y * dY + f.x * pY) : f.y * dY;
-        const double wz = AMOEBA ? 0.5 * (f.y * dZ + f.x * pZ) : f.y * dZ;
-        const double4 pi = ms[i];
-        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        double k2, k3, k4, k5;
-        damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, ma[i], aj, k2, k3, k4, k5);
-        double q[LD];
-        for (int c = 0; c < LD; c++) q[c] = mq[(long)LD * i + c];
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        static_grad_pair<AMOEBA>(dx, dy, dz, k2, k3, k4, k5, q, wx, wy, wz, cx, cy, cz);
-        gx -= cx;
-        gy -= cy;
-        gz -= cz;
-    }
-    // listed dipole sources: remove (1 - s_u) of the pair
-    for (int k = pp_ri[j]; k < pp_ri[j + 1]; k++) {
-        const int i = pp_ci[k];
-        const double4 pi = ps[i];
-        const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        double k2, k3, k4, k5;
-        damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, pa[i], aj, k2, k3, k4, k5);
-        const double f = pp_f[k] * (AMOEBA ? 0.5 : 1.0);
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        if (AMOEBA) {
-            dip_grad_pair(dx, dy, dz, k3, k4, dX, dY, dZ, ipd[n + 3L * i], ipd[n + 3L * i + 1], ipd[n + 3L * i + 2],
-                          f, cx, cy, cz);
-            dip_grad_pair(dx, dy, dz, k3, k4, pX, pY, pZ, ipd[3L * i], ipd[3L * i + 1], ipd[3L * i + 2], f, cx, cy,
-                          cz);
-        } else {
-            dip_grad_pair(dx, dy, dz, k3, k4, dX, dY, dZ, ipd[3L * i], ipd[3L * i + 1], ipd[3L * i + 2], f, cx, cy,
-                          cz);
+    if (valid) {
+        for (int s = q; s < nparts; s += kGfQ) {
+            const double *p = part + (long)s * 3 * npol;
+            gx += p[j];
+            gy += p[npol + j];
+            gz += p[2L * npol + j];
         }
-        gx -= cx;
-        gy -= cy;
-        gz -= cz;
+        const double4 pj = ps[j];
+        const double aj = pa[j];
+        const double dX = ipd[3L * j], dY = ipd[3L * j + 1], dZ = ipd[3L * j + 2];
+        const double pX = AMOEBA ? ipd[n + 3L * j] : dX, pY = AMOEBA ? ipd[n + 3L * j + 1] : dY,
+                     pZ = AMOEBA ? ipd[n + 3L * j + 2] : dZ;
+        // listed static sources: remove (1 - s_p) of the mu_D contraction and
+        // (1 - s_d) of the mu_P one (AMBER: both slots hold 1 - s)
+        for (int k = sp_ri[j] + q; k < sp_ri[j + 1]; k += kGfQ) {
+            const int i = sp_ci[k];
+            const double2 f = sp_f[k];
+            const double wx = AMOEBA ? 0.5 * (f.y * dX + f.x * pX) : f.y * dX;
+            const double wy = AMOEBA ? 0.5 * (f.y * dY + f.x * pY) : f.y * dY;
+            const double wz = AMOEBA ? 0.5 * (f.y * dZ + f.x * pZ) : f.y * dZ;
+            const double4 pi = ms[i];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, ma[i], aj, k2, k3, k4, k5);
+            double qm[LD];
+            for (int c = 0; c < LD; c++) qm[c] = mq[(long)LD * i + c];
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            static_grad_pair<AMOEBA>(dx, dy, dz, k2, k3, k4, k5, qm, wx, wy, wz, cx, cy, cz);
+            gx -= cx;
+            gy -= cy;
+            gz -= cz;
+        }
+        // listed dipole sources: remove (1 - s_u) of the pair
+        for (int k = pp_ri[j] + q; k < pp_ri[j + 1]; k += kGfQ) {
+            const int i = pp_ci[k];
+            const double4 pi = ps[i];
+            const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pj.w, pa[i], aj, k2, k3, k4, k5);
+            const double f = pp_f[k] * (AMOEBA ? 0.5 : 1.0);
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            if (AMOEBA) {
+                dip_grad_pair(dx, dy, dz, k3, k4, dX, dY, dZ, ipd[n + 3L * i], ipd[n + 3L * i + 1],
+                              ipd[n + 3L * i + 2], f, cx, cy, cz);
+                dip_grad_pair(dx, dy, dz, k3, k4, pX, pY, pZ, ipd[3L * i], ipd[3L * i + 1], ipd[3L * i + 2], f, cx,
+                              cy, cz);
+            } else {
+                dip_grad_pair(dx, dy, dz, k3, k4, dX, dY, dZ, ipd[3L * i], ipd[3L * i + 1], ipd[3L * i + 2], f, cx,
+                              cy, cz);
+            }
+            gx -= cx;
+            gy -= cy;
+            gz -= cz;
+        }
+    }
+    if (q > 0) {
+        sh[q - 1][0][li] = gx;
+        sh[q - 1][1][li] = gy;
+        sh[q - 1][2][li] = gz;
+    }
+    __syncthreads();
+    if (q != 0 || !valid) return;
+    for (int g = 0; g < kGfQ - 1; g++) {
+        gx += sh[g][0][li];
+        gy += sh[g][1][li];
+        gz += sh[g][2][li];
     }
     gpol[3L * j] = gx;
     gpol[3L * j + 1] = gy;
@@ -462,10 +486,119 @@ __global__ __launch_bounds__(kBlock) void k_gmm_pairs(int nmm, int npol, int chu
     }
 }
 
-// E_D2M -> emm[m][3], Egrd_D2M -> egrd[m][6] (AMOEBA), and the MM-site gradient
-// -q E + mu . Egrd - Q : EHes -> gtr[m][3].
+// ------------------------------------- static-multipole pairs, both ends
+// Each (polarizable site i, MM site m) pair evaluated once instead of once per
+// end.  The pair's energy -nu_i . E_m(r_i) (nu = 1/2 (mu_D + mu_P) for AMOEBA,
+// mu for AMBER; every pair at scale 1 as in the per-target sweeps) depends on
+// r_i - r_m alone at fixed multipole orientation, so its gradient at m is minus
+// its gradient at i, sum_b nu_b Egrd_ab of m's multipoles (static_grad_pair);
+// the MM end also takes the field and field gradient of nu for the rotation
+// term.  One workgroup per (MM block J, polarizable block I) of kBlock sites:
+// wave w holds the MM sites of sub-tile w of J as targets (position,
+// multipoles, field, field gradient and translational gradient in registers),
+// the polarizable sites of I are staged in LDS; at step s lane l meets source
+// (l + s) mod 64 of sub-tile k, and the source's gradient sum passes one lane
+// down per step by the DPP wave rotate (after 64 steps it is back at its lane).
+// Outputs: polarizable-end sums part_pol[J][3][npol], MM-end sums
+// part_mm[I][NA][nmm]: E, Egrd and the translational gradient (AMOEBA), E (AMBER).
 template <bool AMOEBA>
-__global__ __launch_bounds__(kBlock) void k_gmm_finalize(int nmm, int npol, int nparts,
+__global__ __launch_bounds__(kBlock) void k_gstat_both(int nmm, int npol, const double4 *__restrict__ ms,
+                                                       const double *__restrict__ ma, const double *__restrict__ mq,
+                                                       const double4 *__restrict__ ps,
+                                                       const double *__restrict__ pa,
+                                                       const int *__restrict__ mm_polar,
+                                                       const double *__restrict__ ipd, double *__restrict__ part_pol,
+                                                       double *__restrict__ part_mm) {
+    constexpr int LD = AMOEBA ? 10 : 1;
+    constexpr int NA = AMOEBA ? 12 : 3;
+    static_assert(kBlock == 256, "four 64-site sub-tiles per block");
+    __shared__ double4 s_pos[kBlock];
+    __shared__ double s_a[kBlock];
+    __shared__ double s_n[3][kBlock];
+    __shared__ double s_acc[kBlock / 64][3][64];
+    const int tid = threadIdx.x, l = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int J = blockIdx.x, I = blockIdx.y;
+    const long n = 3L * npol;
+    // sources: polarizable sites of I; padding far away with zero dipoles
+    {
+        const int i = I * kBlock + tid;
+        if (i < npol) {
+            s_pos[tid] = ps[i];
+            s_a[tid] = pa[i];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+                s_n[c][tid] = AMOEBA ? 0.5 * (ipd[3L * i + c] + ipd[n + 3L * i + c]) : ipd[3L * i + c];
+        } else {
+            s_pos[tid] = make_double4(-1e15 - 1e9 * tid, -1e15, -1e15, 1.0);
+            s_a[tid] = 0.39;
+#pragma unroll
+            for (int c = 0; c < 3; c++) s_n[c][tid] = 0.0;
+        }
+    }
+    // target: MM site m of J; padding far away with zero multipoles
+    const int m = J * kBlock + tid;
+    const bool valid = m < nmm;
+    const double4 pm = valid ? ms[m] : make_double4(1e15 + 1e9 * tid, 1e15, 1e15, 1.0);
+    const double am = valid ? ma[m] : 0.39;
+    const int self = valid ? mm_polar[m] - I * kBlock : -1;  // the polarizable index of m within I
+    double q[LD];
+#pragma unroll
+    for (int c = 0; c < LD; c++) q[c] = valid ? mq[(long)LD * m + c] : 0.0;
+    double acc[NA];
+#pragma unroll
+    for (int c = 0; c < NA; c++) acc[c] = 0.0;
+    __syncthreads();
+#pragma unroll 1
+    for (int k = 0; k < kBlock / 64; k++) {
+        double jg[3] = {0.0, 0.0, 0.0};  // gradient at the source now at this lane
+        for (int s = 0; s < 64; s++) {
+            const int js = k * 64 + ((l + s) & 63);
+            const double4 pi = s_pos[js];
+            const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            const double ir = js == self ? 0.0 : fast_rsqrt(r2);
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, ir, pi.w, pm.w, s_a[js], am, k2, k3, k4, k5);
+            const double nx = s_n[0][js], ny = s_n[1][js], nz = s_n[2][js];
+            // gradient at the polarizable end, d = r_i - r_m = -e
+            double tx = 0.0, ty = 0.0, tz = 0.0;
+            static_grad_pair<AMOEBA>(-ex, -ey, -ez, k2, k3, k4, k5, q, nx, ny, nz, tx, ty, tz);
+            jg[0] += tx;
+            jg[1] += ty;
+            jg[2] += tz;
+            dip_field_at<AMOEBA>(ex, ey, ez, k2, k3, k4, nx, ny, nz, acc);
+            if (AMOEBA) {
+                acc[9] -= tx;
+                acc[10] -= ty;
+                acc[11] -= tz;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) jg[c] = lane_next(jg[c]);
+        }
+        // sources k * 64 + lane: the four waves' rows added in wave order
+#pragma unroll
+        for (int c = 0; c < 3; c++) s_acc[w][c][l] = jg[c];
+        __syncthreads();
+        if (tid < 3 * 64) {
+            const int c = tid >> 6, jl = tid & 63, i = I * kBlock + k * 64 + jl;
+            const double sum = (s_acc[0][c][jl] + s_acc[1][c][jl]) + (s_acc[2][c][jl] + s_acc[3][c][jl]);
+            if (i < npol) part_pol[((long)J * 3 + c) * npol + i] = sum;
+        }
+        __syncthreads();
+    }
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < NA; c++) part_mm[((long)I * NA + c) * nmm + m] = acc[c];
+    }
+}
+
+// E_D2M -> emm[m][3], Egrd_D2M -> egrd[m][6] (AMOEBA), and the MM-site gradient
+// -q E + mu . Egrd - Q : EHes -> gtr[m][3].  direct (AMOEBA): the partials'
+// entries 9-11 hold the translational gradient itself (k_gstat_both), and the
+// formula applies to the listed-pair corrections only.
+template <bool AMOEBA>
+__global__ __launch_bounds__(kBlock) void k_gmm_finalize(int nmm, int npol, int nparts, int direct,
                                                          const double *__restrict__ part,
                                                          const double4 *__restrict__ ms,
                                                          const double *__restrict__ ma, const double *__restrict__ mq,
@@ -477,53 +610,75 @@ __global__ __launch_bounds__(kBlock) void k_gmm_finalize(int nmm, int npol, int 
                                                          double *__restrict__ egrd, double *__restrict__ gtr) {
     constexpr int NA = AMOEBA ? 12 : 3;
     constexpr int LD = AMOEBA ? 10 : 1;
-    const int m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= nmm) return;
+    __shared__ double sh[kGfQ - 1][2 * NA][kGfSites];
+    const int qg = threadIdx.x / kGfSites, li = threadIdx.x % kGfSites;
+    const int m = blockIdx.x * kGfSites + li;
+    const bool valid = m < nmm;
     const long n = 3L * npol;
-    double acc[NA];
-    for (int c = 0; c < NA; c++) acc[c] = 0.0;
-    for (int s = 0; s < nparts; s++) {
-        const double *p = part + (long)s * NA * nmm;
-        for (int c = 0; c < NA; c++) acc[c] += p[(long)c * nmm + m];
-    }
+    double acc[NA], corr[NA];
+    for (int c = 0; c < NA; c++) acc[c] = corr[c] = 0.0;
     double q[LD];
-    for (int c = 0; c < LD; c++) q[c] = mq[(long)LD * m + c];
+    for (int c = 0; c < LD; c++) q[c] = valid ? mq[(long)LD * m + c] : 0.0;
     const double *Q = q + (AMOEBA ? 4 : 0);
     const double trQ = AMOEBA ? Q[0] + Q[2] + Q[5] : 0.0;
-    const double4 pm = ms[m];
-    const double am = ma[m];
-    // listed sources: remove 1/2 ((1 - s_d) mu_P + (1 - s_p) mu_D) (AMBER: (1 - s) mu)
-    for (int k = mp_ri[m]; k < mp_ri[m + 1]; k++) {
-        const int i = mp_ci[k];
-        const double2 f = mp_f[k];
-        double nx, ny, nz;
-        if (AMOEBA) {
-            nx = -0.5 * (f.x * ipd[n + 3L * i] + f.y * ipd[3L * i]);
-            ny = -0.5 * (f.x * ipd[n + 3L * i + 1] + f.y * ipd[3L * i + 1]);
-            nz = -0.5 * (f.x * ipd[n + 3L * i + 2] + f.y * ipd[3L * i + 2]);
-        } else {
-            nx = -f.x * ipd[3L * i];
-            ny = -f.x * ipd[3L * i + 1];
-            nz = -f.x * ipd[3L * i + 2];
+    if (valid) {
+        for (int s = qg; s < nparts; s += kGfQ) {
+            const double *p = part + (long)s * NA * nmm;
+            for (int c = 0; c < NA; c++) acc[c] += p[(long)c * nmm + m];
         }
-        const double4 pi = ps[i];
-        const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
-        const double r2 = ex * ex + ey * ey + ez * ez;
-        double k2, k3, k4, k5;
-        damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pm.w, pa[i], am, k2, k3, k4, k5);
-        dip_at_mm<AMOEBA>(ex, ey, ez, k2, k3, k4, k5, nx, ny, nz, Q, trQ, acc);
+        const double4 pm = ms[m];
+        const double am = ma[m];
+        // listed sources: remove 1/2 ((1 - s_d) mu_P + (1 - s_p) mu_D) (AMBER: (1 - s) mu)
+        for (int k = mp_ri[m] + qg; k < mp_ri[m + 1]; k += kGfQ) {
+            const int i = mp_ci[k];
+            const double2 f = mp_f[k];
+            double nx, ny, nz;
+            if (AMOEBA) {
+                nx = -0.5 * (f.x * ipd[n + 3L * i] + f.y * ipd[3L * i]);
+                ny = -0.5 * (f.x * ipd[n + 3L * i + 1] + f.y * ipd[3L * i + 1]);
+                nz = -0.5 * (f.x * ipd[n + 3L * i + 2] + f.y * ipd[3L * i + 2]);
+            } else {
+                nx = -f.x * ipd[3L * i];
+                ny = -f.x * ipd[3L * i + 1];
+                nz = -f.x * ipd[3L * i + 2];
+            }
+            const double4 pi = ps[i];
+            const double ex = pm.x - pi.x, ey = pm.y - pi.y, ez = pm.z - pi.z;
+            const double r2 = ex * ex + ey * ey + ez * ez;
+            double k2, k3, k4, k5;
+            damp_k5<AMOEBA>(r2, rsqrt(r2), pi.w, pm.w, pa[i], am, k2, k3, k4, k5);
+            dip_at_mm<AMOEBA>(ex, ey, ez, k2, k3, k4, k5, nx, ny, nz, Q, trQ, corr);
+        }
     }
+    if (qg > 0) {
+        for (int c = 0; c < NA; c++) {
+            sh[qg - 1][c][li] = acc[c];
+            sh[qg - 1][NA + c][li] = corr[c];
+        }
+    }
+    __syncthreads();
+    if (qg != 0 || !valid) return;
+    for (int g = 0; g < kGfQ - 1; g++)
+        for (int c = 0; c < NA; c++) {
+            acc[c] += sh[g][c][li];
+            corr[c] += sh[g][NA + c][li];
+        }
+    // the formula's terms: of the corrections alone (direct), else of the totals
+    double f[NA];
+    for (int c = 0; c < NA; c++) f[c] = AMOEBA && direct ? corr[c] : acc[c] + corr[c];
     double g[3];
     for (int c = 0; c < 3; c++) {
-        g[c] = -q[0] * acc[c];
-        emm[3L * m + c] = acc[c];
+        g[c] = -q[0] * f[c];
+        emm[3L * m + c] = acc[c] + corr[c];
     }
     if (AMOEBA) {
-        const double *G = acc + 3;  // xx xy yy xz yz zz
-        g[0] += q[1] * G[0] + q[2] * G[1] + q[3] * G[3] - acc[9];
-        g[1] += q[1] * G[1] + q[2] * G[2] + q[3] * G[4] - acc[10];
-        g[2] += q[1] * G[3] + q[2] * G[4] + q[3] * G[5] - acc[11];
-        for (int c = 0; c < 6; c++) egrd[6L * m + c] = G[c];
+        const double *G = f + 3;  // xx xy yy xz yz zz
+        g[0] += q[1] * G[0] + q[2] * G[1] + q[3] * G[3] - f[9];
+        g[1] += q[1] * G[1] + q[2] * G[2] + q[3] * G[4] - f[10];
+        g[2] += q[1] * G[3] + q[2] * G[4] + q[3] * G[5] - f[11];
+        if (direct)
+            for (int c = 0; c < 3; c++) g[c] += acc[9 + c];
+        for (int c = 0; c < 6; c++) egrd[6L * m + c] = acc[3 + c] + corr[3 + c];
     }
     for (int c = 0; c < 3; c++) gtr[3L * m + c] = g[c];
 }
@@ -675,67 +830,85 @@ __global__ __launch_bounds__(kBlock) void k_gassemble(int nmm, const double *__r
 }
 
 // ------------------------------------------------------------- launchers
-void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb, hipEvent_t eve) {
-    const int nmm = a.nmm, npol = a.npol;
-    const int pb = (npol + kBlock - 1) / kBlock, mb = (nmm + kBlock - 1) / kBlock;
-    if (evb) HIP_CHECK(hipEventRecord(evb, st));
-    if (npol > 0) {
-        const int cs = (nmm + a.split_s - 1) / a.split_s, cp = (npol + a.split_p - 1) / a.split_p;
-        double *part_p = a.part_pol + (long)a.split_s * 3 * npol;
-        const int cm = (npol + a.split_m - 1) / a.split_m;
-        // the dipole-dipole part pair-once into part_p[0] (one partial), or per target
-        const int nb = a.sym_pairs ? tmv_sym_blocks(npol) : 0;
-        const long np = (long)nb * (nb + 1) / 2;
-        double *sym_i = a.sympart, *sym_j = a.sym_pairs ? a.sympart + np * 3 * kSymBlock : nullptr;
-        if (a.amoeba) {
-            hipLaunchKernelGGL(k_gpol_static<true>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms,
-                               a.ma, a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
-            if (a.sym_pairs)
-                hipLaunchKernelGGL(k_gpol_dd_sym<true>, dim3((unsigned)np), dim3(kSymBlock), 0, st, npol,
-                                   a.sym_pairs, a.ps, a.pa, a.ipd, sym_i, sym_j);
-            else
-                hipLaunchKernelGGL(k_gpol_dd<true>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa,
-                                   a.ipd, part_p);
-            hipLaunchKernelGGL(k_gmm_pairs<true>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms,
-                               a.ma, a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
-        } else {
-            hipLaunchKernelGGL(k_gpol_static<false>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms,
-                               a.ma, a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
-            if (a.sym_pairs)
-                hipLaunchKernelGGL(k_gpol_dd_sym<false>, dim3((unsigned)np), dim3(kSymBlock), 0, st, npol,
-                                   a.sym_pairs, a.ps, a.pa, a.ipd, sym_i, sym_j);
-            else
-                hipLaunchKernelGGL(k_gpol_dd<false>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps,
-                                   a.pa, a.ipd, part_p);
-            hipLaunchKernelGGL(k_gmm_pairs<false>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms,
-                               a.ma, a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
-        }
-        if (a.sym_pairs)
-            hipLaunchKernelGGL(k_sym3_reduce, dim3(nb), dim3(4 * kSymBlock), 0, st, npol, nb, a.sympart, part_p);
-        HIP_CHECK(hipGetLastError());
-    }
-    if (eve) HIP_CHECK(hipEventRecord(eve, st));
-    launch_geomgrad_finish(st, a, a.split_s + (a.sym_pairs ? 1 : a.split_p), a.split_m);
+// From this size each (polarizable, MM) pair of the static part is evaluated
+// once for both ends (k_gstat_both), while its partial sums (one per block of
+// the other end) stay under 4 GiB; below it the two per-target sweeps with
+// source splits fill the chip better.
+bool grad_both_ends(int npol, int nmm, bool amoeba) {
+    if (npol < 5000) return false;
+    const size_t pb = (npol + kBlock - 1) / kBlock, mb = (nmm + kBlock - 1) / kBlock;
+    return sizeof(double) * (mb * 3 * (size_t)npol + pb * (amoeba ? 12 : 3) * (size_t)nmm) <= (size_t(4) << 30);
 }
 
-void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm) {
+template <bool AMOEBA>
+static void geomgrad_sweeps(hipStream_t st, const GradArgs &a, int &nparts_pol, int &nparts_mm) {
     const int nmm = a.nmm, npol = a.npol;
     const int pb = (npol + kBlock - 1) / kBlock, mb = (nmm + kBlock - 1) / kBlock;
+    const int cs = (nmm + a.split_s - 1) / a.split_s, cp = (npol + a.split_p - 1) / a.split_p;
+    const int cm = (npol + a.split_m - 1) / a.split_m;
+    // static part: both ends at once (mb polarizable-end partials, pb MM-end
+    // ones) or per target (split_s and split_m partials)
+    const int nstat = a.fused ? mb : a.split_s;
+    if (a.fused) {
+        hipLaunchKernelGGL(k_gstat_both<AMOEBA>, dim3(mb, pb), dim3(kBlock), 0, st, nmm, npol, a.ms, a.ma, a.mq,
+                           a.ps, a.pa, a.mm_polar, a.ipd, a.part_pol, a.part_mm);
+        nparts_mm = pb;
+    } else {
+        hipLaunchKernelGGL(k_gpol_static<AMOEBA>, dim3(pb, a.split_s), dim3(kBlock), 0, st, nmm, npol, cs, a.ms, a.ma,
+                           a.mq, a.ps, a.pa, a.polar_mm, a.ipd, a.part_pol);
+        hipLaunchKernelGGL(k_gmm_pairs<AMOEBA>, dim3(mb, a.split_m), dim3(kBlock), 0, st, nmm, npol, cm, a.ms, a.ma,
+                           a.mq, a.ps, a.pa, a.mm_polar, a.ipd, a.part_mm);
+        nparts_mm = a.split_m;
+    }
+    // the dipole-dipole part pair-once into one partial after the static ones,
+    // or per target
+    double *part_p = a.part_pol + (long)nstat * 3 * npol;
+    if (a.sym_pairs) {
+        const int nb = tmv_sym_blocks(npol);
+        const long np = (long)nb * (nb + 1) / 2;
+        double *sym_i = a.sympart, *sym_j = a.sympart + np * 3 * kSymBlock;
+        hipLaunchKernelGGL(k_gpol_dd_sym<AMOEBA>, dim3((unsigned)np), dim3(kSymBlock), 0, st, npol, a.sym_pairs,
+                           a.ps, a.pa, a.ipd, sym_i, sym_j);
+        hipLaunchKernelGGL(k_sym3_reduce, dim3(nb), dim3(4 * kSymBlock), 0, st, npol, nb, a.sympart, part_p);
+        nparts_pol = nstat + 1;
+    } else {
+        hipLaunchKernelGGL(k_gpol_dd<AMOEBA>, dim3(pb, a.split_p), dim3(kBlock), 0, st, npol, cp, a.ps, a.pa, a.ipd,
+                           part_p);
+        nparts_pol = nstat + a.split_p;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_geomgrad(hipStream_t st, const GradArgs &a, hipEvent_t evb, hipEvent_t eve) {
+    int nparts_pol = 0, nparts_mm = 0;
+    if (evb) HIP_CHECK(hipEventRecord(evb, st));
+    if (a.npol > 0) {
+        if (a.amoeba) geomgrad_sweeps<true>(st, a, nparts_pol, nparts_mm);
+        else geomgrad_sweeps<false>(st, a, nparts_pol, nparts_mm);
+    }
+    if (eve) HIP_CHECK(hipEventRecord(eve, st));
+    launch_geomgrad_finish(st, a, nparts_pol, nparts_mm, a.fused);
+}
+
+void launch_geomgrad_finish(hipStream_t st, const GradArgs &a, int nparts_pol, int nparts_mm, bool direct) {
+    const int nmm = a.nmm, npol = a.npol;
+    const int mb = (nmm + kBlock - 1) / kBlock;
+    const int pbf = (npol + kGfSites - 1) / kGfSites, mbf = (nmm + kGfSites - 1) / kGfSites;
     if (npol > 0) {
         if (a.amoeba) {
-            hipLaunchKernelGGL(k_gpol_finalize<true>, dim3(pb), dim3(kBlock), 0, st, npol, nparts_pol, a.part_pol,
+            hipLaunchKernelGGL(k_gpol_finalize<true>, dim3(pbf), dim3(kBlock), 0, st, npol, nparts_pol, a.part_pol,
                                a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci,
                                a.pp.f, a.gpol);
-            hipLaunchKernelGGL(k_gmm_finalize<true>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, nparts_mm, a.part_mm,
-                               a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci, a.mp.f2, a.emm, a.egrd,
-                               a.gtr);
+            hipLaunchKernelGGL(k_gmm_finalize<true>, dim3(mbf), dim3(kBlock), 0, st, nmm, npol, nparts_mm,
+                               (int)direct, a.part_mm, a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci,
+                               a.mp.f2, a.emm, a.egrd, a.gtr);
         } else {
-            hipLaunchKernelGGL(k_gpol_finalize<false>, dim3(pb), dim3(kBlock), 0, st, npol, nparts_pol, a.part_pol,
+            hipLaunchKernelGGL(k_gpol_finalize<false>, dim3(pbf), dim3(kBlock), 0, st, npol, nparts_pol, a.part_pol,
                                a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.sp.ri, a.sp.ci, a.sp.f2, a.pp.ri, a.pp.ci,
                                a.pp.f, a.gpol);
-            hipLaunchKernelGGL(k_gmm_finalize<false>, dim3(mb), dim3(kBlock), 0, st, nmm, npol, nparts_mm,
-                               a.part_mm, a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci, a.mp.f2, a.emm,
-                               a.egrd, a.gtr);
+            hipLaunchKernelGGL(k_gmm_finalize<false>, dim3(mbf), dim3(kBlock), 0, st, nmm, npol, nparts_mm,
+                               (int)direct, a.part_mm, a.ms, a.ma, a.mq, a.ps, a.pa, a.ipd, a.mp.ri, a.mp.ci,
+                               a.mp.f2, a.emm, a.egrd, a.gtr);
         }
         HIP_CHECK(hipGetLastError());
     }

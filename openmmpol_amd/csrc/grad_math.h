@@ -94,16 +94,14 @@ __device__ __forceinline__ void dip_grad_pair(double dx, double dy, double dz, d
     gz += cd * dz + cw * wz + cm * mz;
 }
 
-// Field, field gradient and Q : EHes at the target of a dipole nu, e = r_target - r_source:
+// Field and (AMOEBA) field gradient at the target of a dipole nu,
+// e = r_target - r_source:
 //   E    = 3 k3 (nu.e) e - k2 nu
 //   Egrd = 15 k4 (nu.e) e e - 3 k3 (nu e + e nu + (nu.e) I)
-//   Q:EHes = 105 k5 (nu.e)(eQe) e - 15 k4 [nu (eQe) + 2 e (nu.Qe) + 2 (nu.e) Qe + (nu.e) tr(Q) e]
-//            + 3 k3 [nu tr(Q) + 2 Q nu]
-// acc: E (0-2), Egrd xx xy yy xz yz zz (3-8), Q:EHes (9-11).
+// acc: E (0-2), Egrd xx xy yy xz yz zz (3-8).  Returns nu.e.
 template <bool AMOEBA>
-__device__ __forceinline__ void dip_at_mm(double ex, double ey, double ez, double k2, double k3, double k4, double k5,
-                                          double nx, double ny, double nz, const double *Q, double trQ,
-                                          double *acc) {
+__device__ __forceinline__ double dip_field_at(double ex, double ey, double ez, double k2, double k3, double k4,
+                                               double nx, double ny, double nz, double *acc) {
     const double ne = nx * ex + ny * ey + nz * ez;
     const double c3 = 3.0 * k3 * ne;
     acc[0] += c3 * ex - k2 * nx;
@@ -117,6 +115,21 @@ __device__ __forceinline__ void dip_at_mm(double ex, double ey, double ez, doubl
         acc[6] += c4 * ex * ez - t3 * (nx * ez + nz * ex);
         acc[7] += c4 * ey * ez - t3 * (ny * ez + nz * ey);
         acc[8] += c4 * ez * ez - t3 * (2.0 * nz * ez + ne);
+    }
+    return ne;
+}
+
+// Field, field gradient and Q : EHes at the target of a dipole nu, e = r_target - r_source:
+//   E, Egrd as dip_field_at
+//   Q:EHes = 105 k5 (nu.e)(eQe) e - 15 k4 [nu (eQe) + 2 e (nu.Qe) + 2 (nu.e) Qe + (nu.e) tr(Q) e]
+//            + 3 k3 [nu tr(Q) + 2 Q nu]
+// acc: E (0-2), Egrd xx xy yy xz yz zz (3-8), Q:EHes (9-11).
+template <bool AMOEBA>
+__device__ __forceinline__ void dip_at_mm(double ex, double ey, double ez, double k2, double k3, double k4, double k5,
+                                          double nx, double ny, double nz, const double *Q, double trQ,
+                                          double *acc) {
+    const double ne = dip_field_at<AMOEBA>(ex, ey, ez, k2, k3, k4, nx, ny, nz, acc);
+    if (AMOEBA) {
         // Q = xx, xy, yy, xz, yz, zz
         const double qx = Q[0] * ex + Q[1] * ey + Q[3] * ez;
         const double qy = Q[1] * ex + Q[2] * ey + Q[4] * ez;
