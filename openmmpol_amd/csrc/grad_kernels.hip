@@ -2,8 +2,10 @@
 // mod_geomgrad.F90:113-273, and rotation_geomgrad, rotate_multipoles.F90:3-230),
 // FP64 throughout.
 //
-// Two pair sweeps, one thread per target, sources staged through LDS in tiles
-// of 256 and split over gridDim.y:
+// Below 5,000 polarizable sites, two pair sweeps, one thread per target, sources
+// staged through LDS in tiles of 256 and split over gridDim.y (from 5,000 the
+// static part of both runs once per pair, k_gstat_both, and the dipole-dipole
+// part once per unordered pair, k_gpol_dd_sym):
 //   polarizable target j   sum_b w_b Egrd_ab of the static multipoles, with
 //                          w = 1/2 (mu_D + mu_P) (AMOEBA) or mu (AMBER), and
 //                          1/2 [mu_D,j . Egrd(mu_P) + mu_P,j . Egrd(mu_D)] of
