@@ -501,7 +501,8 @@ __global__ __launch_bounds__(kBlock) void k_bpairs_d64(const BSeg *__restrict__ 
     const long n = 3L * npol;
     const double4 *__restrict__ ps = S.ps64;
     const double *__restrict__ pa = S.pa64;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = lane >> 4, l16 = lane & 15;
+    const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, l16 = lane & 15;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
     __shared__ __align__(16) double As[3 * kMfmaSites][kD64Ld];
     __shared__ double4 s_tp[kMfmaSites], s_sp[kD64Src];
     __shared__ double s_ta[kMfmaSites], s_sa[kD64Src];
