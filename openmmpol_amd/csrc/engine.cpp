@@ -1047,7 +1047,7 @@ MixedResult mixed_solve(System &s, DeviceState &d, int M, const double *b, doubl
 // the true one; asking for that much (instead of stopping at the tolerance)
 // makes one refinement step enough for most warm-started solves.
 double refine_eta() { return 1e-6; }
-double refine_frac() { return 0.5; }
+double refine_frac() { return 0.8; }
 
 // ===================================================================== API
 void device_release(System &s) {
