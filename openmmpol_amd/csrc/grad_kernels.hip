@@ -563,13 +563,14 @@ __global__ __launch_bounds__(kBlock) void k_gstat_both(int nmm, int npol, const 
             double k2, k3, k4, k5;
             damp_k5<AMOEBA>(r2, ir, pi.w, pm.w, s_a[js], am, k2, k3, k4, k5);
             const double nx = s_n[0][js], ny = s_n[1][js], nz = s_n[2][js];
+            // field (and gradient) of nu at the MM end; nu.e is shared with the
             // gradient at the polarizable end, d = r_i - r_m = -e
+            const double ne = dip_field_at<AMOEBA>(ex, ey, ez, k2, k3, k4, nx, ny, nz, acc);
             double tx = 0.0, ty = 0.0, tz = 0.0;
-            static_grad_pair<AMOEBA>(-ex, -ey, -ez, k2, k3, k4, k5, q, nx, ny, nz, tx, ty, tz);
+            static_grad_pair_wd<AMOEBA>(-ex, -ey, -ez, k2, k3, k4, k5, q, nx, ny, nz, -ne, tx, ty, tz);
             jg[0] += tx;
             jg[1] += ty;
             jg[2] += tz;
-            dip_field_at<AMOEBA>(ex, ey, ez, k2, k3, k4, nx, ny, nz, acc);
             if (AMOEBA) {
                 acc[9] -= tx;
                 acc[10] -= ty;

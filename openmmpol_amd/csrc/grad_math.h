@@ -46,11 +46,11 @@ __device__ __forceinline__ void damp_k5(double r2, double ir, double ti, double 
 
 // g += sum_b w_b Egrd_ab of a static multipole q (charge; AMOEBA: + dipole q[1..3],
 // quadrupole / 3 in q[4..9] as xx, xy, yy, xz, yz, zz).
+// static_grad_pair with w.d given (wd).
 template <bool AMOEBA>
-__device__ __forceinline__ void static_grad_pair(double dx, double dy, double dz, double k2, double k3, double k4,
-                                                 double k5, const double *q, double wx, double wy, double wz,
-                                                 double &gx, double &gy, double &gz) {
-    const double wd = wx * dx + wy * dy + wz * dz;
+__device__ __forceinline__ void static_grad_pair_wd(double dx, double dy, double dz, double k2, double k3, double k4,
+                                                    double k5, const double *q, double wx, double wy, double wz,
+                                                    double wd, double &gx, double &gy, double &gz) {
     if (AMOEBA) {
         const double mx = q[1], my = q[2], mz = q[3];
         const double md = mx * dx + my * dy + mz * dz;
@@ -78,6 +78,13 @@ __device__ __forceinline__ void static_grad_pair(double dx, double dy, double dz
         gy += c * dy + cw * wy;
         gz += c * dz + cw * wz;
     }
+}
+
+template <bool AMOEBA>
+__device__ __forceinline__ void static_grad_pair(double dx, double dy, double dz, double k2, double k3, double k4,
+                                                 double k5, const double *q, double wx, double wy, double wz,
+                                                 double &gx, double &gy, double &gz) {
+    static_grad_pair_wd<AMOEBA>(dx, dy, dz, k2, k3, k4, k5, q, wx, wy, wz, wx * dx + wy * dy + wz * dz, gx, gy, gz);
 }
 
 // g += f * sum_b w_b Egrd_ab(mu) = f * [15 k4 (mu.d)(w.d) d - 3 k3 (mu (w.d) + w (mu.d) + (mu.w) d)]
