@@ -95,7 +95,7 @@ def load_golden():
         return json.load(f)
 
 
-def run_mode(spec, mode, tol, steps, warmup, d_zero_ptr):
+def run_mode(spec, mode, tol, steps, warmup, d_zero_ptr, host_steps=0):
     from openmmpol_amd import synth
     s = synth.make_system(spec)
     s.set_precision(mode)
@@ -127,9 +127,24 @@ def run_mode(spec, mode, tol, steps, warmup, d_zero_ptr):
     stats = s.kernel_stats(reset=True)
     s.profile_kernels(False)
     ipd = s.ipd.copy()
+    # the same step through the drop-in host-pointer entry (ommp_set_external_field:
+    # the field copied in from host memory, the dipoles back after the solve)
+    host = None
+    if host_steps:
+        ext = np.zeros((s.pol_atoms, 3))
+        s.reset_guess(True)
+        s.set_external_field(ext)
+        s.get_polelec_energy()
+        t0 = time.perf_counter()
+        for _ in range(host_steps):
+            s.reset_guess(True)
+            s.set_external_field(ext)
+            s.get_polelec_energy()
+            _ = s.ipd[0, 0, 0]
+        host = (time.perf_counter() - t0) / host_steps
     s.terminate()
     return dict(elapsed=elapsed, lat=np.array(lat), epol=e, ipd=ipd, iters=iters[-1], stats=stats,
-                status=status)
+                status=status, host=host)
 
 
 class DeviceBuffers:
@@ -613,7 +628,7 @@ def main():
 
     mixed = None
     if "headline" in legs:
-        mixed = run_mode(spec, "mixed", args.tol, args.steps, args.warmup, d_zero.value)
+        mixed = run_mode(spec, "mixed", args.tol, args.steps, args.warmup, d_zero.value, host_steps=args.steps)
         print(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} solves/s", file=sys.stderr, flush=True)
     fp64 = None
     if "fp64" in legs:
@@ -713,6 +728,9 @@ def main():
                    "n_sites": n, "tolerance": args.tol, "external_field": "zero", "parallelism": "single GPU"},
         "p50_ms": float(np.percentile(mixed["lat"], 50)),
         "p99_ms": float(np.percentile(mixed["lat"], 99)),
+        "host_entry_solves_per_s": 1.0 / mixed["host"] if mixed.get("host") else None,
+        "host_entry_note": "the same step through the drop-in host-pointer entry (field copied in from "
+                           "pageable host memory, dipoles copied back); not the value",
         "iterations_mixed": {"outer_fp64": mixed["iters"][2], "inner_fp32_total": mixed["iters"][3]},
         "epol_mixed": mixed["epol"],
         "epol_oracle_golden": ref_e,
