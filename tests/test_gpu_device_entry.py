@@ -3,8 +3,9 @@ benchmark's steps run through them), give the results of the host-pointer
 entries: ommp_amd_set_external_field_device against ommp_set_external_field,
 and ommp_amd_set_external_fields_multi_device (dipoles left in device memory)
 against ommp_amd_set_external_fields_multi, on a 3,000-site sphere in both
-precisions.  Bar: the same solver on the same data, so 1e-12 of the largest
-dipole component (the single-field solves repeat bit for bit)."""
+precisions; ommp_amd_device_ipd points at the dipoles ommp_get_ipd returns.
+Bar: the same solver on the same data, so 1e-12 of the largest dipole
+component (the single-field solves repeat bit for bit)."""
 import ctypes
 
 import numpy as np
@@ -66,9 +67,12 @@ def test_single_field_device_entry_matches_host_entry(mode):
         s.reset_guess(True)
         s.set_external_field_device(dev.upload(ext))
         e_dev = s.get_polelec_energy()
+        # the dipoles read where the solver left them in device memory
+        ipd_devmem = dev.download(s.device_ipd_ptr(), (s.n_ipd, s.pol_atoms, 3))
         ipd_dev = np.array(s.ipd)
     finally:
         dev.free()
+    assert np.array_equal(ipd_devmem, ipd_dev)
     diff = np.abs(ipd_dev - ipd_host).max() / np.abs(ipd_host).max()
     print(f"{mode}: dipoles {diff:.2e} of the largest, bitwise {np.array_equal(ipd_dev, ipd_host)}, "
           f"E_pol {e_host:.12e} / {e_dev:.12e}", flush=True)
